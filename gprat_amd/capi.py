@@ -1,0 +1,278 @@
+"""ctypes binding of the C ABI in include/gprat_amd.h.
+
+This is how a non-C++ host would bind the backend (the reference's pybind
+module is the other route). Every call raises GpratError on a non-zero
+status, carrying the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+LIBRARY_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgprat_amd.so")
+
+OK, ERR_ARGUMENT, ERR_HIP, ERR_OUT_OF_MEMORY, ERR_NOT_SPD, ERR_NO_DEVICE, ERR_INTERNAL = range(7)
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+_VP = C.c_void_p
+_SZ = C.c_size_t
+
+# name -> argtypes (restype int unless listed in _RESTYPE)
+SIGNATURES = {
+    "gprat_last_error": [],
+    "gprat_version": [],
+    "gprat_device_count": [_I],
+    "gprat_device_describe": [C.c_int, C.c_char_p, _SZ],
+    "gprat_context_create": [C.c_int, C.c_int, C.POINTER(_VP)],
+    "gprat_context_destroy": [_VP],
+    "gprat_context_synchronize": [_VP],
+    "gprat_device_alloc": [_VP, _SZ, C.POINTER(_VP)],
+    "gprat_device_free": [_VP, _VP],
+    "gprat_memcpy": [_VP, _VP, _VP, _SZ],
+    "gprat_cholesky": [_VP, _D, _SZ, _D, C.c_int, C.c_int, C.c_int, _D],
+    "gprat_predict": [_VP, _D, _SZ, _D, _SZ, _D, _SZ, _D, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _D],
+    "gprat_predict_with_uncertainty": [_VP, _D, _SZ, _D, _SZ, _D, _SZ, _D, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.c_int, _D, _D],
+    "gprat_predict_with_full_cov": [_VP, _D, _SZ, _D, _SZ, _D, _SZ, _D, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_int, _D, _D],
+    "gprat_compute_loss": [_VP, _D, _SZ, _D, _SZ, _D, C.c_int, C.c_int, C.c_int, _D],
+    "gprat_loss_gradient": [_VP, _D, _SZ, _D, _SZ, _D, C.c_int, C.c_int, C.c_int, _I, _D, _D],
+    "gprat_model_fit": [_VP, _D, _SZ, _D, _SZ, C.c_int, C.c_int, _D, C.c_int, C.POINTER(_VP)],
+    "gprat_model_predict": [_VP, _D, _SZ, C.c_int, _D, _D],
+    "gprat_model_loss": [_VP, _D],
+    "gprat_model_bytes": [_VP, C.POINTER(_SZ)],
+    "gprat_model_release": [_VP],
+    "gprat_context_set_profiling": [_VP, C.c_int],
+    "gprat_context_kernel_stats": [_VP, C.c_char_p, C.POINTER(C.c_long), _D],
+}
+_RESTYPE = {"gprat_last_error": C.c_char_p, "gprat_version": C.c_char_p}
+
+
+class GpratError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[status {status}] {message}")
+        self.status = status
+
+
+_lib = None
+
+
+def lib():
+    """Load libgprat_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY_PATH):
+            raise ImportError(f"{LIBRARY_PATH} is missing: build the backend first (__graft_entry__.build())")
+        l = C.CDLL(LIBRARY_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(l, name)
+            f.argtypes = args
+            f.restype = _RESTYPE.get(name, C.c_int)
+        _lib = l
+    return _lib
+
+
+def _check(status: int):
+    if status != OK:
+        raise GpratError(status, lib().gprat_last_error().decode())
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(_D)
+
+
+def _arr(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    _check(lib().gprat_device_count(C.byref(n)))
+    return n.value
+
+
+class Context:
+    """One GPU's library context (streams + caching device pool)."""
+
+    def __init__(self, device: int = 0, n_streams: int = 1):
+        h = _VP()
+        _check(lib().gprat_context_create(device, n_streams, C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().gprat_context_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- reference-shaped entry points ------------------------------------
+    def cholesky(self, train_in, theta, n_tiles, n_tile_size, n_reg):
+        x = _arr(train_in)
+        t = _arr(theta)
+        out = np.empty(n_tiles * (n_tiles + 1) // 2 * n_tile_size * n_tile_size)
+        _check(lib().gprat_cholesky(self.handle, _dptr(x), x.size, _dptr(t), n_tiles, n_tile_size, n_reg, _dptr(out)))
+        return out.reshape(-1, n_tile_size, n_tile_size)
+
+    def predict(self, train_in, train_out, test_in, theta, n_tiles, n_tile_size, m_tiles, m_tile_size, n_reg,
+                variance=False, full_cov=False):
+        x, y, z, t = _arr(train_in), _arr(train_out), _arr(test_in), _arr(theta)
+        m = m_tiles * m_tile_size
+        mean = np.empty(m)
+        var = np.empty(m)
+        args = (self.handle, _dptr(x), x.size, _dptr(y), y.size, _dptr(z), z.size, _dptr(t), n_tiles, n_tile_size,
+                m_tiles, m_tile_size, n_reg)
+        if full_cov:
+            _check(lib().gprat_predict_with_full_cov(*args, _dptr(mean), _dptr(var)))
+        elif variance:
+            _check(lib().gprat_predict_with_uncertainty(*args, _dptr(mean), _dptr(var)))
+        else:
+            _check(lib().gprat_predict(*args, _dptr(mean)))
+            return mean
+        return mean, var
+
+    def compute_loss(self, train_in, train_out, theta, n_tiles, n_tile_size, n_reg):
+        x, y, t = _arr(train_in), _arr(train_out), _arr(theta)
+        out = C.c_double(0.0)
+        _check(lib().gprat_compute_loss(self.handle, _dptr(x), x.size, _dptr(y), y.size, _dptr(t), n_tiles,
+                                        n_tile_size, n_reg, C.byref(out)))
+        return out.value
+
+    def loss_gradient(self, train_in, train_out, theta, n_tiles, n_tile_size, n_reg, trainable=(1, 1, 1)):
+        x, y, t = _arr(train_in), _arr(train_out), _arr(theta)
+        tr = (C.c_int * 3)(*[int(bool(v)) for v in trainable])
+        loss = C.c_double(0.0)
+        grad = np.zeros(3)
+        _check(lib().gprat_loss_gradient(self.handle, _dptr(x), x.size, _dptr(y), y.size, _dptr(t), n_tiles,
+                                         n_tile_size, n_reg, tr, C.byref(loss), _dptr(grad)))
+        return loss.value, grad
+
+    # -- resident models -----------------------------------------------------
+    def fit(self, train_in, train_out, n, n_reg, theta, precision=PRECISION_FP64):
+        return Model(self, train_in, train_out, n, n_reg, theta, precision)
+
+    # -- measurement -----------------------------------------------------------
+    def set_profiling(self, enable: bool):
+        _check(lib().gprat_context_set_profiling(self.handle, int(enable)))
+
+    def kernel_stats(self, family: str):
+        n = C.c_long(0)
+        ms = C.c_double(0.0)
+        _check(lib().gprat_context_kernel_stats(self.handle, family.encode(), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def synchronize(self):
+        _check(lib().gprat_context_synchronize(self.handle))
+
+    # -- device buffers ------------------------------------------------------
+    def device_array(self, host: np.ndarray) -> "DeviceArray":
+        """Copy a float64 array into a new device buffer."""
+        return DeviceArray(self, _arr(host))
+
+    def device_empty(self, count: int) -> "DeviceArray":
+        return DeviceArray(self, None, count)
+
+
+class DeviceArray:
+    """A float64 device buffer owned by a context (pointer usable in any entry point)."""
+
+    def __init__(self, ctx: Context, host, count: int = 0):
+        self.ctx = ctx
+        self.size = host.size if host is not None else count
+        p = _VP()
+        _check(lib().gprat_device_alloc(ctx.handle, max(self.size, 1) * 8, C.byref(p)))
+        self.ptr = p
+        if host is not None and self.size:
+            _check(lib().gprat_memcpy(ctx.handle, self.ptr, host.ctypes.data_as(_VP), self.size * 8))
+
+    def as_dptr(self):
+        return C.cast(self.ptr, _D)
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.size)
+        if self.size:
+            _check(lib().gprat_memcpy(self.ctx.handle, out.ctypes.data_as(_VP), self.ptr, self.size * 8))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().gprat_device_free(self.ctx.handle, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _ptr_len(a):
+    """(double*, length) for a host array or a DeviceArray."""
+    if isinstance(a, DeviceArray):
+        return a.as_dptr(), a.size, a
+    h = _arr(a)
+    return _dptr(h), h.size, h
+
+
+class Model:
+    """A fitted GP whose factor stays in HBM until release(). Inputs may be
+    host arrays or DeviceArrays."""
+
+    def __init__(self, ctx: Context, train_in, train_out, n, n_reg, theta, precision=PRECISION_FP64):
+        (xp, xl, _xk), (yp, yl, _yk) = _ptr_len(train_in), _ptr_len(train_out)
+        t = _arr(theta)
+        h = _VP()
+        _check(lib().gprat_model_fit(ctx.handle, xp, xl, yp, yl, n, n_reg, _dptr(t), precision, C.byref(h)))
+        self.ctx = ctx
+        self.handle = h
+        self.n = n
+        self.n_reg = n_reg
+
+    def predict(self, test_in, m, variance=True, out=None):
+        """Mean (and variance). `out=(mean, var)` may hold DeviceArrays to keep
+        results in HBM; then nothing is returned."""
+        zp, zl, _zk = _ptr_len(test_in)
+        if out is not None:
+            dm, dv = out
+            _check(lib().gprat_model_predict(self.handle, zp, zl, m, dm.as_dptr(),
+                                             dv.as_dptr() if (variance and dv is not None) else None))
+            return None
+        mean = np.empty(m)
+        var = np.empty(m) if variance else None
+        _check(lib().gprat_model_predict(self.handle, zp, zl, m, _dptr(mean), _dptr(var) if variance else None))
+        return (mean, var) if variance else mean
+
+    def loss(self):
+        out = C.c_double(0.0)
+        _check(lib().gprat_model_loss(self.handle, C.byref(out)))
+        return out.value
+
+    def nbytes(self):
+        b = _SZ(0)
+        _check(lib().gprat_model_bytes(self.handle, C.byref(b)))
+        return b.value
+
+    def release(self):
+        if self.handle:
+            lib().gprat_model_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass

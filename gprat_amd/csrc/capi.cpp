@@ -1,0 +1,482 @@
+// extern "C" boundary of the backend. Converts exceptions into status codes
+// plus a thread-local message, validates arguments, and serialises calls per
+// context. The reference-shaped entry points (gprat_cholesky, gprat_predict,
+// ...) fit a transient model, use it and release it back to the pool.
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+
+#include "engine.hpp"
+
+namespace
+{
+
+thread_local std::string g_last_error;
+
+template <class Fn>
+int guarded(Fn &&fn)
+{
+    try
+    {
+        fn();
+        g_last_error.clear();
+        return GPRAT_OK;
+    }
+    catch (const gpamd::Error &e)
+    {
+        g_last_error = e.what();
+        return e.status;
+    }
+    catch (const std::bad_alloc &)
+    {
+        g_last_error = "host allocation failed";
+        return GPRAT_ERR_OUT_OF_MEMORY;
+    }
+    catch (const std::exception &e)
+    {
+        g_last_error = e.what();
+        return GPRAT_ERR_INTERNAL;
+    }
+    catch (...)
+    {
+        g_last_error = "unknown error";
+        return GPRAT_ERR_INTERNAL;
+    }
+}
+
+gpamd::Context &ctx_ref(gprat_context *c)
+{
+    gpamd::require(c != nullptr, "context is null");
+    return *reinterpret_cast<gpamd::Context *>(c);
+}
+
+struct Tiling
+{
+    int n, m;
+};
+
+// n = n_tiles * n_tile_size, m = m_tiles * m_tile_size (core/src/utils_c.cpp:8-52).
+Tiling check_tiling(int n_tiles, int n_tile_size, int m_tiles, int m_tile_size, int n_reg)
+{
+    gpamd::require(n_tiles >= 1 && n_tile_size >= 1, "n_tiles and n_tile_size must be >= 1");
+    gpamd::require(m_tiles >= 0 && m_tile_size >= 0, "m_tiles and m_tile_size must be >= 0");
+    gpamd::require(n_reg >= 1, "n_reg must be >= 1");
+    const long n = static_cast<long>(n_tiles) * n_tile_size;
+    const long m = static_cast<long>(m_tiles) * m_tile_size;
+    gpamd::require(n <= (1L << 30) && m <= (1L << 30), "problem too large");
+    return { static_cast<int>(n), static_cast<int>(m) };
+}
+
+void fit_transient(gpamd::Context &c,
+                   gpamd::Model &model,
+                   const double *train_in,
+                   size_t train_in_len,
+                   const double *train_out,
+                   size_t train_out_len,
+                   const double theta[3],
+                   int n,
+                   int n_reg)
+{
+    gpamd::fit(c, model, train_in, train_in_len, train_out, train_out_len, n, n_reg, theta, GPRAT_PRECISION_FP64);
+}
+
+}  // namespace
+
+struct gprat_context
+{
+};
+struct gprat_model
+{
+};
+
+extern "C" {
+
+const char *gprat_last_error(void) { return g_last_error.c_str(); }
+
+const char *gprat_version(void) { return "gprat_amd 0.1 (gfx950)"; }
+
+int gprat_device_count(int *count)
+{
+    return guarded(
+        [&]
+        {
+            gpamd::require(count != nullptr, "count is null");
+            int n = 0;
+            hipError_t e = hipGetDeviceCount(&n);
+            if (e != hipSuccess)
+            {
+                (void)hipGetLastError();
+                n = 0;
+            }
+            *count = n;
+        });
+}
+
+int gprat_device_describe(int device, char *buf, size_t buf_len)
+{
+    return guarded(
+        [&]
+        {
+            gpamd::require(buf != nullptr && buf_len > 0, "buffer is null");
+            hipDeviceProp_t p;
+            GP_HIP(hipGetDeviceProperties(&p, device));
+            std::snprintf(buf,
+                          buf_len,
+                          "GPU %d: %s (%s), %d CUs, %.1f GiB HBM, %d MHz",
+                          device,
+                          p.name,
+                          p.gcnArchName,
+                          p.multiProcessorCount,
+                          static_cast<double>(p.totalGlobalMem) / (1024.0 * 1024.0 * 1024.0),
+                          p.clockRate / 1000);
+        });
+}
+
+int gprat_context_create(int device, int n_streams, gprat_context **out)
+{
+    return guarded(
+        [&]
+        {
+            gpamd::require(out != nullptr, "out is null");
+            gpamd::require(n_streams >= 1, "n_streams must be >= 1");
+            int n = 0;
+            if (hipGetDeviceCount(&n) != hipSuccess)
+            {
+                (void)hipGetLastError();
+                n = 0;
+            }
+            if (n == 0)
+                throw gpamd::Error(GPRAT_ERR_NO_DEVICE, "no HIP device available");
+            if (device < 0 || device >= n)
+                throw gpamd::Error(GPRAT_ERR_NO_DEVICE, "Requested GPU device is not available.");
+            auto *c = new gpamd::Context(device, n_streams);
+            *out = reinterpret_cast<gprat_context *>(c);
+        });
+}
+
+int gprat_context_destroy(gprat_context *ctx)
+{
+    return guarded([&] { delete reinterpret_cast<gpamd::Context *>(ctx); });
+}
+
+int gprat_device_alloc(gprat_context *ctx, size_t bytes, void **ptr)
+{
+    return guarded(
+        [&]
+        {
+            auto &c = ctx_ref(ctx);
+            gpamd::require(ptr != nullptr, "ptr is null");
+            std::lock_guard<std::recursive_mutex> g(c.mu);
+            c.make_current();
+            void *p = nullptr;
+            GP_HIP(hipMalloc(&p, bytes ? bytes : 1));
+            *ptr = p;
+        });
+}
+
+int gprat_device_free(gprat_context *ctx, void *ptr)
+{
+    return guarded(
+        [&]
+        {
+            auto &c = ctx_ref(ctx);
+            std::lock_guard<std::recursive_mutex> g(c.mu);
+            c.make_current();
+            c.sync();
+            GP_HIP(hipFree(ptr));
+        });
+}
+
+int gprat_memcpy(gprat_context *ctx, void *dst, const void *src, size_t bytes)
+{
+    return guarded(
+        [&]
+        {
+            auto &c = ctx_ref(ctx);
+            std::lock_guard<std::recursive_mutex> g(c.mu);
+            c.make_current();
+            GP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, c.main));
+            c.sync();
+        });
+}
+
+int gprat_context_synchronize(gprat_context *ctx)
+{
+    return guarded(
+        [&]
+        {
+            auto &c = ctx_ref(ctx);
+            std::lock_guard<std::recursive_mutex> g(c.mu);
+            c.make_current();
+            c.sync();
+        });
+}
+
+int gprat_cholesky(gprat_context *ctx,
+                   const double *train_in,
+                   size_t train_in_len,
+                   const double theta[3],
+                   int n_tiles,
+                   int n_tile_size,
+                   int n_reg,
+                   double *tiles_out)
+{
+    return guarded(
+        [&]
+        {
+            auto &c = ctx_ref(ctx);
+            const Tiling t = check_tiling(n_tiles, n_tile_size, 0, 0, n_reg);
+            gpamd::require(tiles_out != nullptr, "tiles_out is null");
+            std::lock_guard<std::recursive_mutex> g(c.mu);
+            gpamd::Model model;
+            gpamd::factor(c, model, train_in, train_in_len, t.n, n_reg, theta, GPRAT_PRECISION_FP64);
+            gpamd::cholesky_tiles(model, n_tiles, n_tile_size, tiles_out);
+        });
+}
+
+static int predict_common(gprat_context *ctx,
+                          const double *train_in,
+                          size_t train_in_len,
+                          const double *train_out,
+                          size_t train_out_len,
+                          const double *test_in,
+                          size_t test_in_len,
+                          const double theta[3],
+                          int n_tiles,
+                          int n_tile_size,
+                          int m_tiles,
+                          int m_tile_size,
+                          int n_reg,
+                          double *mean_out,
+                          double *var_out,
+                          bool want_var)
+{
+    return guarded(
+        [&]
+        {
+            auto &c = ctx_ref(ctx);
+            const Tiling t = check_tiling(n_tiles, n_tile_size, m_tiles, m_tile_size, n_reg);
+            if (t.m > 0)
+            {
+                gpamd::require(mean_out != nullptr, "mean_out is null");
+                gpamd::require(!want_var || var_out != nullptr, "var_out is null");
+            }
+            std::lock_guard<std::recursive_mutex> g(c.mu);
+            gpamd::Model model;
+            fit_transient(c, model, train_in, train_in_len, train_out, train_out_len, theta, t.n, n_reg);
+            gpamd::predict(model, test_in, test_in_len, t.m, mean_out, want_var ? var_out : nullptr);
+        });
+}
+
+int gprat_predict(gprat_context *ctx,
+                  const double *train_in,
+                  size_t train_in_len,
+                  const double *train_out,
+                  size_t train_out_len,
+                  const double *test_in,
+                  size_t test_in_len,
+                  const double theta[3],
+                  int n_tiles,
+                  int n_tile_size,
+                  int m_tiles,
+                  int m_tile_size,
+                  int n_reg,
+                  double *mean_out)
+{
+    return predict_common(ctx, train_in, train_in_len, train_out, train_out_len, test_in, test_in_len, theta, n_tiles,
+                          n_tile_size, m_tiles, m_tile_size, n_reg, mean_out, nullptr, false);
+}
+
+int gprat_predict_with_uncertainty(gprat_context *ctx,
+                                   const double *train_in,
+                                   size_t train_in_len,
+                                   const double *train_out,
+                                   size_t train_out_len,
+                                   const double *test_in,
+                                   size_t test_in_len,
+                                   const double theta[3],
+                                   int n_tiles,
+                                   int n_tile_size,
+                                   int m_tiles,
+                                   int m_tile_size,
+                                   int n_reg,
+                                   double *mean_out,
+                                   double *var_out)
+{
+    return predict_common(ctx, train_in, train_in_len, train_out, train_out_len, test_in, test_in_len, theta, n_tiles,
+                          n_tile_size, m_tiles, m_tile_size, n_reg, mean_out, var_out, true);
+}
+
+int gprat_predict_with_full_cov(gprat_context *ctx,
+                                const double *train_in,
+                                size_t train_in_len,
+                                const double *train_out,
+                                size_t train_out_len,
+                                const double *test_in,
+                                size_t test_in_len,
+                                const double theta[3],
+                                int n_tiles,
+                                int n_tile_size,
+                                int m_tiles,
+                                int m_tile_size,
+                                int n_reg,
+                                double *mean_out,
+                                double *var_out)
+{
+    // Only diag(Sigma) leaves the reference's full-covariance path
+    // (core/src/cpu/gp_functions.cpp:674-695); it equals the latent variance.
+    return predict_common(ctx, train_in, train_in_len, train_out, train_out_len, test_in, test_in_len, theta, n_tiles,
+                          n_tile_size, m_tiles, m_tile_size, n_reg, mean_out, var_out, true);
+}
+
+int gprat_compute_loss(gprat_context *ctx,
+                       const double *train_in,
+                       size_t train_in_len,
+                       const double *train_out,
+                       size_t train_out_len,
+                       const double theta[3],
+                       int n_tiles,
+                       int n_tile_size,
+                       int n_reg,
+                       double *loss_out)
+{
+    return guarded(
+        [&]
+        {
+            auto &c = ctx_ref(ctx);
+            const Tiling t = check_tiling(n_tiles, n_tile_size, 0, 0, n_reg);
+            gpamd::require(loss_out != nullptr, "loss_out is null");
+            std::lock_guard<std::recursive_mutex> g(c.mu);
+            gpamd::Model model;
+            fit_transient(c, model, train_in, train_in_len, train_out, train_out_len, theta, t.n, n_reg);
+            *loss_out = gpamd::loss(model);
+        });
+}
+
+int gprat_loss_gradient(gprat_context *ctx,
+                        const double *train_in,
+                        size_t train_in_len,
+                        const double *train_out,
+                        size_t train_out_len,
+                        const double theta[3],
+                        int n_tiles,
+                        int n_tile_size,
+                        int n_reg,
+                        const int trainable[3],
+                        double *loss_out,
+                        double grad_out[3])
+{
+    return guarded(
+        [&]
+        {
+            auto &c = ctx_ref(ctx);
+            const Tiling t = check_tiling(n_tiles, n_tile_size, 0, 0, n_reg);
+            gpamd::require(loss_out != nullptr && grad_out != nullptr && trainable != nullptr, "null output");
+            std::lock_guard<std::recursive_mutex> g(c.mu);
+            gpamd::loss_gradient(c, train_in, train_in_len, train_out, train_out_len, theta, t.n, n_reg, trainable,
+                                 loss_out, grad_out);
+        });
+}
+
+int gprat_model_fit(gprat_context *ctx,
+                    const double *train_in,
+                    size_t train_in_len,
+                    const double *train_out,
+                    size_t train_out_len,
+                    int n,
+                    int n_reg,
+                    const double theta[3],
+                    int precision,
+                    gprat_model **out)
+{
+    return guarded(
+        [&]
+        {
+            auto &c = ctx_ref(ctx);
+            gpamd::require(out != nullptr, "out is null");
+            std::lock_guard<std::recursive_mutex> g(c.mu);
+            auto m = std::make_unique<gpamd::Model>();
+            gpamd::fit(c, *m, train_in, train_in_len, train_out, train_out_len, n, n_reg, theta, precision);
+            *out = reinterpret_cast<gprat_model *>(m.release());
+        });
+}
+
+int gprat_model_predict(gprat_model *model, const double *test_in, size_t test_in_len, int m, double *mean_out, double *var_out)
+{
+    return guarded(
+        [&]
+        {
+            gpamd::require(model != nullptr, "model is null");
+            auto *md = reinterpret_cast<gpamd::Model *>(model);
+            std::lock_guard<std::recursive_mutex> g(md->ctx->mu);
+            gpamd::predict(*md, test_in, test_in_len, m, mean_out, var_out);
+        });
+}
+
+int gprat_model_loss(gprat_model *model, double *loss_out)
+{
+    return guarded(
+        [&]
+        {
+            gpamd::require(model != nullptr && loss_out != nullptr, "null argument");
+            auto *md = reinterpret_cast<gpamd::Model *>(model);
+            std::lock_guard<std::recursive_mutex> g(md->ctx->mu);
+            *loss_out = gpamd::loss(*md);
+        });
+}
+
+int gprat_model_bytes(const gprat_model *model, size_t *bytes)
+{
+    return guarded(
+        [&]
+        {
+            gpamd::require(model != nullptr && bytes != nullptr, "null argument");
+            *bytes = reinterpret_cast<const gpamd::Model *>(model)->bytes();
+        });
+}
+
+int gprat_model_release(gprat_model *model)
+{
+    return guarded(
+        [&]
+        {
+            auto *md = reinterpret_cast<gpamd::Model *>(model);
+            if (!md)
+                return;
+            std::lock_guard<std::recursive_mutex> g(md->ctx->mu);
+            delete md;
+        });
+}
+
+int gprat_context_set_profiling(gprat_context *ctx, int enable)
+{
+    return guarded(
+        [&]
+        {
+            auto &c = ctx_ref(ctx);
+            std::lock_guard<std::recursive_mutex> g(c.mu);
+            c.make_current();
+            c.sync();
+            c.profiling = enable != 0;
+            c.stats.clear();
+        });
+}
+
+int gprat_context_kernel_stats(gprat_context *ctx, const char *family, long *launches, double *total_ms)
+{
+    return guarded(
+        [&]
+        {
+            auto &c = ctx_ref(ctx);
+            gpamd::require(family != nullptr && launches != nullptr && total_ms != nullptr, "null argument");
+            std::lock_guard<std::recursive_mutex> g(c.mu);
+            c.make_current();
+            c.sync();
+            auto it = c.stats.find(family);
+            *launches = it == c.stats.end() ? 0 : it->second.launches;
+            *total_ms = it == c.stats.end() ? 0.0 : it->second.ms;
+        });
+}
+
+}  // extern "C"

@@ -1,0 +1,644 @@
+// Host orchestration of the gfx950 GP backend.
+//
+// Data layout in HBM: the training covariance is one dense row-major
+// n_pad x lda matrix (n_pad = n rounded up to 128) whose lower triangle is
+// overwritten by the Cholesky factor; rows/columns >= n hold an identity
+// block. Vectors are zero padded to n_pad. The tile split of the reference
+// API (n_tiles x n_tile_size) only shapes inputs and the cholesky() output;
+// the factorisation uses its own blocking.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <numbers>
+
+namespace gpamd
+{
+
+[[noreturn]] void throw_hip(hipError_t e, const char *expr, const char *file, int line)
+{
+    const int status = (e == hipErrorOutOfMemory) ? GPRAT_ERR_OUT_OF_MEMORY : GPRAT_ERR_HIP;
+    throw Error(status,
+                std::string("HIP error ") + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ") in " + expr + " at "
+                    + file + ":" + std::to_string(line));
+}
+
+// ---------------------------------------------------------------------------
+// DevicePool
+
+DevicePool::~DevicePool()
+{
+    hipSetDevice(device_);
+    for (auto &kv : free_)
+        hipFree(kv.second);
+    for (auto &kv : live_)
+        hipFree(kv.first);
+}
+
+static std::size_t size_class(std::size_t bytes)
+{
+    const std::size_t g = bytes < (std::size_t(1) << 20) ? 4096 : (std::size_t(2) << 20);
+    return ((bytes + g - 1) / g) * g;
+}
+
+void *DevicePool::get(std::size_t bytes)
+{
+    const std::size_t want = size_class(bytes ? bytes : 1);
+    auto it = free_.lower_bound(want);
+    // reuse a cached block if it is not more than 25% larger than needed
+    if (it != free_.end() && it->first <= want + want / 4)
+    {
+        void *p = it->second;
+        live_[p] = it->first;
+        cached_ -= it->first;
+        free_.erase(it);
+        return p;
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess)
+    {
+        (void)hipGetLastError();
+        release_cached();
+        e = hipMalloc(&p, want);
+        if (e != hipSuccess)
+        {
+            (void)hipGetLastError();
+            throw Error(GPRAT_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(want) + " bytes failed");
+        }
+    }
+    live_[p] = want;
+    return p;
+}
+
+void DevicePool::put(void *p)
+{
+    if (!p)
+        return;
+    auto it = live_.find(p);
+    if (it == live_.end())
+        return;
+    free_.emplace(it->second, p);
+    cached_ += it->second;
+    live_.erase(it);
+}
+
+void DevicePool::release_cached()
+{
+    hipDeviceSynchronize();
+    for (auto &kv : free_)
+        hipFree(kv.second);
+    free_.clear();
+    cached_ = 0;
+}
+
+template <class T>
+DevBuf<T>::DevBuf(Context &c, std::size_t count) : pool_(&c.pool), n_(count)
+{
+    p_ = static_cast<T *>(pool_->get(count * sizeof(T)));
+}
+
+template <class T>
+void DevBuf<T>::reset()
+{
+    if (p_ && pool_)
+        pool_->put(p_);
+    p_ = nullptr;
+    n_ = 0;
+}
+
+template class DevBuf<double>;
+template class DevBuf<float>;
+template class DevBuf<int>;
+
+// ---------------------------------------------------------------------------
+// Context
+
+Context::Context(int dev, int ns) : device(dev), n_streams(ns), pool(dev)
+{
+    make_current();
+    GP_HIP(hipStreamCreateWithFlags(&main, hipStreamNonBlocking));
+    GP_HIP(hipStreamCreateWithFlags(&panel, hipStreamNonBlocking));
+    info = DevBuf<int>(*this, 1);
+    scalars = DevBuf<double>(*this, 64);
+}
+
+Context::~Context()
+{
+    hipSetDevice(device);
+    hipDeviceSynchronize();
+    for (auto &p : pending)
+    {
+        hipEventDestroy(p.begin);
+        hipEventDestroy(p.end);
+    }
+    for (auto e : spare_events)
+        hipEventDestroy(e);
+    info.reset();
+    scalars.reset();
+    if (main)
+        hipStreamDestroy(main);
+    if (panel)
+        hipStreamDestroy(panel);
+}
+
+void Context::make_current() const { GP_HIP(hipSetDevice(device)); }
+
+hipEvent_t Context::event()
+{
+    if (!spare_events.empty())
+    {
+        hipEvent_t e = spare_events.back();
+        spare_events.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    GP_HIP(hipEventCreate(&e));
+    return e;
+}
+
+void Context::recycle(hipEvent_t e) { spare_events.push_back(e); }
+
+void Context::prof_begin(hipStream_t s, const char *family, Pending &p)
+{
+    p.family = family;
+    p.begin = event();
+    p.end = event();
+    GP_HIP(hipEventRecord(p.begin, s));
+}
+
+void Context::prof_end(hipStream_t s, Pending &p)
+{
+    if (hipEventRecord(p.end, s) == hipSuccess)
+        pending.push_back(p);
+    else
+    {
+        (void)hipGetLastError();
+        recycle(p.begin);
+        recycle(p.end);
+    }
+}
+
+void Context::resolve_profiling()
+{
+    for (auto &p : pending)
+    {
+        float ms = 0.0f;
+        GP_HIP(hipEventElapsedTime(&ms, p.begin, p.end));
+        KernelStat &st = stats[p.family];
+        st.launches += 1;
+        st.ms += ms;
+        recycle(p.begin);
+        recycle(p.end);
+    }
+    pending.clear();
+}
+
+void Context::sync()
+{
+    GP_HIP(hipStreamSynchronize(panel));
+    GP_HIP(hipStreamSynchronize(main));
+    if (!pending.empty())
+        resolve_profiling();
+}
+
+// Optional timing bracket around a launch sequence.
+class Timed
+{
+  public:
+    Timed(Context &c, hipStream_t s, const char *family) : c_(c), s_(s), on_(c.profiling)
+    {
+        if (on_)
+            c_.prof_begin(s_, family, p_);
+    }
+    ~Timed()
+    {
+        if (on_)
+            c_.prof_end(s_, p_);
+    }
+
+  private:
+    Context &c_;
+    hipStream_t s_;
+    bool on_;
+    Context::Pending p_;
+};
+
+// ---------------------------------------------------------------------------
+// Layout
+
+long padded(long n) { return round_up(std::max<long>(n, 1), 128); }
+
+long leading_dim(long n_pad)
+{
+    // Large power-of-two strides map consecutive rows to the same HBM
+    // channels; offset them by half a KiB.
+    if (n_pad >= 4096 && n_pad % 1024 == 0)
+        return n_pad + 64;
+    return n_pad;
+}
+
+static int outer_block(int n_pad)
+{
+    if (const char *e = std::getenv("GPRAT_CHOLESKY_BLOCK"))
+    {
+        int v = std::atoi(e);
+        if (v >= 128 && v % 128 == 0)
+            return v;
+    }
+    return n_pad <= 4096 ? 256 : 512;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 Cholesky: blocked right-looking with one panel of look-ahead.
+//
+// Outer panels of width Bo are factored on the `panel` stream in 64-wide
+// steps (fused POTRF+TRSM, then an MFMA update of the rest of the panel).
+// On the `main` stream the trailing update of step k first refreshes panel
+// k+1 only, releases it to the panel stream, and then updates the remaining
+// lower trailing matrix while panel k+1 is being factored.
+void cholesky_fp64(Context &c, double *A, long lda, int n_pad)
+{
+    require(n_pad % 128 == 0 && lda >= n_pad && lda % 2 == 0, "cholesky: bad layout");
+    hipStream_t U = c.main, P = c.panel;
+    const int Bo = outer_block(n_pad);
+    const int nblk = static_cast<int>(ceil_div(n_pad, Bo));
+    k::set_int(U, c.info.get(), INT_MAX);
+
+    std::vector<hipEvent_t> used;
+    auto ev = [&]()
+    {
+        hipEvent_t e = c.event();
+        used.push_back(e);
+        return e;
+    };
+    {
+    Timed whole(c, U, "cholesky");
+    hipEvent_t start = ev();
+    GP_HIP(hipEventRecord(start, U));
+    GP_HIP(hipStreamWaitEvent(P, start, 0));
+
+    for (int kb = 0; kb < nblk; kb++)
+    {
+        const int p0 = kb * Bo;
+        const int pe = std::min(n_pad, p0 + Bo);
+        {
+            Timed t(c, P, "potrf_trsm");
+            for (int c0 = p0; c0 < pe; c0 += 64)
+            {
+                k::potrf_trsm64(P, A, lda, c0, n_pad, c.info.get());
+                if (c0 + 64 < pe)
+                {
+                    const long o = static_cast<long>(c0 + 64) * lda;
+                    k::GemmArgs g{};
+                    g.A = A + o + c0;
+                    g.lda = lda;
+                    g.B = A + o + c0;
+                    g.ldb = lda;
+                    g.C = A + o + c0 + 64;
+                    g.ldc = lda;
+                    g.M = n_pad - c0 - 64;
+                    g.N = pe - c0 - 64;
+                    g.K = 64;
+                    g.b_trans = true;
+                    g.lower = true;
+                    g.diag_off = 0;
+                    g.role = k::GemmRole::Panel;
+                    k::gemm_f64(P, g);
+                }
+            }
+        }
+        hipEvent_t panel_done = ev();
+        GP_HIP(hipEventRecord(panel_done, P));
+        if (pe >= n_pad)
+        {
+            GP_HIP(hipStreamWaitEvent(U, panel_done, 0));
+            break;
+        }
+        GP_HIP(hipStreamWaitEvent(U, panel_done, 0));
+        const int q1 = std::min(n_pad, pe + Bo);
+        {
+            Timed t(c, U, "syrk_trailing");
+            const long o = static_cast<long>(pe) * lda;
+            k::GemmArgs g{};
+            g.A = A + o + p0;
+            g.lda = lda;
+            g.B = A + o + p0;
+            g.ldb = lda;
+            g.C = A + o + pe;
+            g.ldc = lda;
+            g.M = n_pad - pe;
+            g.N = q1 - pe;
+            g.K = pe - p0;
+            g.b_trans = true;
+            g.lower = true;
+            g.role = k::GemmRole::Trailing;
+            k::gemm_f64(U, g);
+        }
+        hipEvent_t next_ready = ev();
+        GP_HIP(hipEventRecord(next_ready, U));
+        GP_HIP(hipStreamWaitEvent(P, next_ready, 0));
+        if (q1 < n_pad)
+        {
+            Timed t(c, U, "syrk_trailing");
+            const long o = static_cast<long>(q1) * lda;
+            k::GemmArgs g{};
+            g.A = A + o + p0;
+            g.lda = lda;
+            g.B = A + o + p0;
+            g.ldb = lda;
+            g.C = A + o + q1;
+            g.ldc = lda;
+            g.M = n_pad - q1;
+            g.N = n_pad - q1;
+            g.K = pe - p0;
+            g.b_trans = true;
+            g.lower = true;
+            g.role = k::GemmRole::Trailing;
+            k::gemm_f64(U, g);
+        }
+    }
+    }
+    c.sync();
+    for (auto e : used)
+        c.recycle(e);
+    int info = INT_MAX;
+    GP_HIP(hipMemcpy(&info, c.info.get(), sizeof(int), hipMemcpyDefault));
+    if (info != INT_MAX)
+        throw Error(GPRAT_ERR_NOT_SPD,
+                    "covariance matrix is not positive definite (first non-positive pivot at column "
+                        + std::to_string(info) + ")");
+}
+
+// alpha = K^-1 y by the blocked forward and backward solves.
+void solve_spd(Context &c, const double *L, long lda, int n_pad, const double *y, double *work, double *out)
+{
+    hipStream_t s = c.main;
+    Timed t(c, s, "solve");
+    const int nb = n_pad / 64;
+    // work <- y (rhs of the forward solve, updated in place)
+    GP_HIP(hipMemcpyAsync(work, y, sizeof(double) * n_pad, hipMemcpyDeviceToDevice, s));
+    for (int j = 0; j < nb; j++)
+        k::solve_lower_step(s, L, lda, work, out, n_pad, j);
+    // out holds L^-1 y; use it as the rhs of the backward solve, result in work
+    for (int j = nb - 1; j >= 0; j--)
+        k::solve_upper_step(s, L, lda, out, work, n_pad, j);
+    GP_HIP(hipMemcpyAsync(out, work, sizeof(double) * n_pad, hipMemcpyDeviceToDevice, s));
+}
+
+// B[a:b, :ncols] <- L[a:b, a:b]^-1 B[a:b, :ncols] by recursive halving; the
+// off-diagonal coupling is one MFMA product per level.
+// With tri_rhs the right-hand side is lower triangular (B = I for L^-1), so
+// rows [a, b) only have non-zero columns below b and the column range of
+// every step is cut accordingly.
+void trsm_lower_left(Context &c, const double *L, long lda, double *B, long ldb, int ncols, double *colsq, int a, int b, bool tri_rhs)
+{
+    if (b - a == 64)
+    {
+        k::trsm_left64(c.main, L, lda, B, ldb, a, tri_rhs ? std::min(ncols, b) : ncols, colsq);
+        return;
+    }
+    const int blocks = (b - a) / 64;
+    const int mid = a + 64 * ((blocks + 1) / 2);
+    trsm_lower_left(c, L, lda, B, ldb, ncols, colsq, a, mid, tri_rhs);
+    k::GemmArgs g{};
+    g.A = L + static_cast<long>(mid) * lda + a;
+    g.lda = lda;
+    g.B = B + static_cast<long>(a) * ldb;
+    g.ldb = ldb;
+    g.C = B + static_cast<long>(mid) * ldb;
+    g.ldc = ldb;
+    g.M = b - mid;
+    g.N = tri_rhs ? std::min(ncols, mid) : ncols;
+    g.K = mid - a;
+    g.role = k::GemmRole::Solve;
+    k::gemm_f64(c.main, g);
+    trsm_lower_left(c, L, lda, B, ldb, ncols, colsq, mid, b, tri_rhs);
+}
+
+// ---------------------------------------------------------------------------
+// Model operations
+
+static void upload(Context &c, double *dst, const double *src, std::size_t count)
+{
+    if (count)
+        GP_HIP(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDefault, c.main));
+}
+
+void factor(Context &c, Model &m, const double *x, std::size_t x_len, int n, int r, const double theta[3], int precision)
+{
+    require(n >= 1, "need at least one training point");
+    require(r >= 1, "n_reg must be >= 1");
+    require(x != nullptr && x_len >= static_cast<std::size_t>(n) + r - 1, "training input must hold n + n_reg - 1 values");
+    require(theta != nullptr, "theta is null");
+    require(precision == GPRAT_PRECISION_FP64, "only GPRAT_PRECISION_FP64 is available in this build");
+    c.make_current();
+    m.ctx = &c;
+    m.n = n;
+    m.r = r;
+    m.n_pad = static_cast<int>(padded(n));
+    m.lda = leading_dim(m.n_pad);
+    m.sek = SEK{ theta[0], theta[1], theta[2] };
+    m.precision = precision;
+    const std::size_t xl = static_cast<std::size_t>(n) + r - 1;
+    m.x = DevBuf<double>(c, xl);
+    m.L = DevBuf<double>(c, static_cast<std::size_t>(m.n_pad) * m.lda);
+    upload(c, m.x.get(), x, xl);
+    {
+        Timed t(c, c.main, "assemble");
+        k::assemble_lower(c.main, m.x.get(), r, m.sek, m.L.get(), m.lda, n, m.n_pad);
+    }
+    cholesky_fp64(c, m.L.get(), m.lda, m.n_pad);
+}
+
+void attach_targets(Model &m, const double *y, std::size_t y_len)
+{
+    Context &c = *m.ctx;
+    require(y != nullptr && y_len >= static_cast<std::size_t>(m.n), "training output must hold n values");
+    c.make_current();
+    m.y = DevBuf<double>(c, m.n_pad);
+    m.alpha = DevBuf<double>(c, m.n_pad);
+    DevBuf<double> work(c, m.n_pad);
+    k::fill(c.main, m.y.get(), m.n_pad, 0.0);
+    upload(c, m.y.get(), y, static_cast<std::size_t>(m.n));
+    solve_spd(c, m.L.get(), m.lda, m.n_pad, m.y.get(), work.get(), m.alpha.get());
+    c.sync();
+}
+
+void fit(Context &c,
+         Model &m,
+         const double *x,
+         std::size_t x_len,
+         const double *y,
+         std::size_t y_len,
+         int n,
+         int r,
+         const double theta[3],
+         int precision)
+{
+    require(y != nullptr && y_len >= static_cast<std::size_t>(n), "training output must hold n values");
+    factor(c, m, x, x_len, n, r, theta, precision);
+    attach_targets(m, y, y_len);
+}
+
+void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean, double *var)
+{
+    Context &c = *m.ctx;
+    require(mt >= 0, "predict: negative test count");
+    if (mt == 0)
+        return;
+    require(z != nullptr && z_len >= static_cast<std::size_t>(mt) + m.r - 1,
+            "predict: test input must hold m + n_reg - 1 values");
+    require(mean != nullptr, "predict: mean output is null");
+    c.make_current();
+    const std::size_t zl = static_cast<std::size_t>(mt) + m.r - 1;
+    DevBuf<double> dz(c, zl);
+    upload(c, dz.get(), z, zl);
+    const int splits = k::mean_splits(m.n, mt);
+    DevBuf<double> partial(c, static_cast<std::size_t>(splits) * mt);
+    DevBuf<double> dmean(c, mt);
+    {
+        Timed t(c, c.main, "mean");
+        k::predict_mean(c.main, m.x.get(), dz.get(), m.alpha.get(), m.r, m.sek, m.n, mt, partial.get(), splits, dmean.get());
+    }
+    DevBuf<double> dvar;
+    if (var)
+    {
+        const int m_pad = static_cast<int>(round_up(mt, 64));
+        const long ldb = m_pad;
+        DevBuf<double> B(c, static_cast<std::size_t>(m.n_pad) * ldb);
+        DevBuf<double> colsq(c, m_pad);
+        dvar = DevBuf<double>(c, mt);
+        {
+            Timed t(c, c.main, "assemble");
+            k::assemble_cross_t(c.main, m.x.get(), dz.get(), m.r, m.sek, B.get(), ldb, m.n, m.n_pad, mt, m_pad);
+        }
+        k::fill(c.main, colsq.get(), m_pad, 0.0);
+        {
+            Timed t(c, c.main, "trsm");
+            trsm_lower_left(c, m.L.get(), m.lda, B.get(), ldb, m_pad, colsq.get(), 0, m.n_pad, false);
+        }
+        k::variance_finish(c.main, colsq.get(), m.sek.vertical, mt, dvar.get());
+        GP_HIP(hipMemcpyAsync(var, dvar.get(), sizeof(double) * mt, hipMemcpyDefault, c.main));
+    }
+    GP_HIP(hipMemcpyAsync(mean, dmean.get(), sizeof(double) * mt, hipMemcpyDefault, c.main));
+    c.sync();
+}
+
+double loss(Model &m)
+{
+    Context &c = *m.ctx;
+    c.make_current();
+    k::loss_terms(c.main, m.L.get(), m.lda, m.y.get(), m.alpha.get(), m.n, c.scalars.get());
+    double t[3];
+    GP_HIP(hipMemcpyAsync(t, c.scalars.get(), sizeof t, hipMemcpyDefault, c.main));
+    c.sync();
+    // core/src/cpu/gp_optimizer.cpp:205-218
+    const double Nn = static_cast<double>(m.n);
+    double l = t[0] + t[1];
+    l += Nn * std::log(2.0 * std::numbers::pi);
+    return 0.5 * l / Nn;
+}
+
+// Softplus chain factors, as core/src/cpu/gp_optimizer.cpp:12-36.
+static double to_unconstrained(double p, bool noise)
+{
+    return noise ? std::log(std::exp(p - 1e-6) - 1.0) : std::log(std::exp(p) - 1.0);
+}
+static double sigmoid(double p) { return 1.0 / (1.0 + std::exp(-p)); }
+
+// Loss and softplus-space gradient at theta. K^-1 is formed as W^T W with
+// W = L^-1 (recursive TRSM on the triangular identity, then one triangular
+// MFMA product), n^3 flops in total instead of the reference's explicit
+// 2 n^3 double solve (core/src/cpu/gp_functions.cpp:953-966); the trace and
+// quadratic terms of core/src/cpu/tiled_algorithms.cpp:296-426 then come from
+// one fused pass over the lower triangle of K^-1.
+void loss_gradient(Context &c,
+                   const double *x,
+                   std::size_t x_len,
+                   const double *y,
+                   std::size_t y_len,
+                   const double theta[3],
+                   int n,
+                   int r,
+                   const int trainable[3],
+                   double *loss_out,
+                   double grad[3])
+{
+    Model m;
+    fit(c, m, x, x_len, y, y_len, n, r, theta, GPRAT_PRECISION_FP64);
+    k::loss_terms(c.main, m.L.get(), m.lda, m.y.get(), m.alpha.get(), m.n, c.scalars.get());
+    double t[3];
+    GP_HIP(hipMemcpyAsync(t, c.scalars.get(), sizeof t, hipMemcpyDefault, c.main));
+    c.sync();
+    const double Nn = static_cast<double>(n);
+    {
+        double l = t[0] + t[1];
+        l += Nn * std::log(2.0 * std::numbers::pi);
+        *loss_out = 0.5 * l / Nn;
+    }
+    grad[0] = grad[1] = grad[2] = 0.0;
+    if (!(trainable[0] || trainable[1] || trainable[2]))
+        return;
+    const std::size_t elems = static_cast<std::size_t>(m.n_pad) * m.lda;
+    DevBuf<double> W(c, elems);
+    DevBuf<double> Kinv(c, elems);
+    k::fill(c.main, W.get(), static_cast<long>(elems), 0.0);
+    k::set_diagonal(c.main, W.get(), m.lda, m.n_pad, 1.0);
+    {
+        Timed tm(c, c.main, "inverse");
+        trsm_lower_left(c, m.L.get(), m.lda, W.get(), m.lda, m.n_pad, nullptr, 0, m.n_pad, true);
+        k::GemmArgs g{};
+        g.A = W.get();
+        g.lda = m.lda;
+        g.B = W.get();
+        g.ldb = m.lda;
+        g.C = Kinv.get();
+        g.ldc = m.lda;
+        g.M = g.N = g.K = m.n_pad;
+        g.a_trans = true;
+        g.b_trans = false;
+        g.lower = true;
+        g.k_from_tile = true;
+        g.set = true;
+        g.role = k::GemmRole::Inverse;
+        k::gemm_f64(c.main, g);
+    }
+    W.reset();
+    const double sig_l = sigmoid(to_unconstrained(m.sek.lengthscale, false));
+    const double sig_v = sigmoid(to_unconstrained(m.sek.vertical, false));
+    const double sig_n = sigmoid(to_unconstrained(m.sek.noise, true));
+    DevBuf<double> part(c, static_cast<std::size_t>(k::grad_tiles(n)) * 5);
+    {
+        Timed tm(c, c.main, "gradient");
+        k::gradient_sums(c.main, Kinv.get(), m.lda, m.x.get(), m.alpha.get(), r, m.sek, sig_l, sig_v, n, part.get(),
+                         c.scalars.get() + 8);
+    }
+    double sums[5];
+    GP_HIP(hipMemcpyAsync(sums, c.scalars.get() + 8, sizeof sums, hipMemcpyDefault, c.main));
+    c.sync();
+    // core/src/cpu/gp_optimizer.cpp:222-225 and tiled_algorithms.cpp:413-426
+    if (trainable[0])
+        grad[0] = 0.5 / Nn * (sums[0] - sums[2]);
+    if (trainable[1])
+        grad[1] = 0.5 / Nn * (sums[1] - sums[3]);
+    if (trainable[2])
+        grad[2] = sig_n * (0.5 / Nn * (sums[4] - t[2]));
+}
+
+void cholesky_tiles(Model &m, int T, int b, double *out)
+{
+    Context &c = *m.ctx;
+    c.make_current();
+    require(static_cast<long>(T) * b == m.n, "cholesky_tiles: tiling does not match the model");
+    const std::size_t count = static_cast<std::size_t>(T) * (T + 1) / 2 * b * b;
+    DevBuf<double> d(c, count);
+    k::gather_tiles(c.main, m.L.get(), m.lda, m.x.get(), m.r, m.sek, T, b, d.get());
+    GP_HIP(hipMemcpyAsync(out, d.get(), count * sizeof(double), hipMemcpyDefault, c.main));
+    c.sync();
+}
+
+}  // namespace gpamd

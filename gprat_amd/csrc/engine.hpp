@@ -1,0 +1,159 @@
+// Host-side engine of the backend: one Context per GPU (streams, caching
+// device pool, profiling), resident Models (padded factor + alpha), and the
+// blocked algorithms that schedule the kernels.
+#pragma once
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace gpamd
+{
+
+// Caching device allocator: blocks are reused by size class so repeated
+// calls of the same shape never reach hipMalloc.
+class DevicePool
+{
+  public:
+    explicit DevicePool(int device) : device_(device) { }
+    ~DevicePool();
+    void *get(std::size_t bytes);
+    void put(void *p);
+    void release_cached();
+    std::size_t cached_bytes() const { return cached_; }
+
+  private:
+    int device_;
+    std::multimap<std::size_t, void *> free_;
+    std::map<void *, std::size_t> live_;
+    std::size_t cached_ = 0;
+};
+
+struct Context;
+
+// RAII handle on a pooled device block.
+template <class T>
+class DevBuf
+{
+  public:
+    DevBuf() = default;
+    DevBuf(Context &c, std::size_t count);
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : pool_(o.pool_), p_(o.p_), n_(o.n_) { o.p_ = nullptr; }
+    DevBuf &operator=(DevBuf &&o) noexcept
+    {
+        reset();
+        pool_ = o.pool_;
+        p_ = o.p_;
+        n_ = o.n_;
+        o.p_ = nullptr;
+        return *this;
+    }
+    ~DevBuf() { reset(); }
+    void reset();
+    T *get() const { return p_; }
+    std::size_t size() const { return n_; }
+    std::size_t bytes() const { return n_ * sizeof(T); }
+
+  private:
+    DevicePool *pool_ = nullptr;
+    T *p_ = nullptr;
+    std::size_t n_ = 0;
+};
+
+struct KernelStat
+{
+    long launches = 0;
+    double ms = 0.0;
+};
+
+struct Context
+{
+    Context(int device, int n_streams);
+    ~Context();
+
+    int device;
+    int n_streams;
+    hipStream_t main = nullptr;   // trailing updates, solves, predictions
+    hipStream_t panel = nullptr;  // Cholesky panel factorisation (look-ahead)
+    DevicePool pool;
+    std::recursive_mutex mu;
+
+    DevBuf<int> info;         // smallest non-positive pivot column
+    DevBuf<double> scalars;   // reduction results
+
+    // profiling
+    bool profiling = false;
+    std::map<std::string, KernelStat> stats;
+    struct Pending
+    {
+        std::string family;
+        hipEvent_t begin, end;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> spare_events;
+    hipEvent_t event();
+    void recycle(hipEvent_t e);
+    // Brackets one launch (or launch sequence) of a kernel family.
+    void prof_begin(hipStream_t s, const char *family, Pending &p);
+    void prof_end(hipStream_t s, Pending &p);
+    void resolve_profiling();  // after a synchronize
+
+    void make_current() const;
+    void sync();
+};
+
+// Resident fitted model.
+struct Model
+{
+    Context *ctx = nullptr;
+    int n = 0, n_pad = 0, r = 0;
+    long lda = 0;
+    SEK sek{};
+    int precision = GPRAT_PRECISION_FP64;
+    DevBuf<double> x;      // training series (n + r - 1)
+    DevBuf<double> y;      // targets, zero padded to n_pad
+    DevBuf<double> L;      // n_pad x lda, lower triangle holds the factor
+    DevBuf<double> alpha;  // K^-1 y (n_pad)
+    std::size_t bytes() const { return x.bytes() + y.bytes() + L.bytes() + alpha.bytes(); }
+};
+
+// Algorithms (engine.cpp)
+long padded(long n);
+long leading_dim(long n_pad);
+void cholesky_fp64(Context &c, double *A, long lda, int n_pad);
+void solve_spd(Context &c, const double *L, long lda, int n_pad, const double *y, double *work, double *out);
+void trsm_lower_left(Context &c, const double *L, long lda, double *B, long ldb, int ncols, double *colsq, int a, int b, bool tri_rhs);
+
+void factor(Context &c, Model &m, const double *x, std::size_t x_len, int n, int r, const double theta[3], int precision);
+void attach_targets(Model &m, const double *y, std::size_t y_len);
+void fit(Context &c,
+         Model &m,
+         const double *x,
+         std::size_t x_len,
+         const double *y,
+         std::size_t y_len,
+         int n,
+         int r,
+         const double theta[3],
+         int precision);
+void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean, double *var);
+double loss(Model &m);
+void cholesky_tiles(Model &m, int T, int b, double *out);
+void loss_gradient(Context &c,
+                   const double *x,
+                   std::size_t x_len,
+                   const double *y,
+                   std::size_t y_len,
+                   const double theta[3],
+                   int n,
+                   int r,
+                   const int trainable[3],
+                   double *loss_out,
+                   double grad[3]);
+
+}  // namespace gpamd

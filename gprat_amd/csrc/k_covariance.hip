@@ -1,0 +1,315 @@
+// SE covariance generation on gfx950: training covariance (lower band),
+// transposed cross-covariance, the fused predictive mean and the tile gather
+// for cholesky(). All are HBM-write-bound or exp-bound streaming kernels;
+// 64-lane wavefronts own 64 consecutive columns so every store is a fully
+// coalesced 512-byte row segment.
+#include "kernels.hpp"
+
+namespace gpamd::k
+{
+namespace
+{
+
+constexpr int TB = 64;       // tile edge
+constexpr int THREADS = 256;  // 4 wavefronts, each covering 16 rows of the tile
+
+// SE entry as core/src/cpu/gp_algorithms.cpp:11-28 evaluates it.
+__device__ __forceinline__ double se(double scale, double v, double d) { return v * exp(scale * d); }
+
+__global__ __launch_bounds__(THREADS) void k_assemble_lower(const double *__restrict__ x,
+                                                             int r,
+                                                             double scale,
+                                                             double v,
+                                                             double noise,
+                                                             double *__restrict__ A,
+                                                             long lda,
+                                                             int n)
+{
+    const int ti = blockIdx.y, tj = blockIdx.x;
+    if (tj > ti)
+        return;
+    const int c = threadIdx.x % TB;
+    const long j = static_cast<long>(tj) * TB + c;
+    for (int rr = threadIdx.x / TB; rr < TB; rr += THREADS / TB)
+    {
+        const long i = static_cast<long>(ti) * TB + rr;
+        double val;
+        if (i < n && j < n)
+        {
+            val = se(scale, v, window_sqdist(x, i, x, j, r));
+            if (i == j)
+                val += noise;
+        }
+        else
+        {
+            val = (i == j) ? 1.0 : 0.0;
+        }
+        A[i * lda + j] = val;
+    }
+}
+
+__global__ __launch_bounds__(THREADS) void k_assemble_cross_t(const double *__restrict__ x,
+                                                               const double *__restrict__ z,
+                                                               int r,
+                                                               double scale,
+                                                               double v,
+                                                               double *__restrict__ B,
+                                                               long ldb,
+                                                               int n,
+                                                               int m)
+{
+    const int ti = blockIdx.y, tj = blockIdx.x;
+    const int c = threadIdx.x % TB;
+    const long j = static_cast<long>(tj) * TB + c;  // test index
+    for (int rr = threadIdx.x / TB; rr < TB; rr += THREADS / TB)
+    {
+        const long i = static_cast<long>(ti) * TB + rr;  // train index
+        double val = 0.0;
+        if (i < n && j < m)
+            val = se(scale, v, window_sqdist(z, j, x, i, r));  // (test - train), as gp_algorithms.cpp:113-140
+        B[i * ldb + j] = val;
+    }
+}
+
+// Grid (ceil(m/64), splits). Thread (lane = test point, wave = train stride
+// class) sums k(z_j, x_i) alpha_i over its share of split s; the four wave
+// partials are combined in a fixed order, so the result is deterministic.
+__global__ __launch_bounds__(THREADS) void k_mean_partial(const double *__restrict__ x,
+                                                           const double *__restrict__ z,
+                                                           const double *__restrict__ alpha,
+                                                           int r,
+                                                           double scale,
+                                                           double v,
+                                                           int n,
+                                                           int m,
+                                                           int chunk,
+                                                           double *__restrict__ partial)
+{
+    __shared__ double red[THREADS];
+    const int lane = threadIdx.x % TB, w = threadIdx.x / TB;
+    const long j = static_cast<long>(blockIdx.x) * TB + lane;
+    const long i0 = static_cast<long>(blockIdx.y) * chunk;
+    const long i1 = min(static_cast<long>(n), i0 + chunk);
+    double acc = 0.0;
+    if (j < m)
+        for (long i = i0 + w; i < i1; i += THREADS / TB)
+            acc += se(scale, v, window_sqdist(z, j, x, i, r)) * alpha[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (w == 0 && j < m)
+    {
+        double s = red[lane];
+        for (int q = 1; q < THREADS / TB; q++)
+            s += red[q * TB + lane];
+        partial[static_cast<long>(blockIdx.y) * m + j] = s;
+    }
+}
+
+__global__ __launch_bounds__(THREADS) void k_mean_reduce(const double *__restrict__ partial, int splits, int m, double *__restrict__ mean)
+{
+    const long j = static_cast<long>(blockIdx.x) * THREADS + threadIdx.x;
+    if (j >= m)
+        return;
+    double s = 0.0;
+    for (int q = 0; q < splits; q++)
+        s += partial[static_cast<long>(q) * m + j];
+    mean[j] = s;
+}
+
+// grid.x = lower tile index t = i(i+1)/2 + j, grid.y = row chunk of the tile.
+__global__ __launch_bounds__(THREADS) void k_gather_tiles(const double *__restrict__ L,
+                                                           long lda,
+                                                           const double *__restrict__ x,
+                                                           int r,
+                                                           double scale,
+                                                           double v,
+                                                           int b,
+                                                           int rows_per_block,
+                                                           double *__restrict__ out)
+{
+    const long t = blockIdx.x;
+    // invert t = i(i+1)/2 + j
+    long i = static_cast<long>((sqrt(8.0 * static_cast<double>(t) + 1.0) - 1.0) / 2.0);
+    while (i * (i + 1) / 2 > t)
+        i--;
+    while ((i + 1) * (i + 2) / 2 <= t)
+        i++;
+    const long j = t - i * (i + 1) / 2;
+    const long rbeg = static_cast<long>(blockIdx.y) * rows_per_block;
+    const long rend = min(static_cast<long>(b), rbeg + rows_per_block);
+    double *tile = out + t * static_cast<long>(b) * b;
+    for (long rr = rbeg; rr < rend; rr++)
+        for (long cc = threadIdx.x; cc < b; cc += THREADS)
+        {
+            const long gi = i * b + rr, gj = j * b + cc;
+            double val;
+            if (i == j && cc > rr)
+                val = se(scale, v, window_sqdist(x, gi, x, gj, r));  // strict upper: original K, no noise
+            else
+                val = L[gi * lda + gj];
+            tile[rr * b + cc] = val;
+        }
+}
+
+// Gradient sums over the lower triangle of K^-1 (one 64x64 tile per
+// workgroup), with dK regenerated from the series instead of stored:
+//   D = -0.5/(l*l) d2, e = exp(D), dK_l = (-2 v / l) D e s_l, dK_v = e s_v
+// part[t][0..4] = sum w Kinv dK_l, sum w Kinv dK_v, sum w a_i a_j dK_l,
+//                 sum w a_i a_j dK_v, sum_diag Kinv   (w = 2 off the diagonal).
+__global__ __launch_bounds__(THREADS) void k_grad_partial(const double *__restrict__ Kinv,
+                                                           long lda,
+                                                           const double *__restrict__ x,
+                                                           const double *__restrict__ alpha,
+                                                           int r,
+                                                           double scale,
+                                                           double fl,
+                                                           double sv,
+                                                           int n,
+                                                           int tiles,
+                                                           double *__restrict__ part)
+{
+    __shared__ double red[5][THREADS];
+    const long t = blockIdx.x;
+    long ti = static_cast<long>((sqrt(8.0 * static_cast<double>(t) + 1.0) - 1.0) / 2.0);
+    while (ti * (ti + 1) / 2 > t)
+        ti--;
+    while ((ti + 1) * (ti + 2) / 2 <= t)
+        ti++;
+    const long tj = t - ti * (ti + 1) / 2;
+    const int c = threadIdx.x % TB;
+    const long j = tj * TB + c;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+    if (j < n)
+    {
+        const double aj = alpha[j];
+        for (int rr = threadIdx.x / TB; rr < TB; rr += THREADS / TB)
+        {
+            const long i = ti * TB + rr;
+            if (i >= n || i < j)
+                continue;
+            const double kij = Kinv[i * lda + j];
+            const double D = scale * window_sqdist(x, i, x, j, r);
+            const double e = exp(D);
+            const double dkl = fl * D * e;
+            const double dkv = e * sv;
+            const double w = (i == j) ? 1.0 : 2.0;
+            const double aa = alpha[i] * aj;
+            s0 += w * kij * dkl;
+            s1 += w * kij * dkv;
+            s2 += w * aa * dkl;
+            s3 += w * aa * dkv;
+            if (i == j)
+                s4 += kij;
+        }
+    }
+    red[0][threadIdx.x] = s0;
+    red[1][threadIdx.x] = s1;
+    red[2][threadIdx.x] = s2;
+    red[3][threadIdx.x] = s3;
+    red[4][threadIdx.x] = s4;
+    __syncthreads();
+    for (int w = THREADS / 2; w > 0; w >>= 1)
+    {
+        if (threadIdx.x < w)
+            for (int q = 0; q < 5; q++)
+                red[q][threadIdx.x] += red[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5)
+        part[t * 5 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// out[q] = sum_t part[t][q], fixed order (single workgroup).
+__global__ __launch_bounds__(THREADS) void k_grad_reduce(const double *__restrict__ part, int tiles, double *__restrict__ out)
+{
+    __shared__ double red[5][THREADS];
+    double s[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+    for (long t = threadIdx.x; t < tiles; t += THREADS)
+        for (int q = 0; q < 5; q++)
+            s[q] += part[t * 5 + q];
+    for (int q = 0; q < 5; q++)
+        red[q][threadIdx.x] = s[q];
+    __syncthreads();
+    for (int w = THREADS / 2; w > 0; w >>= 1)
+    {
+        if (threadIdx.x < w)
+            for (int q = 0; q < 5; q++)
+                red[q][threadIdx.x] += red[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5)
+        out[threadIdx.x] = red[threadIdx.x][0];
+}
+
+}  // namespace
+
+long grad_tiles(int n) { return ceil_div(n, TB) * (ceil_div(n, TB) + 1) / 2; }
+
+void gradient_sums(hipStream_t s,
+                   const double *Kinv,
+                   long lda,
+                   const double *x,
+                   const double *alpha,
+                   int r,
+                   SEK sek,
+                   double sig_l,
+                   double sig_v,
+                   int n,
+                   double *part,
+                   double *out)
+{
+    const long tiles = grad_tiles(n);
+    require(tiles >= 1 && tiles < (1L << 31), "gradient_sums: bad shape");
+    const double fl = -2.0 * sek.vertical / sek.lengthscale * sig_l;
+    hipLaunchKernelGGL(k_grad_partial, dim3(static_cast<unsigned>(tiles)), dim3(THREADS), 0, s, Kinv, lda, x, alpha, r, sek.scale(),
+                       fl, sig_v, n, static_cast<int>(tiles), part);
+    GP_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_grad_reduce, dim3(1), dim3(THREADS), 0, s, part, static_cast<int>(tiles), out);
+    GP_CHECK_LAUNCH();
+}
+
+void assemble_lower(hipStream_t s, const double *x, int r, SEK sek, double *A, long lda, int n, int n_pad)
+{
+    require(n_pad % TB == 0 && n <= n_pad && lda >= n_pad && r >= 1, "assemble_lower: bad shape");
+    dim3 grid(n_pad / TB, n_pad / TB);
+    hipLaunchKernelGGL(k_assemble_lower, grid, dim3(THREADS), 0, s, x, r, sek.scale(), sek.vertical, sek.noise, A, lda, n);
+    GP_CHECK_LAUNCH();
+}
+
+void assemble_cross_t(hipStream_t s, const double *x, const double *z, int r, SEK sek, double *B, long ldb, int n, int n_pad, int m, int m_pad)
+{
+    require(n_pad % TB == 0 && m_pad % TB == 0 && n <= n_pad && m <= m_pad && ldb >= m_pad, "assemble_cross_t: bad shape");
+    dim3 grid(m_pad / TB, n_pad / TB);
+    hipLaunchKernelGGL(k_assemble_cross_t, grid, dim3(THREADS), 0, s, x, z, r, sek.scale(), sek.vertical, B, ldb, n, m);
+    GP_CHECK_LAUNCH();
+}
+
+int mean_splits(int n, int m)
+{
+    const long col_blocks = ceil_div(m, TB);
+    long splits = ceil_div(2048, col_blocks);
+    splits = std::min<long>(splits, ceil_div(n, 512));
+    return static_cast<int>(std::max<long>(splits, 1));
+}
+
+void predict_mean(hipStream_t s, const double *x, const double *z, const double *alpha, int r, SEK sek, int n, int m, double *partial, int splits, double *mean)
+{
+    require(n >= 1 && m >= 1 && splits >= 1, "predict_mean: bad shape");
+    const int chunk = static_cast<int>(ceil_div(n, splits));
+    dim3 grid(ceil_div(m, TB), splits);
+    hipLaunchKernelGGL(k_mean_partial, grid, dim3(THREADS), 0, s, x, z, alpha, r, sek.scale(), sek.vertical, n, m, chunk, partial);
+    GP_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_mean_reduce, dim3(ceil_div(m, THREADS)), dim3(THREADS), 0, s, partial, splits, m, mean);
+    GP_CHECK_LAUNCH();
+}
+
+void gather_tiles(hipStream_t s, const double *L, long lda, const double *x, int r, SEK sek, int T, int b, double *out)
+{
+    require(T >= 1 && b >= 1, "gather_tiles: bad shape");
+    const int rows_per_block = 16;
+    dim3 grid(static_cast<unsigned>(static_cast<long>(T) * (T + 1) / 2), ceil_div(b, rows_per_block));
+    hipLaunchKernelGGL(k_gather_tiles, grid, dim3(THREADS), 0, s, L, lda, x, r, sek.scale(), sek.vertical, b, rows_per_block, out);
+    GP_CHECK_LAUNCH();
+}
+
+}  // namespace gpamd::k

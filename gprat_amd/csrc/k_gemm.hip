@@ -1,0 +1,228 @@
+// FP64 dense update on gfx950 matrix cores (v_mfma_f64_16x16x4_f64).
+//
+// One workgroup = 256 threads = 4 wavefronts computes a 128x128 tile of
+// C (op)= A * B; each wavefront owns a 64x64 quarter as a 4x4 grid of 16x16
+// MFMA accumulators (64 doubles per lane). K advances in chunks of 16
+// through a double-buffered LDS stage stored k-major ([k][row]), so each MFMA
+// operand is one ds_read_b64 per lane: lanes 0-15 read 16 consecutive rows
+// of k-slot 0, lanes 16-31 of k-slot 1, and so on. The LDS row stride of 144
+// doubles puts the four k-slots of a read in disjoint bank halves.
+//
+// The same kernel serves the Cholesky trailing update (A = B^T = panel,
+// lower triangle only), the panel-internal updates, the right-looking TRSM
+// updates (B not transposed) and W^T W (A transposed, triangular K range).
+#include "kernels.hpp"
+
+namespace gpamd::k
+{
+namespace
+{
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 128;
+constexpr int BN = 128;
+constexpr int BK = 16;
+constexpr int LDS_STRIDE = BM + 16;
+constexpr int THREADS = 256;
+
+struct Params
+{
+    const double *A;
+    long lda;
+    const double *B;
+    long ldb;
+    double *C;
+    long ldc;
+    int M, N, K;
+    int lower;
+    long diag_off;
+    int k_from_tile;
+    long kstart_off;
+    int set;
+};
+
+// Global -> register staging of one BK-deep slab of a 128-wide operand.
+// `trans == false`: operand element (row, k) is P[row*ld + k] (rows contiguous in k).
+// `trans == true`:  operand element (row, k) is P[k*ld + row] (k-rows contiguous in row).
+template <bool TRANS>
+__device__ __forceinline__ void load_slab(const double *__restrict__ P, long ld, long row0, int rows, long k0, int tid, double2 (&reg)[4])
+{
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+    {
+        const int idx = tid + THREADS * s;
+        if (!TRANS)
+        {
+            const int row = idx >> 3;        // 0..127
+            const int kk = (idx & 7) * 2;    // 0..14
+            if (row < rows)
+                reg[s] = *reinterpret_cast<const double2 *>(P + (row0 + row) * ld + k0 + kk);
+            else
+                reg[s] = make_double2(0.0, 0.0);
+        }
+        else
+        {
+            const int kk = idx >> 6;           // 0..15
+            const int col = (idx & 63) * 2;    // 0..126
+            if (col < rows)
+                reg[s] = *reinterpret_cast<const double2 *>(P + (k0 + kk) * ld + row0 + col);
+            else
+                reg[s] = make_double2(0.0, 0.0);
+        }
+    }
+}
+
+template <bool TRANS>
+__device__ __forceinline__ void store_slab(double (*S)[LDS_STRIDE], int tid, const double2 (&reg)[4])
+{
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+    {
+        const int idx = tid + THREADS * s;
+        if (!TRANS)
+        {
+            const int row = idx >> 3;
+            const int kk = (idx & 7) * 2;
+            S[kk][row] = reg[s].x;
+            S[kk + 1][row] = reg[s].y;
+        }
+        else
+        {
+            const int kk = idx >> 6;
+            const int col = (idx & 63) * 2;
+            *reinterpret_cast<double2 *>(&S[kk][col]) = reg[s];
+        }
+    }
+}
+
+template <bool AT, bool BT, int ROLE>
+__global__ __launch_bounds__(THREADS) void k_gemm_f64(Params p)
+{
+    __shared__ double As[2][BK][LDS_STRIDE];
+    __shared__ double Bs[2][BK][LDS_STRIDE];
+
+    const long r0 = static_cast<long>(blockIdx.y) * BM;
+    const long c0 = static_cast<long>(blockIdx.x) * BN;
+    if (p.lower && r0 + (BM - 1) + p.diag_off < c0)
+        return;  // tile entirely above the diagonal
+    long kbeg = 0;
+    if (p.k_from_tile)
+    {
+        kbeg = (r0 > c0 ? r0 : c0) + p.kstart_off;
+        kbeg = kbeg < 0 ? 0 : kbeg;
+        kbeg = (kbeg / BK) * BK;
+    }
+    const int rows_a = static_cast<int>(min(static_cast<long>(BM), p.M - r0));
+    const int rows_b = static_cast<int>(min(static_cast<long>(BN), p.N - c0));
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int lm = lane & 15, lk = lane >> 4;
+
+    v4d acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            acc[i][j] = v4d{ 0.0, 0.0, 0.0, 0.0 };
+
+    if (kbeg < p.K)
+    {
+        double2 ra[4], rb[4];
+        load_slab<AT>(p.A, p.lda, r0, rows_a, kbeg, tid, ra);
+        load_slab<!BT>(p.B, p.ldb, c0, rows_b, kbeg, tid, rb);
+        store_slab<AT>(As[0], tid, ra);
+        store_slab<!BT>(Bs[0], tid, rb);
+        __syncthreads();
+
+        int buf = 0;
+        for (long k0 = kbeg; k0 < p.K; k0 += BK)
+        {
+            const bool more = k0 + BK < p.K;
+            if (more)
+            {
+                load_slab<AT>(p.A, p.lda, r0, rows_a, k0 + BK, tid, ra);
+                load_slab<!BT>(p.B, p.ldb, c0, rows_b, k0 + BK, tid, rb);
+            }
+#pragma unroll
+            for (int kq = 0; kq < BK / 4; kq++)
+            {
+                double a[4], b[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    a[i] = As[buf][kq * 4 + lk][wr * 64 + i * 16 + lm];
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    b[j] = Bs[buf][kq * 4 + lk][wc * 64 + j * 16 + lm];
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+            }
+            if (more)
+            {
+                store_slab<AT>(As[buf ^ 1], tid, ra);
+                store_slab<!BT>(Bs[buf ^ 1], tid, rb);
+            }
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+
+    // Accumulator layout of the 16x16x4 f64 MFMA (measured on gfx950): lane
+    // holds column (lane % 16) and rows (lane / 16) + 4 e, e = 0..3.
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+            {
+                const int rr = wr * 64 + i * 16 + lk + 4 * e;
+                const int cc = wc * 64 + j * 16 + lm;
+                if (rr >= rows_a || cc >= rows_b)
+                    continue;
+                const long gi = r0 + rr, gj = c0 + cc;
+                if (p.lower && gi + p.diag_off < gj)
+                    continue;
+                double *dst = p.C + gi * p.ldc + gj;
+                if (p.set)
+                    *dst = acc[i][j][e];
+                else
+                    *dst -= acc[i][j][e];
+            }
+}
+
+}  // namespace
+
+void gemm_f64(hipStream_t s, const GemmArgs &g)
+{
+    if (g.M <= 0 || g.N <= 0)
+        return;
+    require(g.K >= 0 && g.K % BK == 0, "gemm_f64: K must be a multiple of 16");
+    require(g.M % 2 == 0 && g.N % 2 == 0, "gemm_f64: M and N must be even");
+    require(g.lda % 2 == 0 && g.ldb % 2 == 0, "gemm_f64: leading dimensions must be even");
+    require(reinterpret_cast<uintptr_t>(g.A) % 16 == 0 && reinterpret_cast<uintptr_t>(g.B) % 16 == 0,
+            "gemm_f64: operands must be 16-byte aligned");
+    Params p{ g.A, g.lda, g.B, g.ldb, g.C, g.ldc, g.M, g.N, g.K, g.lower ? 1 : 0, g.diag_off, g.k_from_tile ? 1 : 0,
+              g.kstart_off, g.set ? 1 : 0 };
+    dim3 grid(static_cast<unsigned>(ceil_div(g.N, BN)), static_cast<unsigned>(ceil_div(g.M, BM)));
+    require(grid.y <= 65535, "gemm_f64: M too large for one launch");
+    // Instantiated combinations: (N,T) trailing/panel, (N,N) solve, (T,N) inverse.
+    if (!g.a_trans && g.b_trans && g.role == GemmRole::Trailing)
+        hipLaunchKernelGGL((k_gemm_f64<false, true, 0>), grid, dim3(THREADS), 0, s, p);
+    else if (!g.a_trans && g.b_trans)
+        hipLaunchKernelGGL((k_gemm_f64<false, true, 1>), grid, dim3(THREADS), 0, s, p);
+    else if (!g.a_trans && !g.b_trans)
+        hipLaunchKernelGGL((k_gemm_f64<false, false, 2>), grid, dim3(THREADS), 0, s, p);
+    else if (g.a_trans && !g.b_trans)
+        hipLaunchKernelGGL((k_gemm_f64<true, false, 3>), grid, dim3(THREADS), 0, s, p);
+    else
+        throw Error(GPRAT_ERR_INTERNAL, "gemm_f64: unsupported operand layout (T,T)");
+    GP_CHECK_LAUNCH();
+}
+
+}  // namespace gpamd::k

@@ -1,0 +1,141 @@
+// Host-side launchers for the gfx950 kernels. Every matrix is row-major
+// double (or float where stated) with an explicit leading dimension; every
+// launcher checks the shapes it relies on before launching.
+#pragma once
+
+#include "common.hpp"
+
+namespace gpamd::k
+{
+
+// ---- covariance family (k_covariance.hip) ---------------------------------
+
+// Lower band of the padded training covariance into A (n_pad x n_pad, lda):
+// all 64x64 blocks on or below the diagonal. Rows/columns >= n form an
+// identity block, so the Cholesky factor of the padded matrix is
+// blockdiag(L, I).
+void assemble_lower(hipStream_t s, const double *x, int r, SEK sek, double *A, long lda, int n, int n_pad);
+
+// B = K*^T (n_pad x m_pad, ldb): B[i][j] = k(test j, train i); zero outside n x m.
+void assemble_cross_t(hipStream_t s,
+                      const double *x,
+                      const double *z,
+                      int r,
+                      SEK sek,
+                      double *B,
+                      long ldb,
+                      int n,
+                      int n_pad,
+                      int m,
+                      int m_pad);
+
+// mean[j] = sum_i k(test j, train i) alpha[i], K* generated on the fly.
+// `partial` holds splits * m doubles; returns nothing until the reduction
+// kernel (same stream) has written mean.
+int mean_splits(int n, int m);
+void predict_mean(hipStream_t s,
+                  const double *x,
+                  const double *z,
+                  const double *alpha,
+                  int r,
+                  SEK sek,
+                  int n,
+                  int m,
+                  double *partial,
+                  int splits,
+                  double *mean);
+
+// Reference tile layout of cholesky() (lower tiles; K in the strict upper
+// triangle of diagonal tiles).
+void gather_tiles(hipStream_t s, const double *L, long lda, const double *x, int r, SEK sek, int T, int b, double *out);
+
+// Sums for the hyperparameter gradient over the lower triangle of K^-1
+// (i, j < n); dK is regenerated on the fly. `part` holds grad_tiles(n) * 5
+// doubles. out[0..4] = tr(K^-1 dK_l), tr(K^-1 dK_v), a^T dK_l a, a^T dK_v a,
+// tr(K^-1), where dK_l, dK_v include the softplus factors sig_l, sig_v.
+long grad_tiles(int n);
+void gradient_sums(hipStream_t s,
+                   const double *Kinv,
+                   long lda,
+                   const double *x,
+                   const double *alpha,
+                   int r,
+                   SEK sek,
+                   double sig_l,
+                   double sig_v,
+                   int n,
+                   double *part,
+                   double *out);
+
+// ---- dense products on MFMA (k_gemm.hip) ----------------------------------
+
+// Which algorithm a product belongs to; each role is its own kernel symbol
+// so profilers can attribute time (same code otherwise).
+enum class GemmRole
+{
+    Trailing,  // Cholesky trailing update
+    Panel,     // Cholesky panel-internal update
+    Solve,     // TRSM coupling update
+    Inverse,   // W^T W
+};
+
+struct GemmArgs
+{
+    // C[i][j] (op)= sum_k A(i,k) B(k,j), i < M, j < N, k < K (K % 16 == 0).
+    // A(i,k) = a_trans ? A[k*lda+i] : A[i*lda+k]
+    // B(k,j) = b_trans ? B[j*ldb+k] : B[k*ldb+j]
+    const double *A;
+    long lda;
+    const double *B;
+    long ldb;
+    double *C;
+    long ldc;
+    int M, N, K;
+    bool a_trans;
+    bool b_trans;
+    // Only elements with i + diag_off >= j are touched (lower triangle of a
+    // matrix whose (0,0) sits diag_off rows below its diagonal).
+    bool lower;
+    long diag_off;
+    // Entries of A/B with k < max(row tile start, col tile start) + kstart_off
+    // are known zero and skipped (triangular operands, e.g. W^T W).
+    bool k_from_tile;
+    long kstart_off;
+    // true: C = acc; false: C -= acc.
+    bool set;
+    GemmRole role;
+};
+void gemm_f64(hipStream_t s, const GemmArgs &g);
+
+// ---- triangular family (k_triangular.hip) ---------------------------------
+
+// Factor the 64x64 diagonal block at (c0, c0) and solve the panel rows
+// [c0+64, row_end) of columns [c0, c0+64) against it. info (device int) takes
+// the smallest failing global column if a pivot is not positive.
+void potrf_trsm64(hipStream_t s, double *A, long lda, int c0, int row_end, int *info);
+
+// One block step of the forward solve L x = rhs (block j of 64 rows):
+// out[block j] = x_j and rhs[later blocks] -= L[later, j] x_j. Run for
+// j = 0 .. n_pad/64-1 in order.
+void solve_lower_step(hipStream_t s, const double *L, long lda, double *rhs, double *out, int n_pad, int j);
+// One block step of the backward solve L^T x = rhs, run for j descending.
+void solve_upper_step(hipStream_t s, const double *L, long lda, double *rhs, double *out, int n_pad, int j);
+
+// Solve L[r0:r0+64, r0:r0+64] X = B[r0:r0+64, 0:ncols] in place; if colsq
+// is not null, colsq[c] += sum over the 64 solved rows of X[.][c]^2.
+void trsm_left64(hipStream_t s, const double *L, long lda, double *B, long ldb, int r0, int ncols, double *colsq);
+
+// var[j] = v - colsq[j]
+void variance_finish(hipStream_t s, const double *colsq, double v, int m, double *var);
+
+// out[0] = y^T alpha, out[1] = sum_i log(L_ii^2), out[2] = alpha^T alpha, over i < n.
+void loss_terms(hipStream_t s, const double *L, long lda, const double *y, const double *alpha, int n, double *out);
+
+// Fill a device vector.
+void fill(hipStream_t s, double *p, long count, double value);
+// A[i][i] = value for i < n.
+void set_diagonal(hipStream_t s, double *A, long lda, int n, double value);
+// Store one int (used to reset the pivot-failure flag without a memset node).
+void set_int(hipStream_t s, int *p, int value);
+
+}  // namespace gpamd::k

@@ -1,0 +1,240 @@
+/*
+ * gprat_amd — C ABI of the MI355X (gfx950) backend for GPRat's exact-GP path.
+ *
+ * Every entry point takes plain host pointers and sizes, returns an int
+ * status (GPRAT_OK == 0) and leaves a thread-local message for
+ * gprat_last_error(). Device memory is owned by the library: a context owns
+ * one GPU's streams and a caching pool; a model owns a resident Cholesky
+ * factor. Calls on one context are serialised by the library; distinct
+ * contexts are independent.
+ *
+ * Conventions shared by all entry points (they restate the reference's):
+ *  - theta[3] = {lengthscale, vertical_lengthscale, noise_variance}.
+ *  - n = n_tiles * n_tile_size training points, m = m_tiles * m_tile_size
+ *    test points (integer products, as core/src/utils_c.cpp:8-52).
+ *  - `train_in` / `test_in` are 1-D series; sample i has the feature window
+ *    series[i .. i+n_reg-1] (core/src/cpu/gp_algorithms.cpp:11-28), so they
+ *    must hold at least n + n_reg - 1 (m + n_reg - 1) values.
+ *  - `train_out` holds at least n values; y_i = train_out[i].
+ *  - SE kernel v * exp(-0.5 / (l*l) * |x_i - x_j|^2), noise added on the
+ *    training diagonal only; predictive variance is latent (no noise).
+ *  - All arithmetic is FP64 unless a model is fitted with
+ *    GPRAT_PRECISION_MIXED.
+ */
+#ifndef GPRAT_AMD_H
+#define GPRAT_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. GPRAT_ERR_NOT_SPD means the covariance was not positive
+ * definite: the reference ignores this (core/src/cpu/adapter_cblas_fp64.cpp:19);
+ * here it is reported, with the first failing column in the message. */
+enum
+{
+    GPRAT_OK = 0,
+    GPRAT_ERR_ARGUMENT = 1,
+    GPRAT_ERR_HIP = 2,
+    GPRAT_ERR_OUT_OF_MEMORY = 3,
+    GPRAT_ERR_NOT_SPD = 4,
+    GPRAT_ERR_NO_DEVICE = 5,
+    GPRAT_ERR_INTERNAL = 6
+};
+
+/* Factorisation precision for resident models. */
+enum
+{
+    GPRAT_PRECISION_FP64 = 0,
+    /* FP64 panel (POTRF/TRSM) and log-det, FP32-MFMA trailing updates,
+     * FP64 iterative refinement of alpha and of the variance solves. */
+    GPRAT_PRECISION_MIXED = 1
+};
+
+typedef struct gprat_context gprat_context;
+typedef struct gprat_model gprat_model;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *gprat_last_error(void);
+
+/* Library version string. */
+const char *gprat_version(void);
+
+/* Replaces gprat::gpu_count (core/include/target.hpp:202-204,
+ * core/src/target.cpp:127-136). */
+int gprat_device_count(int *count);
+
+/* One-line description of a device; replaces the per-device text of
+ * gprat::print_available_gpus (core/src/target.cpp:112-125). */
+int gprat_device_describe(int device, char *buf, size_t buf_len);
+
+/* Replaces gprat::CUDA_GPU(id, n_streams) plus CUDA_GPU::create/destroy
+ * (core/include/target.hpp:112-180, core/src/target.cpp:24-72). n_streams
+ * >= 1; the library uses at least two streams internally for look-ahead. */
+int gprat_context_create(int device, int n_streams, gprat_context **out);
+int gprat_context_destroy(gprat_context *ctx);
+
+/* Device buffers for callers without their own allocator. Every pointer
+ * argument of the entry points below may be host memory or device memory of
+ * the context's GPU (HIP unified addressing): with device pointers the
+ * inputs never cross PCIe. */
+int gprat_device_alloc(gprat_context *ctx, size_t bytes, void **ptr);
+int gprat_device_free(gprat_context *ctx, void *ptr);
+/* Synchronous copy; either side may be host or device memory. */
+int gprat_memcpy(gprat_context *ctx, void *dst, const void *src, size_t bytes);
+
+/* Wait for all work of the context (no-op for the synchronous calls below,
+ * which return after their results are in host memory). */
+int gprat_context_synchronize(gprat_context *ctx);
+
+/* Replaces gpu::cholesky (core/include/gpu/gp_functions.cuh:189-195).
+ * tiles_out receives the n_tiles*(n_tiles+1)/2 lower tiles (i, j<=i) in
+ * row-major tile order, each n_tile_size^2 values row-major. Diagonal tiles
+ * hold L in their lower triangle and the covariance K in their strict upper
+ * triangle, as the reference's POTRF leaves them. */
+int gprat_cholesky(gprat_context *ctx,
+                   const double *train_in,
+                   size_t train_in_len,
+                   const double theta[3],
+                   int n_tiles,
+                   int n_tile_size,
+                   int n_reg,
+                   double *tiles_out);
+
+/* Replaces gpu::predict (core/include/gpu/gp_functions.cuh:27-37).
+ * mean_out has m values. */
+int gprat_predict(gprat_context *ctx,
+                  const double *train_in,
+                  size_t train_in_len,
+                  const double *train_out,
+                  size_t train_out_len,
+                  const double *test_in,
+                  size_t test_in_len,
+                  const double theta[3],
+                  int n_tiles,
+                  int n_tile_size,
+                  int m_tiles,
+                  int m_tile_size,
+                  int n_reg,
+                  double *mean_out);
+
+/* Replaces gpu::predict_with_uncertainty (gp_functions.cuh:55-65).
+ * mean_out and var_out have m values each. */
+int gprat_predict_with_uncertainty(gprat_context *ctx,
+                                   const double *train_in,
+                                   size_t train_in_len,
+                                   const double *train_out,
+                                   size_t train_out_len,
+                                   const double *test_in,
+                                   size_t test_in_len,
+                                   const double theta[3],
+                                   int n_tiles,
+                                   int n_tile_size,
+                                   int m_tiles,
+                                   int m_tile_size,
+                                   int n_reg,
+                                   double *mean_out,
+                                   double *var_out);
+
+/* Replaces gpu::predict_with_full_cov (gp_functions.cuh:83-93). Like the
+ * reference it returns the mean and the diagonal of the posterior
+ * covariance (core/src/cpu/gp_functions.cpp:663-695). */
+int gprat_predict_with_full_cov(gprat_context *ctx,
+                                const double *train_in,
+                                size_t train_in_len,
+                                const double *train_out,
+                                size_t train_out_len,
+                                const double *test_in,
+                                size_t test_in_len,
+                                const double theta[3],
+                                int n_tiles,
+                                int n_tile_size,
+                                int m_tiles,
+                                int m_tile_size,
+                                int n_reg,
+                                double *mean_out,
+                                double *var_out);
+
+/* Replaces gpu::compute_loss (gp_functions.cuh:108-114): the normalised
+ * negative log marginal likelihood
+ * 0.5 * (y^T K^-1 y + sum_i log(L_ii^2) + n log(2 pi)) / n. */
+int gprat_compute_loss(gprat_context *ctx,
+                       const double *train_in,
+                       size_t train_in_len,
+                       const double *train_out,
+                       size_t train_out_len,
+                       const double theta[3],
+                       int n_tiles,
+                       int n_tile_size,
+                       int n_reg,
+                       double *loss_out);
+
+/* The device half of one optimisation step, replacing the loss and
+ * gradient part of cpu::optimize_step (core/src/cpu/gp_functions.cpp:1032-1269,
+ * core/src/cpu/tiled_algorithms.cpp:296-426), which the reference GPU path
+ * lacks (core/src/gpu/gp_functions.cu:395-423). Writes the loss at theta and,
+ * for each p with trainable[p] != 0, grad_out[p] = the gradient the
+ * reference feeds to Adam: d loss / d u_p in softplus space
+ * (u = log(exp(theta) - 1), noise shifted by 1e-6). Entries with
+ * trainable[p] == 0 are set to 0. The Adam update itself stays on the host. */
+int gprat_loss_gradient(gprat_context *ctx,
+                        const double *train_in,
+                        size_t train_in_len,
+                        const double *train_out,
+                        size_t train_out_len,
+                        const double theta[3],
+                        int n_tiles,
+                        int n_tile_size,
+                        int n_reg,
+                        const int trainable[3],
+                        double *loss_out,
+                        double grad_out[3]);
+
+/* ---- Resident models (fit once, predict many) -------------------------- */
+
+/* Assemble K, factor it and solve alpha = K^-1 y; the factor stays in HBM.
+ * n training points (any n >= 1; no tiling constraint). */
+int gprat_model_fit(gprat_context *ctx,
+                    const double *train_in,
+                    size_t train_in_len,
+                    const double *train_out,
+                    size_t train_out_len,
+                    int n,
+                    int n_reg,
+                    const double theta[3],
+                    int precision,
+                    gprat_model **out);
+
+/* Predictive mean (and latent variance if var_out != NULL) at m test points. */
+int gprat_model_predict(gprat_model *model,
+                        const double *test_in,
+                        size_t test_in_len,
+                        int m,
+                        double *mean_out,
+                        double *var_out);
+
+/* Loss at the fitted hyperparameters. */
+int gprat_model_loss(gprat_model *model, double *loss_out);
+
+/* Device bytes held by the model. */
+int gprat_model_bytes(const gprat_model *model, size_t *bytes);
+
+int gprat_model_release(gprat_model *model);
+
+/* ---- Measurement hooks ------------------------------------------------- */
+
+/* When enabled, the library brackets every launch of its named kernels with
+ * HIP events on the launching stream and accumulates their durations. */
+int gprat_context_set_profiling(gprat_context *ctx, int enable);
+/* Launch count and summed duration (ms) for one kernel family since
+ * profiling was enabled. Families: "syrk_trailing", "gemm_panel",
+ * "potrf_trsm", "assemble", "trsm", "solve", "mean". */
+int gprat_context_kernel_stats(gprat_context *ctx, const char *family, long *launches, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPRAT_AMD_H */
