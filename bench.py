@@ -119,6 +119,47 @@ def cholesky_rate(capi, ctx, n=32768, repeats=2):
     return {"n": n, "ms_best": best, "ms_mean": float(np.mean(times)), "tflops": flops / (best * 1e-3) / 1e12}
 
 
+def mixed_leg(capi, ctx, n=32768, m=8192, r=16, noise=0.05, repeats=2):
+    """Mixed-precision configuration: n=32768 (32 tiles of 1024), n_reg 16,
+    m=8192, noise 0.05. Times one fit+predict request in FP64 and in mixed
+    precision (FP32 matrix-core trailing updates + FP64 refinement of
+    alpha) and reports the mixed result's error against the FP64 one."""
+    from gprat_amd import datagen
+    x, y, z = datagen.gp_problem(n, m, r)
+    theta = (1.0, 1.0, noise)
+    dx, dy, dz = ctx.device_array(x), ctx.device_array(y), ctx.device_array(z)
+    out = {}
+    res = {}
+    for name, prec in (("fp64", capi.PRECISION_FP64), ("mixed", capi.PRECISION_MIXED)):
+        times = []
+        for rep in range(repeats + 1):
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            mdl = capi.Model(ctx, dx, dy, n, r, theta, prec)
+            mean, var = mdl.predict(dz, m)
+            ctx.synchronize()
+            dt = time.perf_counter() - t0
+            if rep > 0:
+                times.append(dt)
+            if rep == repeats:
+                res[name] = (mean, var, mdl.loss(), mdl.refinement())
+            mdl.release()
+        out[name + "_ms"] = 1e3 * min(times)
+    m64, v64, l64, _ = res["fp64"]
+    mmx, vmx, lmx, (it, last) = res["mixed"]
+    out["speedup"] = out["fp64_ms"] / out["mixed_ms"]
+    out["rel_err_vs_fp64"] = {
+        "mean": float(np.max(np.abs(mmx - m64)) / np.max(np.abs(m64))),
+        "var": float(np.max(np.abs(vmx - v64)) / np.max(np.abs(v64))),
+        "loss": abs(lmx - l64) / abs(l64),
+    }
+    out["refinement_steps"] = it
+    out["workload"] = f"fit+predict n={n} (32 tiles of 1024), n_reg={r}, m={m}, noise={noise}"
+    for a in (dx, dy, dz):
+        a.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +167,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--chol-n", type=int, default=32768, help="size of the Cholesky-rate leg (0 skips it)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this backend runs on one GPU; the GP path has no multi-GPU mode in this build "
@@ -159,7 +201,7 @@ def main():
     ctx.synchronize()
     total = time.perf_counter() - t0
     fams = {f: ctx.kernel_stats(f) for f in
-            ("syrk_trailing", "potrf_trsm", "cholesky", "assemble", "solve", "trsm", "mean")}
+            ("syrk_trailing", "panel", "cholesky", "assemble", "solve", "trsm", "mean")}
     ctx.set_profiling(False)
     gpu_mean, gpu_var = dmean.numpy(), dvar.numpy()
 
@@ -186,6 +228,7 @@ def main():
     chol = None
     if args.chol_n > 0:
         chol = cholesky_rate(capi, ctx, args.chol_n)
+    mixed = None if args.no_mixed else mixed_leg(capi, ctx)
 
     result = {
         "metric": METRIC,
@@ -210,6 +253,7 @@ def main():
         "cholesky_n16384_ms": chol_ms,
         "cholesky_n16384_tflops": (n ** 3 / 3.0) / (chol_ms * 1e-3) / 1e12 if chol_ms > 0 else None,
         "cholesky_n32k": chol,
+        "mixed_precision_n32k": mixed,
         "phase_ms_per_step": {k: v[1] / args.steps for k, v in fams.items()},
         "roofline": roofline,
     }

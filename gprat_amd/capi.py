@@ -45,6 +45,7 @@ SIGNATURES = {
     "gprat_model_predict": [_VP, _D, _SZ, C.c_int, _D, _D],
     "gprat_model_loss": [_VP, _D],
     "gprat_model_bytes": [_VP, C.POINTER(_SZ)],
+    "gprat_model_refinement": [_VP, _I, _D],
     "gprat_model_release": [_VP],
     "gprat_context_set_profiling": [_VP, C.c_int],
     "gprat_context_kernel_stats": [_VP, C.c_char_p, C.POINTER(C.c_long), _D],
@@ -260,6 +261,13 @@ class Model:
         out = C.c_double(0.0)
         _check(lib().gprat_model_loss(self.handle, C.byref(out)))
         return out.value
+
+    def refinement(self):
+        """(iterations, last relative correction) of the alpha refinement."""
+        it = C.c_int(0)
+        last = C.c_double(0.0)
+        _check(lib().gprat_model_refinement(self.handle, C.byref(it), C.byref(last)))
+        return it.value, last.value
 
     def nbytes(self):
         b = _SZ(0)

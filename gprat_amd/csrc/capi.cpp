@@ -426,6 +426,18 @@ int gprat_model_loss(gprat_model *model, double *loss_out)
         });
 }
 
+int gprat_model_refinement(const gprat_model *model, int *iterations, double *last_correction)
+{
+    return guarded(
+        [&]
+        {
+            gpamd::require(model != nullptr && iterations != nullptr && last_correction != nullptr, "null argument");
+            const auto *md = reinterpret_cast<const gpamd::Model *>(model);
+            *iterations = md->refine_iterations;
+            *last_correction = md->refine_last;
+        });
+}
+
 int gprat_model_bytes(const gprat_model *model, size_t *bytes)
 {
     return guarded(

@@ -241,6 +241,16 @@ long leading_dim(long n_pad)
     return n_pad;
 }
 
+// Look-ahead schedule (panel k+1 on its own stream while the trailing
+// update of step k runs) or a single-stream schedule; GPRAT_LOOKAHEAD=0/1
+// overrides the default.
+static bool use_lookahead()
+{
+    if (const char *e = std::getenv("GPRAT_LOOKAHEAD"))
+        return std::atoi(e) != 0;
+    return true;
+}
+
 static int outer_block(int n_pad)
 {
     if (const char *e = std::getenv("GPRAT_CHOLESKY_BLOCK"))
@@ -260,13 +270,22 @@ static int outer_block(int n_pad)
 // On the `main` stream the trailing update of step k first refreshes panel
 // k+1 only, releases it to the panel stream, and then updates the remaining
 // lower trailing matrix while panel k+1 is being factored.
-void cholesky_fp64(Context &c, double *A, long lda, int n_pad)
+void cholesky_fp64(Context &c, double *A, long lda, int n_pad, double *Winv) { cholesky(c, A, lda, n_pad, Winv, false); }
+
+void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mixed)
 {
     require(n_pad % 128 == 0 && lda >= n_pad && lda % 2 == 0, "cholesky: bad layout");
-    hipStream_t U = c.main, P = c.panel;
+    hipStream_t U = c.main, P = use_lookahead() ? c.panel : c.main;
     const int Bo = outer_block(n_pad);
     const int nblk = static_cast<int>(ceil_div(n_pad, Bo));
     k::set_int(U, c.info.get(), INT_MAX);
+    // Mixed precision: FP32 k-major copies of the factored panel's trailing
+    // rows, double buffered so panel k+1 can be converted while the trailing
+    // update of step k still reads panel k.
+    DevBuf<float> p32[2];
+    if (mixed)
+        for (auto &b : p32)
+            b = DevBuf<float>(c, static_cast<std::size_t>(Bo) * n_pad);
 
     std::vector<hipEvent_t> used;
     auto ev = [&]()
@@ -286,13 +305,33 @@ void cholesky_fp64(Context &c, double *A, long lda, int n_pad)
         const int p0 = kb * Bo;
         const int pe = std::min(n_pad, p0 + Bo);
         {
-            Timed t(c, P, "potrf_trsm");
+            Timed t(c, P, "panel");
             for (int c0 = p0; c0 < pe; c0 += 64)
             {
-                k::potrf_trsm64(P, A, lda, c0, n_pad, c.info.get());
+                double *w = Winv + static_cast<long>(c0 / 64) * 64 * 64;
+                k::potrf64_inv(P, A, lda, c0, w, c.info.get());
+                if (c0 + 64 >= n_pad)
+                    continue;
+                const long o = static_cast<long>(c0 + 64) * lda;
+                {
+                    // rows below: X = A W^T, in place
+                    k::GemmArgs g{};
+                    g.A = A + o + c0;
+                    g.lda = lda;
+                    g.B = w;
+                    g.ldb = 64;
+                    g.C = A + o + c0;
+                    g.ldc = lda;
+                    g.M = n_pad - c0 - 64;
+                    g.N = 64;
+                    g.K = 64;
+                    g.b_trans = true;
+                    g.set = true;
+                    g.role = k::GemmRole::PanelSolve;
+                    k::gemm_f64(P, g);
+                }
                 if (c0 + 64 < pe)
                 {
-                    const long o = static_cast<long>(c0 + 64) * lda;
                     k::GemmArgs g{};
                     g.A = A + o + c0;
                     g.lda = lda;
@@ -311,6 +350,8 @@ void cholesky_fp64(Context &c, double *A, long lda, int n_pad)
                 }
             }
         }
+        if (mixed && pe < n_pad)
+            k::panel_to_f32t(P, A, lda, pe, p0, n_pad - pe, pe - p0, p32[kb & 1].get(), n_pad);
         hipEvent_t panel_done = ev();
         GP_HIP(hipEventRecord(panel_done, P));
         if (pe >= n_pad)
@@ -320,6 +361,13 @@ void cholesky_fp64(Context &c, double *A, long lda, int n_pad)
         }
         GP_HIP(hipStreamWaitEvent(U, panel_done, 0));
         const int q1 = std::min(n_pad, pe + Bo);
+        if (mixed)
+        {
+            Timed t(c, U, "syrk_trailing");
+            k::gemm_f32_trailing(U, p32[kb & 1].get(), n_pad, 0, 0, A + static_cast<long>(pe) * lda + pe, lda, n_pad - pe,
+                                 q1 - pe, pe - p0, 0);
+        }
+        else
         {
             Timed t(c, U, "syrk_trailing");
             const long o = static_cast<long>(pe) * lda;
@@ -341,7 +389,13 @@ void cholesky_fp64(Context &c, double *A, long lda, int n_pad)
         hipEvent_t next_ready = ev();
         GP_HIP(hipEventRecord(next_ready, U));
         GP_HIP(hipStreamWaitEvent(P, next_ready, 0));
-        if (q1 < n_pad)
+        if (q1 < n_pad && mixed)
+        {
+            Timed t(c, U, "syrk_trailing");
+            k::gemm_f32_trailing(U, p32[kb & 1].get(), n_pad, q1 - pe, q1 - pe, A + static_cast<long>(q1) * lda + q1, lda,
+                                 n_pad - q1, n_pad - q1, pe - p0, 0);
+        }
+        else if (q1 < n_pad)
         {
             Timed t(c, U, "syrk_trailing");
             const long o = static_cast<long>(q1) * lda;
@@ -374,7 +428,7 @@ void cholesky_fp64(Context &c, double *A, long lda, int n_pad)
 }
 
 // alpha = K^-1 y by the blocked forward and backward solves.
-void solve_spd(Context &c, const double *L, long lda, int n_pad, const double *y, double *work, double *out)
+void solve_spd(Context &c, const double *L, long lda, const double *Winv, int n_pad, const double *y, double *work, double *out)
 {
     hipStream_t s = c.main;
     Timed t(c, s, "solve");
@@ -382,28 +436,42 @@ void solve_spd(Context &c, const double *L, long lda, int n_pad, const double *y
     // work <- y (rhs of the forward solve, updated in place)
     GP_HIP(hipMemcpyAsync(work, y, sizeof(double) * n_pad, hipMemcpyDeviceToDevice, s));
     for (int j = 0; j < nb; j++)
-        k::solve_lower_step(s, L, lda, work, out, n_pad, j);
+        k::solve_lower_step(s, L, lda, Winv, work, out, n_pad, j);
     // out holds L^-1 y; use it as the rhs of the backward solve, result in work
     for (int j = nb - 1; j >= 0; j--)
-        k::solve_upper_step(s, L, lda, out, work, n_pad, j);
+        k::solve_upper_step(s, L, lda, Winv, out, work, n_pad, j);
     GP_HIP(hipMemcpyAsync(out, work, sizeof(double) * n_pad, hipMemcpyDeviceToDevice, s));
 }
 
 // B[a:b, :ncols] <- L[a:b, a:b]^-1 B[a:b, :ncols] by recursive halving; the
 // off-diagonal coupling is one MFMA product per level.
-// With tri_rhs the right-hand side is lower triangular (B = I for L^-1), so
-// rows [a, b) only have non-zero columns below b and the column range of
-// every step is cut accordingly.
-void trsm_lower_left(Context &c, const double *L, long lda, double *B, long ldb, int ncols, double *colsq, int a, int b, bool tri_rhs)
+// B[a:b, :ncols] <- L[a:b, a:b]^-1 B[a:b, :ncols] by recursive halving: the
+// 64-row leaves multiply by the saved diagonal-block inverse (in place), the
+// coupling between halves is one MFMA product per level. With tri_rhs the
+// right-hand side is lower triangular (B = I for L^-1), so rows [a, b) only
+// have non-zero columns below b and every step's column range is cut to it.
+void trsm_lower_left(Context &c, const double *L, long lda, const double *Winv, double *B, long ldb, int ncols, int a, int b, bool tri_rhs)
 {
     if (b - a == 64)
     {
-        k::trsm_left64(c.main, L, lda, B, ldb, a, tri_rhs ? std::min(ncols, b) : ncols, colsq);
+        k::GemmArgs g{};
+        g.A = Winv + static_cast<long>(a / 64) * 64 * 64;
+        g.lda = 64;
+        g.B = B + static_cast<long>(a) * ldb;
+        g.ldb = ldb;
+        g.C = B + static_cast<long>(a) * ldb;
+        g.ldc = ldb;
+        g.M = 64;
+        g.N = tri_rhs ? std::min(ncols, b) : ncols;
+        g.K = 64;
+        g.set = true;
+        g.role = k::GemmRole::Solve;
+        k::gemm_f64(c.main, g);
         return;
     }
     const int blocks = (b - a) / 64;
     const int mid = a + 64 * ((blocks + 1) / 2);
-    trsm_lower_left(c, L, lda, B, ldb, ncols, colsq, a, mid, tri_rhs);
+    trsm_lower_left(c, L, lda, Winv, B, ldb, ncols, a, mid, tri_rhs);
     k::GemmArgs g{};
     g.A = L + static_cast<long>(mid) * lda + a;
     g.lda = lda;
@@ -416,7 +484,7 @@ void trsm_lower_left(Context &c, const double *L, long lda, double *B, long ldb,
     g.K = mid - a;
     g.role = k::GemmRole::Solve;
     k::gemm_f64(c.main, g);
-    trsm_lower_left(c, L, lda, B, ldb, ncols, colsq, mid, b, tri_rhs);
+    trsm_lower_left(c, L, lda, Winv, B, ldb, ncols, mid, b, tri_rhs);
 }
 
 // ---------------------------------------------------------------------------
@@ -434,7 +502,7 @@ void factor(Context &c, Model &m, const double *x, std::size_t x_len, int n, int
     require(r >= 1, "n_reg must be >= 1");
     require(x != nullptr && x_len >= static_cast<std::size_t>(n) + r - 1, "training input must hold n + n_reg - 1 values");
     require(theta != nullptr, "theta is null");
-    require(precision == GPRAT_PRECISION_FP64, "only GPRAT_PRECISION_FP64 is available in this build");
+    require(precision == GPRAT_PRECISION_FP64 || precision == GPRAT_PRECISION_MIXED, "unknown precision mode");
     c.make_current();
     m.ctx = &c;
     m.n = n;
@@ -446,12 +514,58 @@ void factor(Context &c, Model &m, const double *x, std::size_t x_len, int n, int
     const std::size_t xl = static_cast<std::size_t>(n) + r - 1;
     m.x = DevBuf<double>(c, xl);
     m.L = DevBuf<double>(c, static_cast<std::size_t>(m.n_pad) * m.lda);
+    m.Winv = DevBuf<double>(c, static_cast<std::size_t>(m.n_pad) * 64);
     upload(c, m.x.get(), x, xl);
     {
         Timed t(c, c.main, "assemble");
         k::assemble_lower(c.main, m.x.get(), r, m.sek, m.L.get(), m.lda, n, m.n_pad);
     }
-    cholesky_fp64(c, m.L.get(), m.lda, m.n_pad);
+    cholesky(c, m.L.get(), m.lda, m.n_pad, m.Winv.get(), precision == GPRAT_PRECISION_MIXED);
+}
+
+// FP64 iterative refinement of alpha against a factor of reduced accuracy:
+// r = y - K alpha with K regenerated from the series in FP64, delta from the
+// factor, alpha += delta, until the correction stops mattering
+// (max|delta| <= 1e-15 max|alpha|) or stops shrinking.
+void refine_alpha(Model &m, double *work)
+{
+    Context &c = *m.ctx;
+    hipStream_t s = c.main;
+    const int n = m.n;
+    const int splits = k::mean_splits(n, n);
+    DevBuf<double> partial(c, static_cast<std::size_t>(splits) * n);
+    DevBuf<double> kx(c, m.n_pad), r(c, m.n_pad), delta(c, m.n_pad);
+    k::fill(s, r.get(), m.n_pad, 0.0);
+    std::vector<double> hd(n), ha(n);
+    double prev = 1e300;
+    m.refine_iterations = 0;
+    for (int it = 0; it < 30; it++)
+    {
+        {
+            Timed t(c, s, "refine");
+            k::predict_mean(s, m.x.get(), m.x.get(), m.alpha.get(), m.r, m.sek, n, n, partial.get(), splits, kx.get());
+            k::residual_finish(s, m.y.get(), kx.get(), m.alpha.get(), m.sek.noise, n, r.get());
+        }
+        solve_spd(c, m.L.get(), m.lda, m.Winv.get(), m.n_pad, r.get(), work, delta.get());
+        k::add_vec(s, m.alpha.get(), delta.get(), n);
+        GP_HIP(hipMemcpyAsync(hd.data(), delta.get(), sizeof(double) * n, hipMemcpyDefault, s));
+        GP_HIP(hipMemcpyAsync(ha.data(), m.alpha.get(), sizeof(double) * n, hipMemcpyDefault, s));
+        c.sync();
+        double dmax = 0.0, amax = 0.0;
+        for (int i = 0; i < n; i++)
+        {
+            dmax = std::max(dmax, std::fabs(hd[i]));
+            amax = std::max(amax, std::fabs(ha[i]));
+        }
+        m.refine_iterations = it + 1;
+        m.refine_last = amax > 0.0 ? dmax / amax : 0.0;
+        if (!(dmax <= 1e-15 * amax) && dmax < 0.5 * prev)
+        {
+            prev = dmax;
+            continue;
+        }
+        break;
+    }
 }
 
 void attach_targets(Model &m, const double *y, std::size_t y_len)
@@ -464,7 +578,9 @@ void attach_targets(Model &m, const double *y, std::size_t y_len)
     DevBuf<double> work(c, m.n_pad);
     k::fill(c.main, m.y.get(), m.n_pad, 0.0);
     upload(c, m.y.get(), y, static_cast<std::size_t>(m.n));
-    solve_spd(c, m.L.get(), m.lda, m.n_pad, m.y.get(), work.get(), m.alpha.get());
+    solve_spd(c, m.L.get(), m.lda, m.Winv.get(), m.n_pad, m.y.get(), work.get(), m.alpha.get());
+    if (m.precision == GPRAT_PRECISION_MIXED)
+        refine_alpha(m, work.get());
     c.sync();
 }
 
@@ -516,11 +632,13 @@ void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean,
             Timed t(c, c.main, "assemble");
             k::assemble_cross_t(c.main, m.x.get(), dz.get(), m.r, m.sek, B.get(), ldb, m.n, m.n_pad, mt, m_pad);
         }
-        k::fill(c.main, colsq.get(), m_pad, 0.0);
         {
             Timed t(c, c.main, "trsm");
-            trsm_lower_left(c, m.L.get(), m.lda, B.get(), ldb, m_pad, colsq.get(), 0, m.n_pad, false);
+            trsm_lower_left(c, m.L.get(), m.lda, m.Winv.get(), B.get(), ldb, m_pad, 0, m.n_pad, false);
         }
+        const int splits = k::colsq_splits(m.n, mt);
+        DevBuf<double> cpart(c, static_cast<std::size_t>(splits) * mt);
+        k::column_sumsq(c.main, B.get(), ldb, m.n, mt, cpart.get(), splits, colsq.get());
         k::variance_finish(c.main, colsq.get(), m.sek.vertical, mt, dvar.get());
         GP_HIP(hipMemcpyAsync(var, dvar.get(), sizeof(double) * mt, hipMemcpyDefault, c.main));
     }
@@ -590,7 +708,7 @@ void loss_gradient(Context &c,
     k::set_diagonal(c.main, W.get(), m.lda, m.n_pad, 1.0);
     {
         Timed tm(c, c.main, "inverse");
-        trsm_lower_left(c, m.L.get(), m.lda, W.get(), m.lda, m.n_pad, nullptr, 0, m.n_pad, true);
+        trsm_lower_left(c, m.L.get(), m.lda, m.Winv.get(), W.get(), m.lda, m.n_pad, 0, m.n_pad, true);
         k::GemmArgs g{};
         g.A = W.get();
         g.lda = m.lda;

@@ -115,19 +115,25 @@ struct Model
     long lda = 0;
     SEK sek{};
     int precision = GPRAT_PRECISION_FP64;
+    int refine_iterations = 0;   // mixed precision: refinement steps taken for alpha
+    double refine_last = 0.0;    // last max|delta| / max|alpha|
     DevBuf<double> x;      // training series (n + r - 1)
     DevBuf<double> y;      // targets, zero padded to n_pad
     DevBuf<double> L;      // n_pad x lda, lower triangle holds the factor
+    DevBuf<double> Winv;   // inverses of the 64x64 diagonal blocks of L (n_pad/64 x 64 x 64)
     DevBuf<double> alpha;  // K^-1 y (n_pad)
-    std::size_t bytes() const { return x.bytes() + y.bytes() + L.bytes() + alpha.bytes(); }
+    std::size_t bytes() const { return x.bytes() + y.bytes() + L.bytes() + Winv.bytes() + alpha.bytes(); }
 };
 
 // Algorithms (engine.cpp)
 long padded(long n);
 long leading_dim(long n_pad);
-void cholesky_fp64(Context &c, double *A, long lda, int n_pad);
-void solve_spd(Context &c, const double *L, long lda, int n_pad, const double *y, double *work, double *out);
-void trsm_lower_left(Context &c, const double *L, long lda, double *B, long ldb, int ncols, double *colsq, int a, int b, bool tri_rhs);
+void cholesky_fp64(Context &c, double *A, long lda, int n_pad, double *Winv);
+// mixed: FP32 matrix-core trailing updates (FP64 panels and diagonal blocks).
+void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mixed);
+void refine_alpha(Model &m, double *work);
+void solve_spd(Context &c, const double *L, long lda, const double *Winv, int n_pad, const double *y, double *work, double *out);
+void trsm_lower_left(Context &c, const double *L, long lda, const double *Winv, double *B, long ldb, int ncols, int a, int b, bool tri_rhs);
 
 void factor(Context &c, Model &m, const double *x, std::size_t x_len, int n, int r, const double theta[3], int precision);
 void attach_targets(Model &m, const double *y, std::size_t y_len);

@@ -6,7 +6,9 @@
 // through a double-buffered LDS stage stored k-major ([k][row]), so each MFMA
 // operand is one ds_read_b64 per lane: lanes 0-15 read 16 consecutive rows
 // of k-slot 0, lanes 16-31 of k-slot 1, and so on. The LDS row stride of 144
-// doubles puts the four k-slots of a read in disjoint bank halves.
+// doubles puts the four k-slots of a read in disjoint bank halves. The
+// steady-state loop has no branch and no predicated load, so the
+// accumulators stay in place and two workgroups share a CU.
 //
 // The same kernel serves the Cholesky trailing update (A = B^T = panel,
 // lower triangle only), the panel-internal updates, the right-looking TRSM
@@ -42,65 +44,92 @@ struct Params
     int set;
 };
 
-// Global -> register staging of one BK-deep slab of a 128-wide operand.
-// `trans == false`: operand element (row, k) is P[row*ld + k] (rows contiguous in k).
-// `trans == true`:  operand element (row, k) is P[k*ld + row] (k-rows contiguous in row).
+// Per-thread geometry of one BK-deep slab of a 128-wide operand, fixed for
+// the whole K loop. `trans == false`: element (row, k) is P[row*ld + k]
+// (rows contiguous in k); `trans == true`: element (row, k) is P[k*ld + row].
+// Rows past the operand edge are clamped to its last row: they only feed
+// accumulator rows/columns that the epilogue never stores, so no load in the
+// loop is predicated.
 template <bool TRANS>
-__device__ __forceinline__ void load_slab(const double *__restrict__ P, long ld, long row0, int rows, long k0, int tid, double2 (&reg)[4])
+struct Slab
 {
-#pragma unroll
-    for (int s = 0; s < 4; s++)
+    unsigned off[4];  // element offsets from the slab origin (fit in 32 bits)
+    int lds[4];       // LDS element offsets of the first value
+
+    __device__ __forceinline__ void init(long ld, int rows, int tid)
     {
-        const int idx = tid + THREADS * s;
-        if (!TRANS)
+#pragma unroll
+        for (int s = 0; s < 4; s++)
         {
-            const int row = idx >> 3;        // 0..127
-            const int kk = (idx & 7) * 2;    // 0..14
-            if (row < rows)
-                reg[s] = *reinterpret_cast<const double2 *>(P + (row0 + row) * ld + k0 + kk);
+            const int idx = tid + THREADS * s;
+            if (!TRANS)
+            {
+                const int row = min(idx >> 3, rows - 1);
+                const int kk = (idx & 7) * 2;
+                off[s] = static_cast<unsigned>(row * ld + kk);
+                lds[s] = kk * LDS_STRIDE + (idx >> 3);
+            }
             else
-                reg[s] = make_double2(0.0, 0.0);
-        }
-        else
-        {
-            const int kk = idx >> 6;           // 0..15
-            const int col = (idx & 63) * 2;    // 0..126
-            if (col < rows)
-                reg[s] = *reinterpret_cast<const double2 *>(P + (k0 + kk) * ld + row0 + col);
-            else
-                reg[s] = make_double2(0.0, 0.0);
+            {
+                const int kk = idx >> 6;
+                const int col = min((idx & 63) * 2, rows - 2);
+                off[s] = static_cast<unsigned>(kk * ld + col);
+                lds[s] = kk * LDS_STRIDE + (idx & 63) * 2;
+            }
         }
     }
-}
 
-template <bool TRANS>
-__device__ __forceinline__ void store_slab(double (*S)[LDS_STRIDE], int tid, const double2 (&reg)[4])
+    __device__ __forceinline__ void load(const double *__restrict__ origin, double2 (&reg)[4]) const
+    {
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+            reg[s] = *reinterpret_cast<const double2 *>(origin + off[s]);
+    }
+
+    __device__ __forceinline__ void store(double *S, const double2 (&reg)[4]) const
+    {
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+        {
+            if (!TRANS)
+            {
+                S[lds[s]] = reg[s].x;
+                S[lds[s] + LDS_STRIDE] = reg[s].y;
+            }
+            else
+            {
+                *reinterpret_cast<double2 *>(S + lds[s]) = reg[s];
+            }
+        }
+    }
+};
+
+// 16 MFMAs per 4-deep k step, four k steps per slab.
+__device__ __forceinline__ void mfma_slab(const double *As, const double *Bs, int wr, int wc, int lm, int lk, v4d (&acc)[4][4])
 {
 #pragma unroll
-    for (int s = 0; s < 4; s++)
+    for (int kq = 0; kq < BK / 4; kq++)
     {
-        const int idx = tid + THREADS * s;
-        if (!TRANS)
-        {
-            const int row = idx >> 3;
-            const int kk = (idx & 7) * 2;
-            S[kk][row] = reg[s].x;
-            S[kk + 1][row] = reg[s].y;
-        }
-        else
-        {
-            const int kk = idx >> 6;
-            const int col = (idx & 63) * 2;
-            *reinterpret_cast<double2 *>(&S[kk][col]) = reg[s];
-        }
+        double a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            a[i] = As[(kq * 4 + lk) * LDS_STRIDE + wr * 64 + i * 16 + lm];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            b[j] = Bs[(kq * 4 + lk) * LDS_STRIDE + wc * 64 + j * 16 + lm];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
 }
 
 template <bool AT, bool BT, int ROLE>
-__global__ __launch_bounds__(THREADS) void k_gemm_f64(Params p)
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gemm_f64(Params p)
 {
-    __shared__ double As[2][BK][LDS_STRIDE];
-    __shared__ double Bs[2][BK][LDS_STRIDE];
+    __shared__ double As[2][BK * LDS_STRIDE];
+    __shared__ double Bs[2][BK * LDS_STRIDE];
 
     const long r0 = static_cast<long>(blockIdx.y) * BM;
     const long c0 = static_cast<long>(blockIdx.x) * BN;
@@ -130,46 +159,40 @@ __global__ __launch_bounds__(THREADS) void k_gemm_f64(Params p)
 
     if (kbeg < p.K)
     {
+        Slab<AT> sa;
+        Slab<!BT> sb;
+        sa.init(p.lda, rows_a, tid);
+        sb.init(p.ldb, rows_b, tid);
+        // slab origins: A(r0, k0) and B(k0, c0) in the operands' own layouts
+        const double *a0 = AT ? p.A + r0 : p.A + r0 * p.lda;
+        const double *b0 = BT ? p.B + c0 * p.ldb : p.B + c0;
+        const long a_step = AT ? BK * p.lda : BK;
+        const long b_step = BT ? BK : BK * p.ldb;
+        const double *ak = a0 + (AT ? kbeg * p.lda : kbeg);
+        const double *bk = b0 + (BT ? kbeg : kbeg * p.ldb);
+
         double2 ra[4], rb[4];
-        load_slab<AT>(p.A, p.lda, r0, rows_a, kbeg, tid, ra);
-        load_slab<!BT>(p.B, p.ldb, c0, rows_b, kbeg, tid, rb);
-        store_slab<AT>(As[0], tid, ra);
-        store_slab<!BT>(Bs[0], tid, rb);
+        sa.load(ak, ra);
+        sb.load(bk, rb);
+        sa.store(As[0], ra);
+        sb.store(Bs[0], rb);
         __syncthreads();
 
         int buf = 0;
-        for (long k0 = kbeg; k0 < p.K; k0 += BK)
+        const long steps = (p.K - kbeg) / BK;
+        for (long t = 1; t < steps; t++)
         {
-            const bool more = k0 + BK < p.K;
-            if (more)
-            {
-                load_slab<AT>(p.A, p.lda, r0, rows_a, k0 + BK, tid, ra);
-                load_slab<!BT>(p.B, p.ldb, c0, rows_b, k0 + BK, tid, rb);
-            }
-#pragma unroll
-            for (int kq = 0; kq < BK / 4; kq++)
-            {
-                double a[4], b[4];
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-                    a[i] = As[buf][kq * 4 + lk][wr * 64 + i * 16 + lm];
-#pragma unroll
-                for (int j = 0; j < 4; j++)
-                    b[j] = Bs[buf][kq * 4 + lk][wc * 64 + j * 16 + lm];
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-#pragma unroll
-                    for (int j = 0; j < 4; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
-            }
-            if (more)
-            {
-                store_slab<AT>(As[buf ^ 1], tid, ra);
-                store_slab<!BT>(Bs[buf ^ 1], tid, rb);
-            }
+            ak += a_step;
+            bk += b_step;
+            sa.load(ak, ra);
+            sb.load(bk, rb);
+            mfma_slab(As[buf], Bs[buf], wr, wc, lm, lk, acc);
+            sa.store(As[buf ^ 1], ra);
+            sb.store(Bs[buf ^ 1], rb);
             __syncthreads();
             buf ^= 1;
         }
+        mfma_slab(As[buf], Bs[buf], wr, wc, lm, lk, acc);
     }
 
     // Accumulator layout of the 16x16x4 f64 MFMA (measured on gfx950): lane
@@ -204,6 +227,8 @@ void gemm_f64(hipStream_t s, const GemmArgs &g)
         return;
     require(g.K >= 0 && g.K % BK == 0, "gemm_f64: K must be a multiple of 16");
     require(g.M % 2 == 0 && g.N % 2 == 0, "gemm_f64: M and N must be even");
+    // per-lane slab offsets are 32-bit element offsets
+    require(128L * std::max(g.lda, g.ldb) + 128 < (1L << 31), "gemm_f64: leading dimension too large");
     require(g.lda % 2 == 0 && g.ldb % 2 == 0, "gemm_f64: leading dimensions must be even");
     require(reinterpret_cast<uintptr_t>(g.A) % 16 == 0 && reinterpret_cast<uintptr_t>(g.B) % 16 == 0,
             "gemm_f64: operands must be 16-byte aligned");
@@ -214,6 +239,8 @@ void gemm_f64(hipStream_t s, const GemmArgs &g)
     // Instantiated combinations: (N,T) trailing/panel, (N,N) solve, (T,N) inverse.
     if (!g.a_trans && g.b_trans && g.role == GemmRole::Trailing)
         hipLaunchKernelGGL((k_gemm_f64<false, true, 0>), grid, dim3(THREADS), 0, s, p);
+    else if (!g.a_trans && g.b_trans && g.role == GemmRole::PanelSolve)
+        hipLaunchKernelGGL((k_gemm_f64<false, true, 4>), grid, dim3(THREADS), 0, s, p);
     else if (!g.a_trans && g.b_trans)
         hipLaunchKernelGGL((k_gemm_f64<false, true, 1>), grid, dim3(THREADS), 0, s, p);
     else if (!g.a_trans && !g.b_trans)

@@ -22,107 +22,136 @@ __device__ __forceinline__ void load_lower_block(const double *__restrict__ A, l
     }
 }
 
-// grid.x = 1 + ceil((row_end - c0 - NB) / THREADS). Every workgroup factors the
-// diagonal block in LDS (identical arithmetic, so identical results);
-// workgroup 0 stores L_cc, the others each solve THREADS panel rows
-// x L_cc^T = a, one row per thread held in registers.
-__global__ __launch_bounds__(THREADS) void k_potrf_trsm64(double *__restrict__ A, long lda, int c0, int row_end, int *__restrict__ info)
+// Cholesky factor and inverse of one 64x64 diagonal block, one workgroup.
+// Thread (wave g, lane c) keeps column c, rows g + 4q (q = 0..15) in
+// registers. Column j is published through LDS, every thread forms
+// l_jj = sqrt(d_jj) and 1/l_jj itself (identical arithmetic), scales its own
+// rows of column j and applies the rank-1 update to its column. The inverse
+// W = L^-1 is then built row by row (right-looking elimination of L W = I)
+// with the factor staged in LDS. L overwrites the block's lower triangle;
+// W (64x64 row-major, zero above the diagonal) goes to `w`.
+__global__ __launch_bounds__(THREADS) void k_potrf64_inv(double *__restrict__ A, long lda, int c0, double *__restrict__ w, int *__restrict__ info)
 {
-    __shared__ double D[NB][NB + 1];
+    __shared__ double col[NB];
+    __shared__ double Ls[NB][NB + 1];
     const int tid = threadIdx.x;
-    load_lower_block(A, lda, c0, D);
+    const int c = tid & (NB - 1), g = tid >> 6;
+    double *blk = A + static_cast<long>(c0) * lda + c0;
+    double d[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+        d[q] = blk[static_cast<long>(g + 4 * q) * lda + c];
+
+    for (int j = 0; j < NB; j++)
+    {
+        if (c == j)
+        {
+#pragma unroll
+            for (int q = 0; q < 16; q++)
+                col[g + 4 * q] = d[q];
+        }
+        __syncthreads();
+        const double djj = col[j];
+        if (tid == 0 && !(djj > 0.0))
+            atomicMin(info, c0 + j);
+        const double ljj = sqrt(djj);
+        const double inv = 1.0 / ljj;
+        const double lcj = (c > j) ? col[c] * inv : 0.0;
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+        {
+            const int r = g + 4 * q;
+            const double lrj = (r > j) ? col[r] * inv : (r == j ? ljj : 0.0);
+            if (c == j)
+                d[q] = lrj;
+            else if (c > j)
+                d[q] -= lrj * lcj;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+    {
+        const int r = g + 4 * q;
+        const double v = (r >= c) ? d[q] : 0.0;
+        Ls[r][c] = v;
+        if (r >= c)
+            blk[static_cast<long>(r) * lda + c] = v;
+    }
     __syncthreads();
-    for (int j = 0; j < NB; j++)
+
+    // W = L^-1: b[q] holds row g + 4q, column c of the running right-hand side.
+    double b[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+        b[q] = (g + 4 * q == c) ? 1.0 : 0.0;
+#pragma unroll
+    for (int i = 0; i < NB; i++)
     {
-        if (tid == 0)
+        const int qi = i >> 2, gi = i & 3;
+        if (g == gi)
         {
-            const double d = D[j][j];
-            if (!(d > 0.0) && blockIdx.x == 0)
-                atomicMin(info, c0 + j);
-            D[j][j] = sqrt(d);
+            b[qi] = b[qi] / Ls[i][i];
+            col[c] = b[qi];
         }
         __syncthreads();
-        if (tid > j && tid < NB)
-            D[tid][j] = D[tid][j] / D[j][j];
-        __syncthreads();
-        const int w = NB - 1 - j;
-        for (int e = tid; e < w * w; e += THREADS)
+        const double wic = col[c];
+#pragma unroll
+        for (int q = 0; q < 16; q++)
         {
-            const int rr = j + 1 + e / w, cc = j + 1 + e % w;
-            if (cc <= rr)
-                D[rr][cc] -= D[rr][j] * D[cc][j];
+            const int r = g + 4 * q;
+            if (r > i)
+                b[q] -= Ls[r][i] * wic;
         }
         __syncthreads();
     }
-    if (blockIdx.x == 0)
+#pragma unroll
+    for (int q = 0; q < 16; q++)
     {
-        for (int e = tid; e < NB * NB; e += THREADS)
-        {
-            const int r = e / NB, c = e % NB;
-            if (c <= r)
-                A[(static_cast<long>(c0) + r) * lda + c0 + c] = D[r][c];
-        }
-        return;
+        const int r = g + 4 * q;
+        w[r * NB + c] = (r >= c) ? b[q] : 0.0;
     }
-    const long row = static_cast<long>(c0) + NB + static_cast<long>(blockIdx.x - 1) * THREADS + tid;
-    if (row >= row_end)
-        return;
-    double *prow = A + row * lda + c0;
-    double xr[NB];
-#pragma unroll
-    for (int q = 0; q < NB / 2; q++)
-    {
-        const double2 t = reinterpret_cast<const double2 *>(prow)[q];
-        xr[2 * q] = t.x;
-        xr[2 * q + 1] = t.y;
-    }
-#pragma unroll
-    for (int j = 0; j < NB; j++)
-    {
-        double s = xr[j];
-#pragma unroll
-        for (int p = 0; p < j; p++)
-            s -= xr[p] * D[j][p];
-        xr[j] = s / D[j][j];
-    }
-#pragma unroll
-    for (int q = 0; q < NB / 2; q++)
-        reinterpret_cast<double2 *>(prow)[q] = make_double2(xr[2 * q], xr[2 * q + 1]);
 }
 
-// Forward block step J: every workgroup solves L_JJ x_J = rhs_J redundantly
-// (wavefront 0, lane i owns row i); workgroup 0 stores x_J, workgroup w >= 1
-// updates rhs_{J+w} -= L_{J+w,J} x_J.
+// x_J = W_JJ rhs_J (or W_JJ^T rhs_J), four column quarters per row combined
+// in a fixed order; every workgroup of a solve step computes it.
+template <bool TRANSPOSED>
+__device__ __forceinline__ void block_matvec(const double *__restrict__ W, const double *__restrict__ rhs, double *xs, double (*part)[NB])
+{
+    __shared__ double rs[NB];
+    const int tid = threadIdx.x;
+    if (tid < NB)
+        rs[tid] = rhs[tid];
+    __syncthreads();
+    const int i = tid & (NB - 1), q = tid >> 6;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; k++)
+    {
+        const int kk = q * 16 + k;
+        s += (TRANSPOSED ? W[kk * NB + i] : W[i * NB + kk]) * rs[kk];
+    }
+    part[q][i] = s;
+    __syncthreads();
+    if (tid < NB)
+        xs[tid] = (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
+    __syncthreads();
+}
+
+// Forward block step J of L x = rhs: x_J = W_JJ rhs_J (workgroup 0 stores it),
+// workgroup w >= 1 updates rhs_{J+w} -= L_{J+w,J} x_J.
 __global__ __launch_bounds__(THREADS) void k_solve_lower_step(const double *__restrict__ L,
                                                                long lda,
+                                                               const double *__restrict__ Winv,
                                                                double *__restrict__ rhs,
                                                                double *__restrict__ out,
                                                                int J)
 {
-    __shared__ double D[NB][NB + 1];
     __shared__ double xs[NB];
     __shared__ double part[4][NB];
     const int tid = threadIdx.x;
     const long base = static_cast<long>(J) * NB;
-    load_lower_block(L, lda, base, D);
-    if (tid < NB)
-        xs[tid] = rhs[base + tid];
-    __syncthreads();
-    if (tid < NB)
-    {
-        double r = xs[tid];
-        double mine = 0.0;
-        for (int p = 0; p < NB; p++)
-        {
-            const double xp = __shfl(r, p) / D[p][p];
-            if (tid == p)
-                mine = xp;
-            if (tid > p)
-                r -= D[tid][p] * xp;
-        }
-        xs[tid] = mine;
-    }
-    __syncthreads();
+    block_matvec<false>(Winv + base * NB, rhs + base, xs, part);
     if (blockIdx.x == 0)
     {
         if (tid < NB)
@@ -142,37 +171,20 @@ __global__ __launch_bounds__(THREADS) void k_solve_lower_step(const double *__re
         rhs[I * NB + tid] -= (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
 }
 
-// Backward block step J: x_J = L_JJ^-T rhs_J, then rhs_{J-w} -= L_{J,J-w}^T x_J.
+// Backward block step J of L^T x = rhs: x_J = W_JJ^T rhs_J, then
+// rhs_{J-w} -= L_{J,J-w}^T x_J.
 __global__ __launch_bounds__(THREADS) void k_solve_upper_step(const double *__restrict__ L,
                                                                long lda,
+                                                               const double *__restrict__ Winv,
                                                                double *__restrict__ rhs,
                                                                double *__restrict__ out,
                                                                int J)
 {
-    __shared__ double D[NB][NB + 1];
     __shared__ double xs[NB];
     __shared__ double part[4][NB];
     const int tid = threadIdx.x;
     const long base = static_cast<long>(J) * NB;
-    load_lower_block(L, lda, base, D);
-    if (tid < NB)
-        xs[tid] = rhs[base + tid];
-    __syncthreads();
-    if (tid < NB)
-    {
-        double r = xs[tid];
-        double mine = 0.0;
-        for (int p = NB - 1; p >= 0; p--)
-        {
-            const double xp = __shfl(r, p) / D[p][p];
-            if (tid == p)
-                mine = xp;
-            if (tid < p)
-                r -= D[p][tid] * xp;
-        }
-        xs[tid] = mine;
-    }
-    __syncthreads();
+    block_matvec<true>(Winv + base * NB, rhs + base, xs, part);
     if (blockIdx.x == 0)
     {
         if (tid < NB)
@@ -191,46 +203,38 @@ __global__ __launch_bounds__(THREADS) void k_solve_upper_step(const double *__re
         rhs[I * NB + tid] -= (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
 }
 
-// Solve L_rr X = B[r0:r0+64, :] in place, one right-hand-side column per
-// thread held in registers; optionally accumulate the column sums of X^2.
-__global__ __launch_bounds__(THREADS) void k_trsm_left64(const double *__restrict__ L,
-                                                          long lda,
-                                                          double *__restrict__ B,
-                                                          long ldb,
-                                                          int r0,
-                                                          int ncols,
-                                                          double *__restrict__ colsq)
+// colsq[j] over rows [0, n) of V (row-major, ldv): grid (ceil(m/64), splits),
+// lane = column, the four waves stride over rows; partials combined in a
+// fixed order per split, then across splits by k_colsq_reduce.
+__global__ __launch_bounds__(THREADS) void k_colsq_partial(const double *__restrict__ V, long ldv, int n, int m, int chunk, double *__restrict__ part)
 {
-    __shared__ double D[NB][NB + 1];
-    load_lower_block(L, lda, r0, D);
+    __shared__ double red[THREADS];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long j = static_cast<long>(blockIdx.x) * 64 + lane;
+    const long i0 = static_cast<long>(blockIdx.y) * chunk;
+    const long i1 = min(static_cast<long>(n), i0 + chunk);
+    double s = 0.0;
+    if (j < m)
+        for (long i = i0 + w; i < i1; i += 4)
+        {
+            const double v = V[i * ldv + j];
+            s += v * v;
+        }
+    red[threadIdx.x] = s;
     __syncthreads();
-    const long c = static_cast<long>(blockIdx.x) * THREADS + threadIdx.x;
-    if (c >= ncols)
+    if (w == 0 && j < m)
+        part[static_cast<long>(blockIdx.y) * m + j] = (red[lane] + red[64 + lane]) + (red[128 + lane] + red[192 + lane]);
+}
+
+__global__ __launch_bounds__(THREADS) void k_colsq_reduce(const double *__restrict__ part, int splits, int m, double *__restrict__ colsq)
+{
+    const long j = static_cast<long>(blockIdx.x) * THREADS + threadIdx.x;
+    if (j >= m)
         return;
-    double xv[NB];
-#pragma unroll
-    for (int i = 0; i < NB; i++)
-        xv[i] = B[(static_cast<long>(r0) + i) * ldb + c];
-#pragma unroll
-    for (int i = 0; i < NB; i++)
-    {
-        double s = xv[i];
-#pragma unroll
-        for (int p = 0; p < i; p++)
-            s -= D[i][p] * xv[p];
-        xv[i] = s / D[i][i];
-    }
-#pragma unroll
-    for (int i = 0; i < NB; i++)
-        B[(static_cast<long>(r0) + i) * ldb + c] = xv[i];
-    if (colsq)
-    {
-        double acc = 0.0;
-#pragma unroll
-        for (int i = 0; i < NB; i++)
-            acc += xv[i] * xv[i];
-        colsq[c] += acc;
-    }
+    double s = 0.0;
+    for (int q = 0; q < splits; q++)
+        s += part[static_cast<long>(q) * m + j];
+    colsq[j] = s;
 }
 
 __global__ void k_variance_finish(const double *__restrict__ colsq, double v, int m, double *__restrict__ var)
@@ -288,6 +292,20 @@ __global__ void k_set_diagonal(double *__restrict__ A, long lda, int n, double v
         A[i * lda + i] = value;
 }
 
+__global__ void k_residual_finish(const double *__restrict__ y, const double *__restrict__ kx, const double *__restrict__ alpha, double noise, int n, double *__restrict__ r)
+{
+    const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        r[i] = y[i] - kx[i] - noise * alpha[i];
+}
+
+__global__ void k_add_vec(double *__restrict__ y, const double *__restrict__ x, long n)
+{
+    const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        y[i] += x[i];
+}
+
 __global__ void k_fill(double *__restrict__ p, long count, double value)
 {
     const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -299,37 +317,46 @@ __global__ void k_set_int(int *p, int value) { *p = value; }
 
 }  // namespace
 
-void potrf_trsm64(hipStream_t s, double *A, long lda, int c0, int row_end, int *info)
+void potrf64_inv(hipStream_t s, double *A, long lda, int c0, double *w, int *info)
 {
-    require(c0 >= 0 && c0 + NB <= row_end && lda % 2 == 0, "potrf_trsm64: bad shape");
-    const long below = row_end - c0 - NB;
-    const unsigned blocks = static_cast<unsigned>(1 + ceil_div(below, THREADS));
-    hipLaunchKernelGGL(k_potrf_trsm64, dim3(blocks), dim3(THREADS), 0, s, A, lda, c0, row_end, info);
+    require(c0 >= 0 && c0 % NB == 0 && lda % 2 == 0, "potrf64_inv: bad shape");
+    hipLaunchKernelGGL(k_potrf64_inv, dim3(1), dim3(THREADS), 0, s, A, lda, c0, w, info);
     GP_CHECK_LAUNCH();
 }
 
-void solve_lower_step(hipStream_t s, const double *L, long lda, double *rhs, double *out, int n_pad, int j)
+void solve_lower_step(hipStream_t s, const double *L, long lda, const double *Winv, double *rhs, double *out, int n_pad, int j)
 {
     const int nb = n_pad / NB;
     require(n_pad % NB == 0 && j >= 0 && j < nb, "solve_lower_step: bad shape");
-    hipLaunchKernelGGL(k_solve_lower_step, dim3(nb - j), dim3(THREADS), 0, s, L, lda, rhs, out, j);
+    hipLaunchKernelGGL(k_solve_lower_step, dim3(nb - j), dim3(THREADS), 0, s, L, lda, Winv, rhs, out, j);
     GP_CHECK_LAUNCH();
 }
 
-void solve_upper_step(hipStream_t s, const double *L, long lda, double *rhs, double *out, int n_pad, int j)
+void solve_upper_step(hipStream_t s, const double *L, long lda, const double *Winv, double *rhs, double *out, int n_pad, int j)
 {
     const int nb = n_pad / NB;
     require(n_pad % NB == 0 && j >= 0 && j < nb, "solve_upper_step: bad shape");
-    hipLaunchKernelGGL(k_solve_upper_step, dim3(j + 1), dim3(THREADS), 0, s, L, lda, rhs, out, j);
+    hipLaunchKernelGGL(k_solve_upper_step, dim3(j + 1), dim3(THREADS), 0, s, L, lda, Winv, rhs, out, j);
     GP_CHECK_LAUNCH();
 }
 
-void trsm_left64(hipStream_t s, const double *L, long lda, double *B, long ldb, int r0, int ncols, double *colsq)
+int colsq_splits(int n, int m)
 {
-    require(r0 >= 0 && ncols >= 0, "trsm_left64: bad shape");
-    if (ncols == 0)
+    const long col_blocks = ceil_div(m, 64);
+    long splits = ceil_div(1024, col_blocks);
+    splits = std::min<long>(splits, ceil_div(n, 256));
+    return static_cast<int>(std::max<long>(splits, 1));
+}
+
+void column_sumsq(hipStream_t s, const double *V, long ldv, int n, int m, double *part, int splits, double *colsq)
+{
+    if (m <= 0)
         return;
-    hipLaunchKernelGGL(k_trsm_left64, dim3(ceil_div(ncols, THREADS)), dim3(THREADS), 0, s, L, lda, B, ldb, r0, ncols, colsq);
+    require(n >= 1 && splits >= 1, "column_sumsq: bad shape");
+    const int chunk = static_cast<int>(ceil_div(n, splits));
+    hipLaunchKernelGGL(k_colsq_partial, dim3(ceil_div(m, 64), splits), dim3(THREADS), 0, s, V, ldv, n, m, chunk, part);
+    GP_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_colsq_reduce, dim3(ceil_div(m, THREADS)), dim3(THREADS), 0, s, part, splits, m, colsq);
     GP_CHECK_LAUNCH();
 }
 
@@ -352,6 +379,22 @@ void set_diagonal(hipStream_t s, double *A, long lda, int n, double value)
     if (n <= 0)
         return;
     hipLaunchKernelGGL(k_set_diagonal, dim3(ceil_div(n, 256)), dim3(256), 0, s, A, lda, n, value);
+    GP_CHECK_LAUNCH();
+}
+
+void residual_finish(hipStream_t s, const double *y, const double *kx, const double *alpha, double noise, int n, double *r)
+{
+    if (n <= 0)
+        return;
+    hipLaunchKernelGGL(k_residual_finish, dim3(ceil_div(n, 256)), dim3(256), 0, s, y, kx, alpha, noise, n, r);
+    GP_CHECK_LAUNCH();
+}
+
+void add_vec(hipStream_t s, double *y, const double *x, long n)
+{
+    if (n <= 0)
+        return;
+    hipLaunchKernelGGL(k_add_vec, dim3(ceil_div(n, 256)), dim3(256), 0, s, y, x, n);
     GP_CHECK_LAUNCH();
 }
 

@@ -77,6 +77,7 @@ enum class GemmRole
     Panel,     // Cholesky panel-internal update
     Solve,     // TRSM coupling update
     Inverse,   // W^T W
+    PanelSolve,  // panel rows times the diagonal-block inverse
 };
 
 struct GemmArgs
@@ -107,29 +108,45 @@ struct GemmArgs
 };
 void gemm_f64(hipStream_t s, const GemmArgs &g);
 
+// ---- FP32 matrix-core trailing update (k_gemm32.hip) -----------------------
+
+// P[k][r] = (float) A[(r0 + r) * lda + p0 + k], r < R, k < W (k-major copy).
+void panel_to_f32t(hipStream_t s, const double *A, long lda, long r0, long p0, int R, int W, float *P, long ldp);
+// C[i][j] -= sum_k P[k][a_off + i] P[k][b_off + j] (FP32 products and sums),
+// for i < M, j < N, i + diag_off >= j; K % 32 == 0.
+void gemm_f32_trailing(hipStream_t s, const float *P, long ldp, long a_off, long b_off, double *C, long ldc, int M, int N, int K, long diag_off);
+
 // ---- triangular family (k_triangular.hip) ---------------------------------
 
-// Factor the 64x64 diagonal block at (c0, c0) and solve the panel rows
-// [c0+64, row_end) of columns [c0, c0+64) against it. info (device int) takes
-// the smallest failing global column if a pivot is not positive.
-void potrf_trsm64(hipStream_t s, double *A, long lda, int c0, int row_end, int *info);
+// Factor the 64x64 diagonal block at (c0, c0) in place (lower triangle) and
+// write its inverse W = L_cc^-1 (64x64 row-major, zero upper) to w. info
+// (device int) takes the smallest global column whose pivot is not positive.
+void potrf64_inv(hipStream_t s, double *A, long lda, int c0, double *w, int *info);
 
-// One block step of the forward solve L x = rhs (block j of 64 rows):
-// out[block j] = x_j and rhs[later blocks] -= L[later, j] x_j. Run for
-// j = 0 .. n_pad/64-1 in order.
-void solve_lower_step(hipStream_t s, const double *L, long lda, double *rhs, double *out, int n_pad, int j);
-// One block step of the backward solve L^T x = rhs, run for j descending.
-void solve_upper_step(hipStream_t s, const double *L, long lda, double *rhs, double *out, int n_pad, int j);
+// Block steps of the vector solves with the saved diagonal-block inverses
+// Winv (block J at Winv + J*64*64). Forward: out[J] = W_JJ rhs[J],
+// rhs[later] -= L[later, J] out[J]; run J = 0 .. n_pad/64-1 in order.
+// Backward (L^T x = rhs): out[J] = W_JJ^T rhs[J], rhs[earlier] -= L[J, earlier]^T out[J];
+// run J descending.
+void solve_lower_step(hipStream_t s, const double *L, long lda, const double *Winv, double *rhs, double *out, int n_pad, int j);
+void solve_upper_step(hipStream_t s, const double *L, long lda, const double *Winv, double *rhs, double *out, int n_pad, int j);
 
-// Solve L[r0:r0+64, r0:r0+64] X = B[r0:r0+64, 0:ncols] in place; if colsq
-// is not null, colsq[c] += sum over the 64 solved rows of X[.][c]^2.
-void trsm_left64(hipStream_t s, const double *L, long lda, double *B, long ldb, int r0, int ncols, double *colsq);
+// colsq[j] = sum_{i<n} V[i][j]^2 for j < m (deterministic two-pass);
+// `part` holds colsq_splits(n, m) * m doubles.
+int colsq_splits(int n, int m);
+void column_sumsq(hipStream_t s, const double *V, long ldv, int n, int m, double *part, int splits, double *colsq);
 
 // var[j] = v - colsq[j]
 void variance_finish(hipStream_t s, const double *colsq, double v, int m, double *var);
 
 // out[0] = y^T alpha, out[1] = sum_i log(L_ii^2), out[2] = alpha^T alpha, over i < n.
 void loss_terms(hipStream_t s, const double *L, long lda, const double *y, const double *alpha, int n, double *out);
+
+// r[i] = y[i] - s[i] - noise * alpha[i] (i < n), the residual of K alpha = y
+// given s = K_noiseless alpha.
+void residual_finish(hipStream_t s, const double *y, const double *kx, const double *alpha, double noise, int n, double *r);
+// y[i] += x[i], i < n.
+void add_vec(hipStream_t s, double *y, const double *x, long n);
 
 // Fill a device vector.
 void fill(hipStream_t s, double *p, long count, double value);

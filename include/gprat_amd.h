@@ -218,6 +218,10 @@ int gprat_model_predict(gprat_model *model,
 /* Loss at the fitted hyperparameters. */
 int gprat_model_loss(gprat_model *model, double *loss_out);
 
+/* Mixed-precision models: refinement steps taken for alpha and the last
+ * relative correction max|delta| / max|alpha| (0 and 0.0 for FP64 models). */
+int gprat_model_refinement(const gprat_model *model, int *iterations, double *last_correction);
+
 /* Device bytes held by the model. */
 int gprat_model_bytes(const gprat_model *model, size_t *bytes);
 

@@ -1,0 +1,48 @@
+"""The seeded generator against its fixture and against a scalar SplitMix64."""
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MASK = (1 << 64) - 1
+
+
+def splitmix64_scalar(seed, count):
+    state, out = seed, []
+    for _ in range(count):
+        state = (state + 0x9E3779B97F4A7C15) & MASK
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK
+        out.append(z ^ (z >> 31))
+    return out
+
+
+def test_splitmix64_matches_scalar_definition():
+    from gprat_amd import datagen
+    for seed in (0, 1, 12345, MASK):
+        assert [int(v) for v in datagen.splitmix64(seed, 8)] == splitmix64_scalar(seed, 8)
+
+
+def test_fixture():
+    from gprat_amd import datagen
+    with open(os.path.join(HERE, "golden", "datagen_vectors.json")) as f:
+        ref = json.load(f)
+    for seed in (0, 1):
+        u, y = datagen.msd_series(16, seed)
+        assert np.array_equal(u, ref[str(seed)]["u"])
+        assert np.array_equal(y, ref[str(seed)]["y"])
+        assert [int(w) for w in datagen.splitmix64(seed, 4)] == ref[str(seed)]["splitmix64_first4"]
+
+
+def test_properties():
+    from gprat_amd import datagen
+    u, y = datagen.msd_series(20000, 0)
+    assert np.all(np.abs(u) <= 1.0) and abs(u.mean()) < 0.02 and abs(u.var() - 1 / 3) < 0.02
+    x, yy, z = datagen.gp_problem(100, 10, 8)
+    assert x.shape == (107,) and np.all(x[:7] == 0.0) and yy.shape == (100,) and z.shape == (17,)
+    # prefix stability: a longer series starts with the shorter one
+    u2, _ = datagen.msd_series(50, 3)
+    u3, _ = datagen.msd_series(10, 3)
+    assert np.array_equal(u2[:10], u3)

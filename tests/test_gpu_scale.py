@@ -1,0 +1,82 @@
+"""GPU parity at the benchmark's full sizes, through the C ABI.
+
+The plain-FP64 scale configuration (n_train=16384 in 32 tiles of 512,
+n_reg=8, n_test=4096) and a 32768-point problem (n_reg=16, noise 0.05) are
+compared directly with the CPU oracle (seconds to tens of seconds on the
+host), plus size-independent properties: repeated requests are bitwise
+identical, and a resident model predicts exactly what the one-shot call does.
+"""
+import numpy as np
+import pytest
+
+from conftest import norm_rel
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from gprat_amd import capi
+    c = capi.Context(0, 1)
+    yield c
+    c.close()
+
+
+def _oracle_gp(oracle, x, y, T, b, r, theta):
+    return oracle.GP(list(x), list(y), T, b, r, list(theta), [True, True, True])
+
+
+@pytest.mark.parametrize("T,b,m,r,theta", [
+    (32, 512, 4096, 8, (1.0, 1.0, 0.1)),       # plain FP64 scale configuration
+    (25, 400, 1000, 8, (1.0, 1.0, 0.1)),       # n = 10000: ragged padding at scale
+])
+def test_full_size_parity(ctx, oracle, T, b, m, r, theta):
+    from gprat_amd import datagen
+    n = T * b
+    x, y, z = datagen.gp_problem(n, m, r)
+    mt, mb = oracle.compute_test_tiles(m, T, b)
+    gp = _oracle_gp(oracle, x, y, T, b, r, theta)
+    om, ov = gp.predict_with_uncertainty(list(z), mt, mb)
+    ol = gp.compute_loss()
+    gm, gv = ctx.predict(x, y, z, theta, T, b, mt, mb, r, variance=True)
+    gl = ctx.compute_loss(x, y, theta, T, b, r)
+    print(f"n={n} m={mt * mb}: mean {norm_rel(gm, om):.3e} var {norm_rel(gv, ov):.3e} "
+          f"loss {abs(gl - ol) / abs(ol):.3e}")
+    assert norm_rel(gm, om) < TOL
+    assert norm_rel(gv, ov) < TOL
+    assert abs(gl - ol) <= TOL * abs(ol)
+
+
+def test_n32768_parity(ctx, oracle):
+    from gprat_amd import datagen
+    T, b, r, m = 32, 1024, 16, 1024
+    theta = (1.0, 1.0, 0.05)
+    x, y, z = datagen.gp_problem(T * b, m, r)
+    gp = _oracle_gp(oracle, x, y, T, b, r, theta)
+    om, ov = gp.predict_with_uncertainty(list(z), 1, m)
+    gm, gv = ctx.predict(x, y, z, theta, T, b, 1, m, r, variance=True)
+    print(f"n=32768 m={m}: mean {norm_rel(gm, om):.3e} var {norm_rel(gv, ov):.3e}")
+    assert norm_rel(gm, om) < TOL
+    assert norm_rel(gv, ov) < TOL
+
+
+def test_deterministic_and_resident(ctx):
+    from gprat_amd import capi, datagen
+    T, b, r, m = 16, 512, 8, 2048
+    theta = (1.0, 1.0, 0.1)
+    x, y, z = datagen.gp_problem(T * b, m, r)
+    a = ctx.predict(x, y, z, theta, T, b, 1, m, r, variance=True)
+    c = ctx.predict(x, y, z, theta, T, b, 1, m, r, variance=True)
+    assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
+    model = capi.Model(ctx, x, y, T * b, r, theta)
+    rm, rv = model.predict(z, m)
+    assert np.array_equal(rm, a[0]) and np.array_equal(rv, a[1])
+    # several predictions against the same resident factor (the reduction
+    # split depends on m, so smaller batches agree to rounding, not bitwise)
+    for mm in (1, 7, 1000):
+        pm, pv = model.predict(z[:mm + r - 1], mm)
+        assert norm_rel(pm, a[0][:mm]) < 1e-13 and norm_rel(pv, a[1][:mm]) < 1e-13
+    assert abs(model.loss() - ctx.compute_loss(x, y, theta, T, b, r)) == 0.0
+    model.release()
