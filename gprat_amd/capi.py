@@ -47,10 +47,20 @@ SIGNATURES = {
     "gprat_model_bytes": [_VP, C.POINTER(_SZ)],
     "gprat_model_refinement": [_VP, _I, _D],
     "gprat_model_release": [_VP],
+    "gprat_batch_fit": [_VP, C.c_int, C.c_void_p, C.c_int, C.POINTER(_VP)],
+    "gprat_batch_losses": [_VP, _D],
+    "gprat_batch_predict": [_VP, C.c_int, _D, _SZ, C.c_int, _D, _D],
+    "gprat_batch_release": [_VP],
     "gprat_context_set_profiling": [_VP, C.c_int],
     "gprat_context_kernel_stats": [_VP, C.c_char_p, C.POINTER(C.c_long), _D],
 }
 _RESTYPE = {"gprat_last_error": C.c_char_p, "gprat_version": C.c_char_p}
+
+
+class GpDesc(C.Structure):
+    """gprat_gp_desc: one GP of a batch (host pointers)."""
+    _fields_ = [("train_in", _D), ("train_in_len", _SZ), ("train_out", _D), ("train_out_len", _SZ),
+                ("n", C.c_int), ("n_reg", C.c_int), ("theta", C.c_double * 3)]
 
 
 class GpratError(RuntimeError):
@@ -277,6 +287,56 @@ class Model:
     def release(self):
         if self.handle:
             lib().gprat_model_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many independent GPs fitted by one persistent launch; factors stay in HBM.
+    `problems` is a list of (train_in, train_out, n, n_reg, theta)."""
+
+    def __init__(self, ctx: Context, problems, precision=PRECISION_FP64):
+        self.ctx = ctx
+        self._keep = []
+        descs = (GpDesc * len(problems))()
+        for i, (x, y, n, r, theta) in enumerate(problems):
+            xa, ya = _arr(x), _arr(y)
+            self._keep += [xa, ya]
+            descs[i].train_in = _dptr(xa)
+            descs[i].train_in_len = xa.size
+            descs[i].train_out = _dptr(ya)
+            descs[i].train_out_len = ya.size
+            descs[i].n = n
+            descs[i].n_reg = r
+            for k in range(3):
+                descs[i].theta[k] = float(theta[k])
+        h = _VP()
+        _check(lib().gprat_batch_fit(ctx.handle, len(problems), C.cast(descs, C.c_void_p), precision, C.byref(h)))
+        self.handle = h
+        self.count = len(problems)
+        self._keep = []
+
+    def losses(self):
+        out = np.empty(self.count)
+        _check(lib().gprat_batch_losses(self.handle, _dptr(out)))
+        return out
+
+    def predict(self, index, test_in, m, variance=True):
+        z = _arr(test_in)
+        mean = np.empty(m)
+        var = np.empty(m) if variance else None
+        _check(lib().gprat_batch_predict(self.handle, index, _dptr(z), z.size, m, _dptr(mean),
+                                         _dptr(var) if variance else None))
+        return (mean, var) if variance else mean
+
+    def release(self):
+        if self.handle:
+            lib().gprat_batch_release(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -45,10 +45,48 @@ int guarded(Fn &&fn)
     }
 }
 
+// Handles. A context is shared by its handle and by every resident model
+// fitted on it, so destroying a context while models are alive only drops
+// the handle's reference; the last model release frees it.
+struct ContextHandle
+{
+    std::shared_ptr<gpamd::Context> ctx;
+};
+
+struct ModelHandle
+{
+    std::shared_ptr<gpamd::Context> owner;
+    std::unique_ptr<gpamd::Model> model;
+};
+
+struct BatchHandle
+{
+    std::shared_ptr<gpamd::Context> owner;
+    std::unique_ptr<gpamd::Batch> batch;
+};
+
 gpamd::Context &ctx_ref(gprat_context *c)
 {
     gpamd::require(c != nullptr, "context is null");
-    return *reinterpret_cast<gpamd::Context *>(c);
+    return *reinterpret_cast<ContextHandle *>(c)->ctx;
+}
+
+std::shared_ptr<gpamd::Context> ctx_shared(gprat_context *c)
+{
+    gpamd::require(c != nullptr, "context is null");
+    return reinterpret_cast<ContextHandle *>(c)->ctx;
+}
+
+gpamd::Model &model_ref(gprat_model *m)
+{
+    gpamd::require(m != nullptr, "model is null");
+    return *reinterpret_cast<ModelHandle *>(m)->model;
+}
+
+const gpamd::Model &model_ref(const gprat_model *m)
+{
+    gpamd::require(m != nullptr, "model is null");
+    return *reinterpret_cast<const ModelHandle *>(m)->model;
 }
 
 struct Tiling
@@ -150,14 +188,15 @@ int gprat_context_create(int device, int n_streams, gprat_context **out)
                 throw gpamd::Error(GPRAT_ERR_NO_DEVICE, "no HIP device available");
             if (device < 0 || device >= n)
                 throw gpamd::Error(GPRAT_ERR_NO_DEVICE, "Requested GPU device is not available.");
-            auto *c = new gpamd::Context(device, n_streams);
-            *out = reinterpret_cast<gprat_context *>(c);
+            auto h = std::make_unique<ContextHandle>();
+            h->ctx = std::make_shared<gpamd::Context>(device, n_streams);
+            *out = reinterpret_cast<gprat_context *>(h.release());
         });
 }
 
 int gprat_context_destroy(gprat_context *ctx)
 {
-    return guarded([&] { delete reinterpret_cast<gpamd::Context *>(ctx); });
+    return guarded([&] { delete reinterpret_cast<ContextHandle *>(ctx); });
 }
 
 int gprat_device_alloc(gprat_context *ctx, size_t bytes, void **ptr)
@@ -393,12 +432,14 @@ int gprat_model_fit(gprat_context *ctx,
     return guarded(
         [&]
         {
-            auto &c = ctx_ref(ctx);
+            auto owner = ctx_shared(ctx);
             gpamd::require(out != nullptr, "out is null");
-            std::lock_guard<std::recursive_mutex> g(c.mu);
-            auto m = std::make_unique<gpamd::Model>();
-            gpamd::fit(c, *m, train_in, train_in_len, train_out, train_out_len, n, n_reg, theta, precision);
-            *out = reinterpret_cast<gprat_model *>(m.release());
+            std::lock_guard<std::recursive_mutex> g(owner->mu);
+            auto h = std::make_unique<ModelHandle>();
+            h->owner = owner;
+            h->model = std::make_unique<gpamd::Model>();
+            gpamd::fit(*owner, *h->model, train_in, train_in_len, train_out, train_out_len, n, n_reg, theta, precision);
+            *out = reinterpret_cast<gprat_model *>(h.release());
         });
 }
 
@@ -407,10 +448,9 @@ int gprat_model_predict(gprat_model *model, const double *test_in, size_t test_i
     return guarded(
         [&]
         {
-            gpamd::require(model != nullptr, "model is null");
-            auto *md = reinterpret_cast<gpamd::Model *>(model);
-            std::lock_guard<std::recursive_mutex> g(md->ctx->mu);
-            gpamd::predict(*md, test_in, test_in_len, m, mean_out, var_out);
+            auto &md = model_ref(model);
+            std::lock_guard<std::recursive_mutex> g(md.ctx->mu);
+            gpamd::predict(md, test_in, test_in_len, m, mean_out, var_out);
         });
 }
 
@@ -419,10 +459,10 @@ int gprat_model_loss(gprat_model *model, double *loss_out)
     return guarded(
         [&]
         {
-            gpamd::require(model != nullptr && loss_out != nullptr, "null argument");
-            auto *md = reinterpret_cast<gpamd::Model *>(model);
-            std::lock_guard<std::recursive_mutex> g(md->ctx->mu);
-            *loss_out = gpamd::loss(*md);
+            gpamd::require(loss_out != nullptr, "loss_out is null");
+            auto &md = model_ref(model);
+            std::lock_guard<std::recursive_mutex> g(md.ctx->mu);
+            *loss_out = gpamd::loss(md);
         });
 }
 
@@ -431,10 +471,10 @@ int gprat_model_refinement(const gprat_model *model, int *iterations, double *la
     return guarded(
         [&]
         {
-            gpamd::require(model != nullptr && iterations != nullptr && last_correction != nullptr, "null argument");
-            const auto *md = reinterpret_cast<const gpamd::Model *>(model);
-            *iterations = md->refine_iterations;
-            *last_correction = md->refine_last;
+            gpamd::require(iterations != nullptr && last_correction != nullptr, "null argument");
+            const auto &md = model_ref(model);
+            *iterations = md.refine_iterations;
+            *last_correction = md.refine_last;
         });
 }
 
@@ -443,8 +483,8 @@ int gprat_model_bytes(const gprat_model *model, size_t *bytes)
     return guarded(
         [&]
         {
-            gpamd::require(model != nullptr && bytes != nullptr, "null argument");
-            *bytes = reinterpret_cast<const gpamd::Model *>(model)->bytes();
+            gpamd::require(bytes != nullptr, "bytes is null");
+            *bytes = model_ref(model).bytes();
         });
 }
 
@@ -453,11 +493,77 @@ int gprat_model_release(gprat_model *model)
     return guarded(
         [&]
         {
-            auto *md = reinterpret_cast<gpamd::Model *>(model);
-            if (!md)
+            if (!model)
                 return;
-            std::lock_guard<std::recursive_mutex> g(md->ctx->mu);
-            delete md;
+            std::unique_ptr<ModelHandle> h(reinterpret_cast<ModelHandle *>(model));
+            std::shared_ptr<gpamd::Context> keep = h->owner;
+            {
+                std::lock_guard<std::recursive_mutex> g(keep->mu);
+                keep->make_current();
+                h->model.reset();
+            }
+            h.reset();
+            keep.reset();  // frees the context if its handle is already gone
+        });
+}
+
+int gprat_batch_fit(gprat_context *ctx, int count, const gprat_gp_desc *gps, int precision, gprat_batch **out)
+{
+    return guarded(
+        [&]
+        {
+            auto owner = ctx_shared(ctx);
+            gpamd::require(out != nullptr, "out is null");
+            std::lock_guard<std::recursive_mutex> g(owner->mu);
+            auto h = std::make_unique<BatchHandle>();
+            h->owner = owner;
+            h->batch = std::make_unique<gpamd::Batch>();
+            gpamd::batch_fit(*owner, *h->batch, count, gps, precision);
+            *out = reinterpret_cast<gprat_batch *>(h.release());
+        });
+}
+
+int gprat_batch_losses(gprat_batch *batch, double *losses)
+{
+    return guarded(
+        [&]
+        {
+            gpamd::require(batch != nullptr && losses != nullptr, "null argument");
+            auto *h = reinterpret_cast<BatchHandle *>(batch);
+            std::lock_guard<std::recursive_mutex> g(h->owner->mu);
+            gpamd::batch_losses(*h->batch, losses);
+        });
+}
+
+int gprat_batch_predict(gprat_batch *batch, int index, const double *test_in, size_t test_in_len, int m, double *mean_out,
+                        double *var_out)
+{
+    return guarded(
+        [&]
+        {
+            gpamd::require(batch != nullptr, "batch is null");
+            auto *h = reinterpret_cast<BatchHandle *>(batch);
+            std::lock_guard<std::recursive_mutex> g(h->owner->mu);
+            gpamd::batch_predict(*h->batch, index, test_in, test_in_len, m, mean_out, var_out);
+        });
+}
+
+int gprat_batch_release(gprat_batch *batch)
+{
+    return guarded(
+        [&]
+        {
+            if (!batch)
+                return;
+            std::unique_ptr<BatchHandle> h(reinterpret_cast<BatchHandle *>(batch));
+            std::shared_ptr<gpamd::Context> keep = h->owner;
+            {
+                std::lock_guard<std::recursive_mutex> g(keep->mu);
+                keep->make_current();
+                h->batch.reset();
+            }
+            h.reset();
+            keep.reset();
         });
 }
 

@@ -600,7 +600,7 @@ void fit(Context &c,
     attach_targets(m, y, y_len);
 }
 
-void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean, double *var)
+void predict_view(const FactorView &m, const double *z, std::size_t z_len, int mt, double *mean, double *var)
 {
     Context &c = *m.ctx;
     require(mt >= 0, "predict: negative test count");
@@ -618,7 +618,7 @@ void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean,
     DevBuf<double> dmean(c, mt);
     {
         Timed t(c, c.main, "mean");
-        k::predict_mean(c.main, m.x.get(), dz.get(), m.alpha.get(), m.r, m.sek, m.n, mt, partial.get(), splits, dmean.get());
+        k::predict_mean(c.main, m.x, dz.get(), m.alpha, m.r, m.sek, m.n, mt, partial.get(), splits, dmean.get());
     }
     DevBuf<double> dvar;
     if (var)
@@ -630,11 +630,11 @@ void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean,
         dvar = DevBuf<double>(c, mt);
         {
             Timed t(c, c.main, "assemble");
-            k::assemble_cross_t(c.main, m.x.get(), dz.get(), m.r, m.sek, B.get(), ldb, m.n, m.n_pad, mt, m_pad);
+            k::assemble_cross_t(c.main, m.x, dz.get(), m.r, m.sek, B.get(), ldb, m.n, m.n_pad, mt, m_pad);
         }
         {
             Timed t(c, c.main, "trsm");
-            trsm_lower_left(c, m.L.get(), m.lda, m.Winv.get(), B.get(), ldb, m_pad, 0, m.n_pad, false);
+            trsm_lower_left(c, m.L, m.lda, m.Winv, B.get(), ldb, m_pad, 0, m.n_pad, false);
         }
         const int splits = k::colsq_splits(m.n, mt);
         DevBuf<double> cpart(c, static_cast<std::size_t>(splits) * mt);
@@ -644,6 +644,115 @@ void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean,
     }
     GP_HIP(hipMemcpyAsync(mean, dmean.get(), sizeof(double) * mt, hipMemcpyDefault, c.main));
     c.sync();
+}
+
+void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean, double *var)
+{
+    predict_view(FactorView{ m.ctx, m.x.get(), m.n, m.n_pad, m.r, m.lda, m.sek, m.L.get(), m.Winv.get(), m.alpha.get() }, z,
+                 z_len, mt, mean, var);
+}
+
+// ---------------------------------------------------------------------------
+// Batched fit of many GPs
+
+void batch_fit(Context &c, Batch &b, int count, const gprat_gp_desc *gps, int precision)
+{
+    require(count >= 1 && gps != nullptr, "batch: need at least one GP");
+    require(precision == GPRAT_PRECISION_FP64, "batch: only GPRAT_PRECISION_FP64 is available for batches");
+    c.make_current();
+    b.ctx = &c;
+    b.count = count;
+    b.gps.resize(count);
+    // arena layout: [inputs of all GPs][factors, inverses, alpha of all GPs]
+    long in_total = 0, out_total = 0;
+    for (int g = 0; g < count; g++)
+    {
+        const gprat_gp_desc &d = gps[g];
+        require(d.n >= 1 && d.n_reg >= 1, "batch: GP " + std::to_string(g) + ": n and n_reg must be >= 1");
+        require(d.train_in != nullptr && d.train_in_len >= static_cast<std::size_t>(d.n) + d.n_reg - 1,
+                "batch: GP " + std::to_string(g) + ": training input must hold n + n_reg - 1 values");
+        require(d.train_out != nullptr && d.train_out_len >= static_cast<std::size_t>(d.n),
+                "batch: GP " + std::to_string(g) + ": training output must hold n values");
+        const long np = round_up(d.n, 64);
+        k::BatchGP &q = b.gps[g];
+        q.n = d.n;
+        q.n_pad = static_cast<int>(np);
+        q.n_reg = d.n_reg;
+        q.lengthscale = d.theta[0];
+        q.vertical = d.theta[1];
+        q.noise = d.theta[2];
+        q.x_off = in_total;
+        in_total += round_up(d.n + d.n_reg - 1, 2);
+        q.y_off = in_total;
+        in_total += np;
+    }
+    for (int g = 0; g < count; g++)
+    {
+        k::BatchGP &q = b.gps[g];
+        const long np = q.n_pad;
+        q.l_off = in_total + out_total;
+        out_total += np * np;
+        q.w_off = in_total + out_total;
+        out_total += np * 64;
+        q.a_off = in_total + out_total;
+        out_total += np;
+    }
+    std::vector<double> staging(static_cast<std::size_t>(in_total), 0.0);
+    for (int g = 0; g < count; g++)
+    {
+        const k::BatchGP &q = b.gps[g];
+        std::copy(gps[g].train_in, gps[g].train_in + q.n + q.n_reg - 1, staging.begin() + q.x_off);
+        std::copy(gps[g].train_out, gps[g].train_out + q.n, staging.begin() + q.y_off);
+    }
+    // largest GPs first, so the persistent workgroups finish together
+    std::vector<int> order(count);
+    for (int g = 0; g < count; g++)
+        order[g] = g;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int z) { return b.gps[a].n > b.gps[z].n; });
+
+    b.arena = DevBuf<double>(c, static_cast<std::size_t>(in_total + out_total));
+    b.desc = DevBuf<double>(c, (sizeof(k::BatchGP) * count + 7) / 8);
+    b.order = DevBuf<int>(c, count);
+    b.info = DevBuf<int>(c, count);
+    b.counter = DevBuf<int>(c, 1);
+    b.losses = DevBuf<double>(c, count);
+    GP_HIP(hipMemcpyAsync(b.arena.get(), staging.data(), staging.size() * sizeof(double), hipMemcpyDefault, c.main));
+    GP_HIP(hipMemcpyAsync(b.desc.get(), b.gps.data(), sizeof(k::BatchGP) * count, hipMemcpyDefault, c.main));
+    GP_HIP(hipMemcpyAsync(b.order.get(), order.data(), sizeof(int) * count, hipMemcpyDefault, c.main));
+    hipDeviceProp_t prop;
+    GP_HIP(hipGetDeviceProperties(&prop, c.device));
+    const int workgroups = std::max(1, std::min(count, prop.multiProcessorCount));
+    {
+        Timed t(c, c.main, "batch_fit");
+        k::batch_fit(c.main, reinterpret_cast<const k::BatchGP *>(b.desc.get()), b.order.get(), count, b.counter.get(),
+                     b.arena.get(), b.losses.get(), b.info.get(), workgroups);
+    }
+    std::vector<int> info(count);
+    GP_HIP(hipMemcpyAsync(info.data(), b.info.get(), sizeof(int) * count, hipMemcpyDefault, c.main));
+    c.sync();
+    for (int g = 0; g < count; g++)
+        if (info[g] != INT_MAX)
+            throw Error(GPRAT_ERR_NOT_SPD, "batch: GP " + std::to_string(g) +
+                                               ": covariance matrix is not positive definite (first non-positive pivot at column " +
+                                               std::to_string(info[g]) + ")");
+}
+
+void batch_losses(Batch &b, double *losses)
+{
+    Context &c = *b.ctx;
+    c.make_current();
+    GP_HIP(hipMemcpyAsync(losses, b.losses.get(), sizeof(double) * b.count, hipMemcpyDefault, c.main));
+    c.sync();
+}
+
+void batch_predict(Batch &b, int index, const double *z, std::size_t z_len, int mt, double *mean, double *var)
+{
+    require(index >= 0 && index < b.count, "batch: GP index out of range");
+    const k::BatchGP &q = b.gps[index];
+    double *base = b.arena.get();
+    predict_view(FactorView{ b.ctx, base + q.x_off, q.n, q.n_pad, q.n_reg, q.n_pad, SEK{ q.lengthscale, q.vertical, q.noise },
+                             base + q.l_off, base + q.w_off, base + q.a_off },
+                 z, z_len, mt, mean, var);
 }
 
 double loss(Model &m)

@@ -125,6 +125,32 @@ struct Model
     std::size_t bytes() const { return x.bytes() + y.bytes() + L.bytes() + Winv.bytes() + alpha.bytes(); }
 };
 
+// Raw view of a factored GP (resident model or one GP of a batch).
+struct FactorView
+{
+    Context *ctx;
+    const double *x;
+    int n, n_pad, r;
+    long lda;
+    SEK sek;
+    const double *L;
+    const double *Winv;
+    const double *alpha;
+};
+
+// Many GPs fitted by one persistent launch; factors stay in one arena.
+struct Batch
+{
+    Context *ctx = nullptr;
+    int count = 0;
+    std::vector<k::BatchGP> gps;
+    DevBuf<double> arena;
+    DevBuf<double> desc;  // device copy of gps
+    DevBuf<int> order, info, counter;
+    DevBuf<double> losses;
+    std::size_t bytes() const { return arena.bytes() + desc.bytes() + losses.bytes(); }
+};
+
 // Algorithms (engine.cpp)
 long padded(long n);
 long leading_dim(long n_pad);
@@ -148,6 +174,10 @@ void fit(Context &c,
          const double theta[3],
          int precision);
 void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean, double *var);
+void predict_view(const FactorView &m, const double *z, std::size_t z_len, int mt, double *mean, double *var);
+void batch_fit(Context &c, Batch &b, int count, const gprat_gp_desc *gps, int precision);
+void batch_losses(Batch &b, double *losses);
+void batch_predict(Batch &b, int index, const double *z, std::size_t z_len, int mt, double *mean, double *var);
 double loss(Model &m);
 void cholesky_tiles(Model &m, int T, int b, double *out);
 void loss_gradient(Context &c,

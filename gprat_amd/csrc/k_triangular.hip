@@ -2,6 +2,7 @@
 // blocked vector solves, the multi-right-hand-side TRSM base case and the
 // loss reduction. All reductions run in a fixed order, so results do not
 // depend on scheduling.
+#include "block64.hpp"
 #include "kernels.hpp"
 
 namespace gpamd::k
@@ -22,94 +23,28 @@ __device__ __forceinline__ void load_lower_block(const double *__restrict__ A, l
     }
 }
 
-// Cholesky factor and inverse of one 64x64 diagonal block, one workgroup.
-// Thread (wave g, lane c) keeps column c, rows g + 4q (q = 0..15) in
-// registers. Column j is published through LDS, every thread forms
-// l_jj = sqrt(d_jj) and 1/l_jj itself (identical arithmetic), scales its own
-// rows of column j and applies the rank-1 update to its column. The inverse
-// W = L^-1 is then built row by row (right-looking elimination of L W = I)
-// with the factor staged in LDS. L overwrites the block's lower triangle;
-// W (64x64 row-major, zero above the diagonal) goes to `w`.
+// One workgroup per diagonal block: L overwrites the block's lower triangle,
+// W = L^-1 (64x64 row-major, zero upper) goes to `w`.
 __global__ __launch_bounds__(THREADS) void k_potrf64_inv(double *__restrict__ A, long lda, int c0, double *__restrict__ w, int *__restrict__ info)
 {
-    __shared__ double col[NB];
-    __shared__ double Ls[NB][NB + 1];
+    __shared__ double D[NB][NB + 1];
+    __shared__ double Wl[NB][NB + 1];
+    __shared__ double S[3][16][17];
     const int tid = threadIdx.x;
-    const int c = tid & (NB - 1), g = tid >> 6;
     double *blk = A + static_cast<long>(c0) * lda + c0;
-    double d[16];
-#pragma unroll
-    for (int q = 0; q < 16; q++)
-        d[q] = blk[static_cast<long>(g + 4 * q) * lda + c];
-
-    for (int j = 0; j < NB; j++)
+    for (int e = tid; e < NB * NB; e += THREADS)
     {
-        if (c == j)
-        {
-#pragma unroll
-            for (int q = 0; q < 16; q++)
-                col[g + 4 * q] = d[q];
-        }
-        __syncthreads();
-        const double djj = col[j];
-        if (tid == 0 && !(djj > 0.0))
-            atomicMin(info, c0 + j);
-        const double ljj = sqrt(djj);
-        const double inv = 1.0 / ljj;
-        const double lcj = (c > j) ? col[c] * inv : 0.0;
-#pragma unroll
-        for (int q = 0; q < 16; q++)
-        {
-            const int r = g + 4 * q;
-            const double lrj = (r > j) ? col[r] * inv : (r == j ? ljj : 0.0);
-            if (c == j)
-                d[q] = lrj;
-            else if (c > j)
-                d[q] -= lrj * lcj;
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int q = 0; q < 16; q++)
-    {
-        const int r = g + 4 * q;
-        const double v = (r >= c) ? d[q] : 0.0;
-        Ls[r][c] = v;
-        if (r >= c)
-            blk[static_cast<long>(r) * lda + c] = v;
+        const int r = e / NB, c = e % NB;
+        D[r][c] = (c <= r) ? blk[static_cast<long>(r) * lda + c] : 0.0;
     }
     __syncthreads();
-
-    // W = L^-1: b[q] holds row g + 4q, column c of the running right-hand side.
-    double b[16];
-#pragma unroll
-    for (int q = 0; q < 16; q++)
-        b[q] = (g + 4 * q == c) ? 1.0 : 0.0;
-#pragma unroll
-    for (int i = 0; i < NB; i++)
+    factor_block64(D, Wl, S, info, c0);
+    for (int e = tid; e < NB * NB; e += THREADS)
     {
-        const int qi = i >> 2, gi = i & 3;
-        if (g == gi)
-        {
-            b[qi] = b[qi] / Ls[i][i];
-            col[c] = b[qi];
-        }
-        __syncthreads();
-        const double wic = col[c];
-#pragma unroll
-        for (int q = 0; q < 16; q++)
-        {
-            const int r = g + 4 * q;
-            if (r > i)
-                b[q] -= Ls[r][i] * wic;
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int q = 0; q < 16; q++)
-    {
-        const int r = g + 4 * q;
-        w[r * NB + c] = (r >= c) ? b[q] : 0.0;
+        const int r = e / NB, c = e % NB;
+        if (c <= r)
+            blk[static_cast<long>(r) * lda + c] = D[r][c];
+        w[r * NB + c] = (c <= r) ? Wl[r][c] : 0.0;
     }
 }
 

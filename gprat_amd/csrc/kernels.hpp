@@ -116,6 +116,22 @@ void panel_to_f32t(hipStream_t s, const double *A, long lda, long r0, long p0, i
 // for i < M, j < N, i + diag_off >= j; K % 32 == 0.
 void gemm_f32_trailing(hipStream_t s, const float *P, long ldp, long a_off, long b_off, double *C, long ldc, int M, int N, int K, long diag_off);
 
+// ---- batched fit of many GPs (k_batch.hip) --------------------------------
+
+// One GP of a batch: offsets (in doubles) of its series, targets, factor,
+// diagonal-block inverses and alpha inside the batch arena.
+struct BatchGP
+{
+    long x_off, y_off, l_off, w_off, a_off;
+    int n, n_pad, n_reg, pad_;
+    double lengthscale, vertical, noise;
+};
+// Persistent fit of `count` GPs (visited in `order`, largest first) by
+// `workgroups` workgroups; losses[g] gets GP g's loss, info[g] its first
+// non-positive pivot column (INT_MAX if none).
+void batch_fit(hipStream_t s, const BatchGP *gps, const int *order, int count, int *counter, double *arena, double *losses,
+               int *info, int workgroups);
+
 // ---- triangular family (k_triangular.hip) ---------------------------------
 
 // Factor the 64x64 diagonal block at (c0, c0) in place (lower triangle) and

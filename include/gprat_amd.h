@@ -227,6 +227,33 @@ int gprat_model_bytes(const gprat_model *model, size_t *bytes);
 
 int gprat_model_release(gprat_model *model);
 
+/* ---- Batches of independent GPs ----------------------------------------- */
+
+/* One GP of a batch (host pointers). n training points, any n >= 1. */
+typedef struct gprat_gp_desc
+{
+    const double *train_in;
+    size_t train_in_len;
+    const double *train_out;
+    size_t train_out_len;
+    int n;
+    int n_reg;
+    double theta[3];
+} gprat_gp_desc;
+
+typedef struct gprat_batch gprat_batch;
+
+/* Fit `count` independent GPs of any sizes in one persistent launch
+ * (assembly, Cholesky, alpha and loss per GP); factors stay resident.
+ * No counterpart in the reference, which fits one GP per call. */
+int gprat_batch_fit(gprat_context *ctx, int count, const gprat_gp_desc *gps, int precision, gprat_batch **out);
+/* losses[g] = loss of GP g at its hyperparameters (count values). */
+int gprat_batch_losses(gprat_batch *batch, double *losses);
+/* Mean (and latent variance if var_out != NULL) of GP `index` at m points. */
+int gprat_batch_predict(gprat_batch *batch, int index, const double *test_in, size_t test_in_len, int m, double *mean_out,
+                        double *var_out);
+int gprat_batch_release(gprat_batch *batch);
+
 /* ---- Measurement hooks ------------------------------------------------- */
 
 /* When enabled, the library brackets every launch of its named kernels with

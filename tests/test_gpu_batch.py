@@ -1,0 +1,81 @@
+"""Batched many-GP fit (one persistent launch) against single-GP fits and
+the CPU oracle: every GP's loss, mean and variance agree to 1e-10."""
+import numpy as np
+import pytest
+
+from conftest import norm_rel
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from gprat_amd import capi
+    c = capi.Context(0, 1)
+    yield c
+    c.close()
+
+
+def _problems(count, seed, n_lo=64, n_hi=700):
+    from gprat_amd import datagen
+    rng = np.random.default_rng(seed)
+    probs = []
+    for g in range(count):
+        n = int(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi))))
+        r = int(rng.choice([4, 8, 16]))
+        theta = (float(np.exp(rng.uniform(np.log(0.3), np.log(3.0)))),
+                 float(np.exp(rng.uniform(np.log(0.5), np.log(2.0)))),
+                 float(np.exp(rng.uniform(np.log(0.01), np.log(0.3)))))
+        x, y, z = datagen.gp_problem(n, 50, r, seed_train=1000 + 2 * g, seed_test=1001 + 2 * g)
+        probs.append((x, y, n, r, theta, z))
+    return probs
+
+
+def test_batch_matches_single_fits(ctx):
+    from gprat_amd import capi
+    probs = _problems(48, seed=5)
+    batch = capi.Batch(ctx, [(x, y, n, r, t) for x, y, n, r, t, _ in probs])
+    try:
+        losses = batch.losses()
+        worst = 0.0
+        for g, (x, y, n, r, t, z) in enumerate(probs):
+            single = capi.Model(ctx, x, y, n, r, t)
+            sm, sv = single.predict(z, 50)
+            bm, bv = batch.predict(g, z, 50)
+            sl = single.loss()
+            single.release()
+            worst = max(worst, norm_rel(bm, sm), norm_rel(bv, sv), abs(losses[g] - sl) / abs(sl))
+            assert norm_rel(bm, sm) < 1e-10, g
+            assert norm_rel(bv, sv) < 1e-10, g
+            assert abs(losses[g] - sl) <= 1e-10 * abs(sl), g
+        print(f"48 GPs, worst relative difference to single fits {worst:.3e}")
+    finally:
+        batch.release()
+
+
+def test_batch_matches_oracle(ctx, oracle):
+    from gprat_amd import capi
+    probs = _problems(6, seed=9, n_lo=100, n_hi=400)
+    batch = capi.Batch(ctx, [(x, y, n, r, t) for x, y, n, r, t, _ in probs])
+    try:
+        losses = batch.losses()
+        for g, (x, y, n, r, t, z) in enumerate(probs):
+            gp = oracle.GP(list(x), list(y), 1, n, r, list(t), [True] * 3)
+            om, ov = gp.predict_with_uncertainty(list(z), 1, 50)
+            ol = gp.compute_loss()
+            bm, bv = batch.predict(g, z, 50)
+            assert norm_rel(bm, om) < 1e-10 and norm_rel(bv, ov) < 1e-10, g
+            assert abs(losses[g] - ol) <= 1e-10 * abs(ol), g
+    finally:
+        batch.release()
+
+
+def test_batch_reports_not_spd(ctx):
+    from gprat_amd import capi
+    probs = _problems(3, seed=2)
+    bad = [(x, y, n, r, t) for x, y, n, r, t, _ in probs]
+    x, y, n, r, t = bad[1]
+    bad[1] = (x, y, n, r, (t[0], t[1], -2.0 * t[1]))
+    with pytest.raises(capi.GpratError) as e:
+        capi.Batch(ctx, bad)
+    assert e.value.status == capi.ERR_NOT_SPD and "GP 1" in str(e.value)

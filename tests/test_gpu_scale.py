@@ -80,3 +80,17 @@ def test_deterministic_and_resident(ctx):
         assert norm_rel(pm, a[0][:mm]) < 1e-13 and norm_rel(pv, a[1][:mm]) < 1e-13
     assert abs(model.loss() - ctx.compute_loss(x, y, theta, T, b, r)) == 0.0
     model.release()
+
+
+def test_model_outlives_context():
+    """A resident model keeps its context alive: destroying the context
+    handle first, then predicting and releasing, is valid."""
+    from gprat_amd import capi, datagen
+    x, y, z = datagen.gp_problem(256, 16, 4)
+    c = capi.Context(0, 1)
+    model = capi.Model(c, x, y, 256, 4, (1.0, 1.0, 0.1))
+    ref = model.predict(z, 16)
+    c.close()
+    again = model.predict(z, 16)
+    assert np.array_equal(ref[0], again[0]) and np.array_equal(ref[1], again[1])
+    model.release()
