@@ -120,10 +120,11 @@ def cholesky_rate(capi, ctx, n=32768, repeats=2):
 
 
 def mixed_leg(capi, ctx, n=32768, m=8192, r=16, noise=0.05, repeats=2):
-    """Mixed-precision configuration: n=32768 (32 tiles of 1024), n_reg 16,
-    m=8192, noise 0.05. Times one fit+predict request in FP64 and in mixed
-    precision (FP32 matrix-core trailing updates + FP64 refinement of
-    alpha) and reports the mixed result's error against the FP64 one."""
+    """Mixed-precision configuration: n in {32768, 65536} (tiles of 1024),
+    n_reg 16, m=8192, noise 0.05. Times one fit+predict request in FP64 and
+    in mixed precision (FP32 matrix-core trailing updates + FP64 refinement
+    of alpha, variance solve on FP32 matrix cores) and reports the mixed
+    result's error against the FP64 one."""
     from gprat_amd import datagen
     x, y, z = datagen.gp_problem(n, m, r)
     theta = (1.0, 1.0, noise)
@@ -154,10 +155,170 @@ def mixed_leg(capi, ctx, n=32768, m=8192, r=16, noise=0.05, repeats=2):
         "loss": abs(lmx - l64) / abs(l64),
     }
     out["refinement_steps"] = it
-    out["workload"] = f"fit+predict n={n} (32 tiles of 1024), n_reg={r}, m={m}, noise={noise}"
+    out["workload"] = f"fit+predict n={n} ({n // 1024} tiles of 1024), n_reg={r}, m={m}, noise={noise}"
     for a in (dx, dy, dz):
         a.free()
     return out
+
+
+def batched_leg(capi, ctx, count=2048, seed=4):
+    """Batched many-GP configuration: 2048 independent GPs, n log-uniform in
+    [64, 2048], n_reg from {4, 8, 16}, hyperparameters log-uniform
+    (l in [0.3, 3], v in [0.5, 2], noise in [0.01, 0.3]), per-GP seeds; one
+    persistent launch fits all of them (assembly, Cholesky, alpha, loss)."""
+    from gprat_amd import datagen
+    rng = np.random.default_rng(seed)
+    probs, flops = [], 0.0
+    for g in range(count):
+        n = int(np.exp(rng.uniform(np.log(64), np.log(2048))))
+        r = int(rng.choice([4, 8, 16]))
+        theta = (float(np.exp(rng.uniform(np.log(0.3), np.log(3.0)))),
+                 float(np.exp(rng.uniform(np.log(0.5), np.log(2.0)))),
+                 float(np.exp(rng.uniform(np.log(0.01), np.log(0.3)))))
+        u, y = datagen.msd_series(n, 100000 + g)
+        probs.append((datagen.padded_input(u, r), y, n, r, theta))
+        flops += n ** 3 / 3.0
+    capi.Batch(ctx, probs[:16]).release()  # warm-up
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    b = capi.Batch(ctx, probs)
+    wall = time.perf_counter() - t0
+    launches, kernel_ms = ctx.kernel_stats("batch_fit")
+    ctx.set_profiling(False)
+    losses = b.losses()
+    b.release()
+    return {
+        "gps": count,
+        "kernel_ms": kernel_ms,
+        "wall_ms_incl_upload": wall * 1e3,
+        "gps_per_s": count / (kernel_ms * 1e-3),
+        "cholesky_tflops": flops / (kernel_ms * 1e-3) / 1e12,
+        "finite_losses": bool(np.all(np.isfinite(losses))),
+        "workload": "2048 GPs, n log-uniform [64, 2048], n_reg in {4, 8, 16}, FP64",
+    }
+
+
+def serving_trace(count=192, seed=7):
+    """Serving configuration: 70% fit requests (n log-uniform in [256, 32768],
+    n_reg = 8, theta = (1, 1, 0.1)), 30% predict requests (n_test uniform in
+    [1, 4096]) against a model fitted earlier in the trace (uniform choice);
+    per-request seeds."""
+    from gprat_amd import datagen
+    rng = np.random.default_rng(seed)
+    reqs, fits = [], []
+    for i in range(count):
+        if not fits or rng.random() < 0.7:
+            n = int(np.exp(rng.uniform(np.log(256), np.log(32768))))
+            u, y = datagen.msd_series(n, 300000 + i)
+            reqs.append(("fit", datagen.padded_input(u, 8), y, n))
+            fits.append(i)
+        else:
+            target = int(rng.choice(fits))
+            m = int(rng.integers(1, 4097))
+            ut, _ = datagen.msd_series(m, 300000 + i)
+            reqs.append(("predict", target, datagen.padded_input(ut, 8), m))
+    return reqs
+
+
+def serving_leg(capi, count=192, budget=200 << 30, slots=4):
+    """Request loop with continuous batching against the sequential plain
+    FP64 path on the same trace.
+
+    * sequential: one request at a time through resident FP64 models (fit =
+      assemble + factor + alpha + loss; predict = mean + variance on the
+      resident factor). `sequential_refit_req_s` prices the reference's API
+      instead, where nothing stays on the device and every predict refits.
+    * saturated: the whole trace queued at t = 0 on the server (mixed precision
+      for single-GP fits, batched FP64 launches for n <= 2048, 200 GB budget).
+    * poisson: open-loop Poisson arrivals at 70% of the saturated rate;
+      latency = completion - arrival.
+    """
+    import threading
+    reqs = serving_trace(count)
+    theta = (1.0, 1.0, 0.1)
+
+    # sequential plain FP64, on a context of its own whose cached blocks are
+    # freed before the server starts
+    seq = capi.Context(0, 1)
+    models, times = {}, []
+    t_start = time.perf_counter()
+    for i, r in enumerate(reqs):
+        s = time.perf_counter()
+        if r[0] == "fit":
+            m = capi.Model(seq, r[1], r[2], r[3], 8, theta, capi.PRECISION_FP64)
+            m.loss()
+            models[i] = m
+        else:
+            models[r[1]].predict(r[2], r[3], variance=True)
+        times.append(time.perf_counter() - s)
+    seq_total = time.perf_counter() - t_start
+    refit_total = seq_total + sum(times[r[1]] for r in reqs if r[0] == "predict")
+    for m in models.values():
+        m.release()
+    models.clear()
+    seq.close()
+
+    def run(srv, arrivals):
+        tickets, submit_t = [None] * len(reqs), [0.0] * len(reqs)
+        t0 = time.perf_counter()
+
+        def submitter():
+            for i, r in enumerate(reqs):
+                if arrivals is not None:
+                    d = t0 + arrivals[i] - time.perf_counter()
+                    if d > 0:
+                        time.sleep(d)
+                submit_t[i] = time.perf_counter()
+                if r[0] == "fit":
+                    tickets[i] = srv.submit_fit(r[1], r[2], r[3], 8, theta)
+                else:
+                    tickets[i] = srv.submit_predict(tickets[r[1]], r[2], r[3])
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        lat, bad = [], 0
+        for i in range(len(reqs)):
+            res = srv.wait(tickets[i])
+            lat.append(time.perf_counter() - submit_t[i])
+            bad += res["status"] != 0
+        total = time.perf_counter() - t0
+        for i, r in enumerate(reqs):
+            if r[0] == "fit":
+                srv.release_model(tickets[i])
+        return total, lat, bad
+
+    with capi.Server(0, slots=slots, hbm_budget=budget, precision=capi.PRECISION_MIXED) as srv:
+        run(srv, None)  # warm-up: device pools and code paths
+        sat_total, sat_lat, sat_bad = run(srv, None)
+        rate = 0.7 * len(reqs) / sat_total
+        gaps = np.random.default_rng(11).exponential(1.0 / rate, len(reqs))
+        po_total, po_lat, po_bad = run(srv, np.cumsum(gaps) - gaps[0])
+        stats = srv.stats()
+    n_fit = sum(r[0] == "fit" for r in reqs)
+    return {
+        "requests": len(reqs),
+        "fits": n_fit,
+        "predicts": len(reqs) - n_fit,
+        "slots": slots,
+        "hbm_budget_gb": budget / 2 ** 30,
+        "sequential_fp64_req_s": len(reqs) / seq_total,
+        "sequential_refit_req_s": len(reqs) / refit_total,
+        "saturated_req_s": len(reqs) / sat_total,
+        "speedup_vs_sequential": seq_total / sat_total,
+        "speedup_vs_sequential_refit": refit_total / sat_total,
+        "saturated_p50_ms": percentile(sat_lat, 50) * 1e3,
+        "saturated_p99_ms": percentile(sat_lat, 99) * 1e3,
+        "poisson_rate_req_s": rate,
+        "poisson_req_s": len(reqs) / po_total,
+        "poisson_p50_ms": percentile(po_lat, 50) * 1e3,
+        "poisson_p99_ms": percentile(po_lat, 99) * 1e3,
+        "sequential_p50_ms": percentile(times, 50) * 1e3,
+        "sequential_p99_ms": percentile(times, 99) * 1e3,
+        "failed": sat_bad + po_bad,
+        "server_stats": {k: stats[k] for k in ("batched_fits", "batch_launches", "evictions", "refits")},
+        "workload": "70% fits n log-uniform [256, 32768] (n_reg 8), 30% predicts m uniform [1, 4096] on earlier models",
+    }
 
 
 def main():
@@ -168,6 +329,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--chol-n", type=int, default=32768, help="size of the Cholesky-rate leg (0 skips it)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched many-GP leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this backend runs on one GPU; the GP path has no multi-GPU mode in this build "
@@ -201,7 +364,7 @@ def main():
     ctx.synchronize()
     total = time.perf_counter() - t0
     fams = {f: ctx.kernel_stats(f) for f in
-            ("syrk_trailing", "panel", "cholesky", "assemble", "solve", "trsm", "mean")}
+            ("syrk_trailing", "panel", "panel_rows", "cholesky", "assemble", "solve", "trsm", "mean")}
     ctx.set_profiling(False)
     gpu_mean, gpu_var = dmean.numpy(), dvar.numpy()
 
@@ -211,7 +374,7 @@ def main():
     syrk_flops = sum(per_launch) * args.steps
     achieved = syrk_flops / (syrk_ms * 1e-3) / 1e12 if syrk_ms > 0 else None
     roofline = {
-        "kernel": "k_gemm_f64<false,true,0> (Cholesky trailing update, FP64 MFMA)",
+        "kernel": "k_gemm_f64<true,false,5> (Cholesky trailing update from the k-major panel copy, FP64 MFMA)",
         "bound": "mfma",
         "achieved": achieved,
         "peak": FP64_MFMA_PEAK_TFLOPS,
@@ -229,6 +392,12 @@ def main():
     if args.chol_n > 0:
         chol = cholesky_rate(capi, ctx, args.chol_n)
     mixed = None if args.no_mixed else mixed_leg(capi, ctx)
+    mixed64k = None if args.no_mixed else mixed_leg(capi, ctx, n=65536, repeats=1)
+    batched = None if args.no_batched else batched_leg(capi, ctx)
+    for a in (dx, dy, dz, dmean, dvar):
+        a.free()
+    ctx.close()  # the serving leg brings its own contexts and HBM budget
+    serving = None if args.no_serving else serving_leg(capi)
 
     result = {
         "metric": METRIC,
@@ -254,6 +423,9 @@ def main():
         "cholesky_n16384_tflops": (n ** 3 / 3.0) / (chol_ms * 1e-3) / 1e12 if chol_ms > 0 else None,
         "cholesky_n32k": chol,
         "mixed_precision_n32k": mixed,
+        "mixed_precision_n64k": mixed64k,
+        "batched_2048_gps": batched,
+        "serving": serving,
         "phase_ms_per_step": {k: v[1] / args.steps for k, v in fams.items()},
         "roofline": roofline,
     }
@@ -263,9 +435,6 @@ def main():
         result["rel_err_vs_oracle"] = err
     else:
         result["cpu_baseline"] = None
-    for a in (dx, dy, dz, dmean, dvar):
-        a.free()
-    ctx.close()
     print(json.dumps(result), flush=True)
 
 

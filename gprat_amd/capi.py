@@ -13,13 +13,35 @@ import numpy as np
 
 LIBRARY_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgprat_amd.so")
 
-OK, ERR_ARGUMENT, ERR_HIP, ERR_OUT_OF_MEMORY, ERR_NOT_SPD, ERR_NO_DEVICE, ERR_INTERNAL = range(7)
+OK, ERR_ARGUMENT, ERR_HIP, ERR_OUT_OF_MEMORY, ERR_NOT_SPD, ERR_NO_DEVICE, ERR_INTERNAL, ERR_PENDING = range(8)
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 
 _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int)
 _VP = C.c_void_p
 _SZ = C.c_size_t
+_LL = C.c_longlong
+
+
+class ServerConfig(C.Structure):
+    """gprat_server_config."""
+    _fields_ = [("device", C.c_int), ("slots", C.c_int), ("hbm_budget", _SZ), ("precision", C.c_int),
+                ("batch_max_n", C.c_int), ("batch_max", C.c_int)]
+
+
+class RequestResult(C.Structure):
+    """gprat_request_result."""
+    _fields_ = [("status", C.c_int), ("kind", C.c_int), ("model", _LL), ("loss", C.c_double),
+                ("queue_ms", C.c_double), ("service_ms", C.c_double), ("refit", C.c_int), ("batched", C.c_int),
+                ("error", C.c_char * 256)]
+
+
+class ServerStats(C.Structure):
+    """gprat_server_stats."""
+    _fields_ = [(k, _LL) for k in ("submitted", "completed", "failed", "fits", "predicts", "batched_fits",
+                                   "batch_launches", "evictions", "refits", "resident_models")] + \
+               [(k, _SZ) for k in ("resident_bytes", "reserved_bytes", "peak_bytes", "budget")]
+
 
 # name -> argtypes (restype int unless listed in _RESTYPE)
 SIGNATURES = {
@@ -51,6 +73,13 @@ SIGNATURES = {
     "gprat_batch_losses": [_VP, _D],
     "gprat_batch_predict": [_VP, C.c_int, _D, _SZ, C.c_int, _D, _D],
     "gprat_batch_release": [_VP],
+    "gprat_server_create": [C.POINTER(ServerConfig), C.POINTER(_VP)],
+    "gprat_server_destroy": [_VP],
+    "gprat_server_submit_fit": [_VP, C.c_void_p, C.POINTER(_LL)],
+    "gprat_server_submit_predict": [_VP, _LL, _D, _SZ, C.c_int, _D, _D, C.POINTER(_LL)],
+    "gprat_server_wait": [_VP, _LL, C.c_double, C.POINTER(RequestResult)],
+    "gprat_server_release_model": [_VP, _LL],
+    "gprat_server_get_stats": [_VP, C.POINTER(ServerStats)],
     "gprat_context_set_profiling": [_VP, C.c_int],
     "gprat_context_kernel_stats": [_VP, C.c_char_p, C.POINTER(C.c_long), _D],
 }
@@ -342,5 +371,81 @@ class Batch:
     def __del__(self):
         try:
             self.release()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request loop with continuous batching on one GPU (gprat_server_*).
+
+    submit_fit / submit_predict return tickets at once; wait(ticket) returns a
+    dict with the request's status, timings and, for predicts, the mean and
+    variance arrays. A fit's ticket is also its model id."""
+
+    def __init__(self, device=0, slots=4, hbm_budget=0, precision=PRECISION_MIXED, batch_max_n=2048, batch_max=256):
+        cfg = ServerConfig(device, slots, hbm_budget, precision, batch_max_n, batch_max)
+        h = _VP()
+        _check(lib().gprat_server_create(C.byref(cfg), C.byref(h)))
+        self.handle = h
+        self._outputs = {}
+
+    def submit_fit(self, train_in, train_out, n, n_reg, theta) -> int:
+        x, y = _arr(train_in), _arr(train_out)
+        d = GpDesc()
+        d.train_in, d.train_in_len = _dptr(x), x.size
+        d.train_out, d.train_out_len = _dptr(y), y.size
+        d.n, d.n_reg = n, n_reg
+        for k in range(3):
+            d.theta[k] = float(theta[k])
+        t = _LL(0)
+        _check(lib().gprat_server_submit_fit(self.handle, C.cast(C.pointer(d), C.c_void_p), C.byref(t)))
+        return t.value
+
+    def submit_predict(self, model: int, test_in, m: int, variance=True) -> int:
+        z = _arr(test_in)
+        mean = np.empty(m)
+        var = np.empty(m) if variance else None
+        t = _LL(0)
+        _check(lib().gprat_server_submit_predict(self.handle, model, _dptr(z), z.size, m, _dptr(mean),
+                                                 _dptr(var) if variance else None, C.byref(t)))
+        self._outputs[t.value] = (mean, var)
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: float = -1.0):
+        """The finished request as a dict, or None if it is still running after timeout_ms."""
+        r = RequestResult()
+        st = lib().gprat_server_wait(self.handle, ticket, float(timeout_ms), C.byref(r))
+        if st == ERR_PENDING:
+            return None
+        _check(st)
+        out = {"status": r.status, "kind": "fit" if r.kind == 0 else "predict", "model": r.model, "loss": r.loss,
+               "queue_ms": r.queue_ms, "service_ms": r.service_ms, "refit": bool(r.refit), "batched": r.batched,
+               "error": r.error.decode()}
+        if ticket in self._outputs:
+            out["mean"], out["var"] = self._outputs.pop(ticket)
+        return out
+
+    def release_model(self, model: int):
+        _check(lib().gprat_server_release_model(self.handle, model))
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        _check(lib().gprat_server_get_stats(self.handle, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.handle:
+            lib().gprat_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass

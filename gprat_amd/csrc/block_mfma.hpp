@@ -1,0 +1,106 @@
+// 64x64 FP64 block products on the matrix cores for one 256-thread
+// workgroup, shared by the kernels that keep whole 64x64 blocks in LDS (the
+// batched fit and the 256-block inverses). Header-only so each kernel file
+// inlines them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace gpamd::k::blk
+{
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int NB = 64;
+constexpr int THREADS = 256;
+constexpr int XS = NB + 2;  // LDS row stride of staged blocks (bank-spread MFMA reads)
+
+// Copy a 64x64 row-major block (leading dimension ld) into LDS stride XS.
+__device__ __forceinline__ void stage_block(double *dst, const double *__restrict__ src, long ld)
+{
+    for (int e = threadIdx.x; e < NB * NB / 2; e += THREADS)
+    {
+        const int r = e >> 5, c = (e & 31) * 2;
+        const double2 v = *reinterpret_cast<const double2 *>(src + static_cast<long>(r) * ld + c);
+        dst[r * XS + c] = v.x;
+        dst[r * XS + c + 1] = v.y;
+    }
+}
+
+__device__ __forceinline__ void zero(v4d (&acc)[2][2])
+{
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+            acc[i][j] = v4d{ 0.0, 0.0, 0.0, 0.0 };
+}
+
+// acc (this wavefront's 32x32 quarter of a 64x64 block) += sum_k A[r][k] B[c][k]
+// with A and B staged in LDS (stride XS). Wavefront w owns rows
+// 32*(w>>1).., columns 32*(w&1)..
+__device__ __forceinline__ void mfma_nt_acc(const double *A, const double *B, v4d (&acc)[2][2])
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lm = lane & 15, lk = lane >> 4;
+    const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
+#pragma unroll 4
+    for (int k0 = 0; k0 < NB; k0 += 4)
+    {
+        double a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+            a[i] = A[(r0 + i * 16 + lm) * XS + k0 + lk];
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+            b[j] = B[(c0 + j * 16 + lm) * XS + k0 + lk];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+}
+
+__device__ __forceinline__ void mfma_nt(const double *A, const double *B, v4d (&acc)[2][2])
+{
+    zero(acc);
+    mfma_nt_acc(A, B, acc);
+}
+
+// Visit this lane's accumulator entries as f(row, col, value) within the
+// 64x64 block. f64 16x16x4 accumulator layout (measured on gfx950): lane
+// holds column lane%16, rows lane/16 + 4e.
+template <class F>
+__device__ __forceinline__ void for_each(const v4d (&acc)[2][2], F f)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lm = lane & 15, lk = lane >> 4;
+    const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                f(r0 + i * 16 + lk + 4 * e, c0 + j * 16 + lm, acc[i][j][e]);
+}
+
+// C (global 64x64 block, ld) = acc or C -= acc, optionally only on or below
+// the block's diagonal.
+__device__ __forceinline__ void store_acc(double *C, long ld, const v4d (&acc)[2][2], bool set, bool lower)
+{
+    for_each(acc,
+             [&](int r, int c, double v)
+             {
+                 if (lower && c > r)
+                     return;
+                 double *dst = C + static_cast<long>(r) * ld + c;
+                 if (set)
+                     *dst = v;
+                 else
+                     *dst -= v;
+             });
+}
+
+}  // namespace gpamd::k::blk

@@ -8,6 +8,7 @@
 #include <new>
 
 #include "engine.hpp"
+#include "server.hpp"
 
 namespace
 {
@@ -189,7 +190,7 @@ int gprat_context_create(int device, int n_streams, gprat_context **out)
             if (device < 0 || device >= n)
                 throw gpamd::Error(GPRAT_ERR_NO_DEVICE, "Requested GPU device is not available.");
             auto h = std::make_unique<ContextHandle>();
-            h->ctx = std::make_shared<gpamd::Context>(device, n_streams);
+            h->ctx = std::make_shared<gpamd::Context>(device, std::max(2, n_streams));
             *out = reinterpret_cast<gprat_context *>(h.release());
         });
 }
@@ -598,3 +599,134 @@ int gprat_context_kernel_stats(gprat_context *ctx, const char *family, long *lau
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Serving
+
+struct gprat_server
+{
+    std::unique_ptr<gpamd::Server> impl;
+};
+
+extern "C" int gprat_server_create(const gprat_server_config *config, gprat_server **out)
+{
+    return guarded(
+        [&]
+        {
+            gpamd::require(config != nullptr && out != nullptr, "server: null argument");
+            *out = nullptr;
+            gpamd::ServerConfig c;
+            c.device = config->device;
+            c.slots = config->slots;
+            c.budget = config->hbm_budget;
+            c.precision = config->precision;
+            c.batch_max_n = config->batch_max_n;
+            c.batch_max = config->batch_max;
+            auto h = std::make_unique<gprat_server>();
+            h->impl = std::make_unique<gpamd::Server>(c);
+            *out = h.release();
+        });
+}
+
+extern "C" int gprat_server_destroy(gprat_server *server)
+{
+    return guarded(
+        [&]
+        {
+            gpamd::require(server != nullptr, "server: null handle");
+            delete server;
+        });
+}
+
+extern "C" int gprat_server_submit_fit(gprat_server *server, const gprat_gp_desc *gp, long long *ticket)
+{
+    return guarded(
+        [&]
+        {
+            gpamd::require(server != nullptr && gp != nullptr && ticket != nullptr, "server: null argument");
+            *ticket = server->impl->submit_fit(*gp);
+        });
+}
+
+extern "C" int gprat_server_submit_predict(gprat_server *server,
+                                           long long model,
+                                           const double *test_in,
+                                           size_t test_in_len,
+                                           int m,
+                                           double *mean_out,
+                                           double *var_out,
+                                           long long *ticket)
+{
+    return guarded(
+        [&]
+        {
+            gpamd::require(server != nullptr && ticket != nullptr, "server: null argument");
+            *ticket = server->impl->submit_predict(model, test_in, test_in_len, m, mean_out, var_out);
+        });
+}
+
+extern "C" int gprat_server_wait(gprat_server *server, long long ticket, double timeout_ms, gprat_request_result *result)
+{
+    int pending = 0;
+    const int st = guarded(
+        [&]
+        {
+            gpamd::require(server != nullptr && result != nullptr, "server: null argument");
+            gpamd::RequestResult r;
+            if (!server->impl->wait(ticket, timeout_ms, r))
+            {
+                pending = 1;
+                return;
+            }
+            std::memset(result, 0, sizeof *result);
+            result->status = r.status;
+            result->kind = r.kind;
+            result->model = r.model;
+            result->loss = r.loss;
+            result->queue_ms = r.queue_ms;
+            result->service_ms = r.service_ms;
+            result->refit = r.refit;
+            result->batched = r.batched;
+            std::snprintf(result->error, sizeof result->error, "%s", r.error.c_str());
+        });
+    if (st == GPRAT_OK && pending)
+    {
+        g_last_error = "request " + std::to_string(ticket) + " has not finished";
+        return GPRAT_ERR_PENDING;
+    }
+    return st;
+}
+
+extern "C" int gprat_server_release_model(gprat_server *server, long long model)
+{
+    return guarded(
+        [&]
+        {
+            gpamd::require(server != nullptr, "server: null handle");
+            server->impl->release_model(model);
+        });
+}
+
+extern "C" int gprat_server_get_stats(gprat_server *server, gprat_server_stats *out)
+{
+    return guarded(
+        [&]
+        {
+            gpamd::require(server != nullptr && out != nullptr, "server: null argument");
+            const gpamd::ServerStats s = server->impl->stats();
+            out->submitted = s.submitted;
+            out->completed = s.completed;
+            out->failed = s.failed;
+            out->fits = s.fits;
+            out->predicts = s.predicts;
+            out->batched_fits = s.batched_fits;
+            out->batch_launches = s.batch_launches;
+            out->evictions = s.evictions;
+            out->refits = s.refits;
+            out->resident_models = s.resident_models;
+            out->resident_bytes = s.resident_bytes;
+            out->reserved_bytes = s.reserved_bytes;
+            out->peak_bytes = s.peak_bytes;
+            out->budget = s.budget;
+        });
+}

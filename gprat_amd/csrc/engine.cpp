@@ -46,6 +46,7 @@ static std::size_t size_class(std::size_t bytes)
 
 void *DevicePool::get(std::size_t bytes)
 {
+    std::lock_guard<std::mutex> g(mu_);
     const std::size_t want = size_class(bytes ? bytes : 1);
     auto it = free_.lower_bound(want);
     // reuse a cached block if it is not more than 25% larger than needed
@@ -62,7 +63,11 @@ void *DevicePool::get(std::size_t bytes)
     if (e != hipSuccess)
     {
         (void)hipGetLastError();
-        release_cached();
+        hipDeviceSynchronize();
+        for (auto &kv : free_)
+            hipFree(kv.second);
+        free_.clear();
+        cached_ = 0;
         e = hipMalloc(&p, want);
         if (e != hipSuccess)
         {
@@ -78,6 +83,7 @@ void DevicePool::put(void *p)
 {
     if (!p)
         return;
+    std::lock_guard<std::mutex> g(mu_);
     auto it = live_.find(p);
     if (it == live_.end())
         return;
@@ -86,13 +92,22 @@ void DevicePool::put(void *p)
     live_.erase(it);
 }
 
-void DevicePool::release_cached()
+void DevicePool::release_cached() { trim(0); }
+
+void DevicePool::trim(std::size_t keep)
 {
+    std::lock_guard<std::mutex> g(mu_);
+    if (cached_ <= keep)
+        return;
     hipDeviceSynchronize();
-    for (auto &kv : free_)
-        hipFree(kv.second);
-    free_.clear();
-    cached_ = 0;
+    // free the largest cached blocks first
+    while (cached_ > keep && !free_.empty())
+    {
+        auto it = std::prev(free_.end());
+        hipFree(it->second);
+        cached_ -= it->first;
+        free_.erase(it);
+    }
 }
 
 template <class T>
@@ -121,7 +136,10 @@ Context::Context(int dev, int ns) : device(dev), n_streams(ns), pool(dev)
 {
     make_current();
     GP_HIP(hipStreamCreateWithFlags(&main, hipStreamNonBlocking));
-    GP_HIP(hipStreamCreateWithFlags(&panel, hipStreamNonBlocking));
+    panel = trail = main;
+    // one stream: the panel factorisation runs in line (no look-ahead)
+    if (ns >= 2)
+        GP_HIP(hipStreamCreateWithFlags(&panel, hipStreamNonBlocking));
     info = DevBuf<int>(*this, 1);
     scalars = DevBuf<double>(*this, 64);
 }
@@ -139,10 +157,12 @@ Context::~Context()
         hipEventDestroy(e);
     info.reset();
     scalars.reset();
+    if (trail && trail != main)
+        hipStreamDestroy(trail);
+    if (panel && panel != main)
+        hipStreamDestroy(panel);
     if (main)
         hipStreamDestroy(main);
-    if (panel)
-        hipStreamDestroy(panel);
 }
 
 void Context::make_current() const { GP_HIP(hipSetDevice(device)); }
@@ -199,7 +219,10 @@ void Context::resolve_profiling()
 
 void Context::sync()
 {
-    GP_HIP(hipStreamSynchronize(panel));
+    if (panel != main)
+        GP_HIP(hipStreamSynchronize(panel));
+    if (trail != main)
+        GP_HIP(hipStreamSynchronize(trail));
     GP_HIP(hipStreamSynchronize(main));
     if (!pending.empty())
         resolve_profiling();
@@ -262,30 +285,177 @@ static int outer_block(int n_pad)
     return n_pad <= 4096 ? 256 : 512;
 }
 
+// C[i][j] -= sum_k P[k][off + i] P[k][off + j] for i < M, j < N, i >= j, with
+// P the k-major copy of a factored panel (leading dimension ldp).
+static void trailing_kmajor(hipStream_t s, const double *P, long ldp, long off, double *C, long ldc, int M, int N, int K)
+{
+    k::GemmArgs g{};
+    g.A = P + off;
+    g.lda = ldp;
+    g.B = P + off;
+    g.ldb = ldp;
+    g.C = C;
+    g.ldc = ldc;
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    g.a_trans = true;
+    g.b_trans = false;
+    g.lower = true;
+    g.role = k::GemmRole::TrailingT;
+    k::gemm_f64(s, g);
+}
+
+// Factor the diagonal block A[p0:pe, p0:pe] in place in 64-wide steps: the
+// fused factor + inverse of each 64x64 block (saved in Winv), the in-place
+// product with that inverse for the block's rows below it, and the update
+// of the rest of the block. Only rows inside the block are touched.
+static void factor_diagonal_block(Context &c, hipStream_t s, double *A, long lda, int p0, int pe, double *Winv)
+{
+    for (int c0 = p0; c0 < pe; c0 += 64)
+    {
+        double *w = Winv + static_cast<long>(c0 / 64) * 64 * 64;
+        k::potrf64_inv(s, A, lda, c0, w, c.info.get());
+        const int rest = pe - c0 - 64;
+        if (rest <= 0)
+            break;
+        const long o = static_cast<long>(c0 + 64) * lda;
+        k::GemmArgs g{};
+        g.A = A + o + c0;  // rows below: X = A W^T, in place (one column tile)
+        g.lda = lda;
+        g.B = w;
+        g.ldb = 64;
+        g.C = A + o + c0;
+        g.ldc = lda;
+        g.M = rest;
+        g.N = 64;
+        g.K = 64;
+        g.b_trans = true;
+        g.set = true;
+        g.role = k::GemmRole::PanelSolve;
+        k::gemm_f64(s, g);
+        k::GemmArgs u{};
+        u.A = A + o + c0;
+        u.lda = lda;
+        u.B = A + o + c0;
+        u.ldb = lda;
+        u.C = A + o + c0 + 64;
+        u.ldc = lda;
+        u.M = rest;
+        u.N = rest;
+        u.K = 64;
+        u.b_trans = true;
+        u.lower = true;
+        u.role = k::GemmRole::Panel;
+        k::gemm_f64(s, u);
+    }
+}
+
+// Wb (k-major, leading dimension ldw: W(r, c) at Wb[c * ldw + r]) = inverse
+// of the factored diagonal block L[p0:p0+w, p0:p0+w]: the 256-wide diagonal
+// inverses first, then adjacent halves joined bottom-up by
+//   W_bt = -W_b L_bt W_t
+// (two MFMA products through `tmp`, at least w/2 x w/2 doubles). Wb must be
+// zero on entry (its strict upper part is never written).
+static void invert_diagonal_block(hipStream_t s, const double *A, long lda, const double *Winv, int p0, int w, double *Wb, long ldw,
+                                  double *tmp)
+{
+    k::invert_blocks256(s, A, lda, Winv, p0, w, Wb, ldw, 256 * ldw + 256);
+    for (int size = 256; size < w; size *= 2)
+        for (int t0 = 0; t0 + size < w; t0 += 2 * size)
+        {
+            const int sb = std::min(size, w - t0 - size);
+            // tmp = -L_bt W_t  (sb x size, leading dimension size)
+            k::fill(s, tmp, static_cast<long>(sb) * size, 0.0);
+            k::GemmArgs g{};
+            g.A = A + static_cast<long>(p0 + t0 + size) * lda + p0 + t0;
+            g.lda = lda;
+            g.B = Wb + static_cast<long>(t0) * ldw + t0;  // B(k, j) = W_t(k, j)
+            g.ldb = ldw;
+            g.b_trans = true;
+            g.C = tmp;
+            g.ldc = size;
+            g.M = sb;
+            g.N = size;
+            g.K = size;
+            g.role = k::GemmRole::Panel;
+            k::gemm_f64(s, g);
+            // W_bt^T = tmp^T W_b^T, stored straight into the k-major layout
+            k::GemmArgs h{};
+            h.A = tmp;  // A(j, k) = tmp(k, j)
+            h.lda = size;
+            h.a_trans = true;
+            h.B = Wb + static_cast<long>(t0 + size) * ldw + t0 + size;  // B(k, i) = W_b(i, k)
+            h.ldb = ldw;
+            h.C = Wb + static_cast<long>(t0) * ldw + t0 + size;
+            h.ldc = ldw;
+            h.M = size;
+            h.N = sb;
+            h.K = sb;
+            h.set = true;
+            h.role = k::GemmRole::Inverse;
+            k::gemm_f64(s, h);
+        }
+}
+
+// Rows r0.. of the panel columns [p0, p0 + w): X = A W^T with W the inverse
+// of the panel's diagonal block (k-major Wb). In place, 128 columns per
+// launch from right to left: X[:, c:c+128] needs A[:, 0:c+128] only (W is
+// lower triangular) and every launch has a single column tile, so no
+// workgroup reads what another writes.
+static void solve_panel_rows(hipStream_t s, double *A, long lda, int p0, int w, int r0, int n_pad, const double *Wb, long ldw)
+{
+    if (r0 >= n_pad)
+        return;
+    for (int cc = w - 128; cc >= 0; cc -= 128)
+    {
+        k::GemmArgs g{};
+        g.A = A + static_cast<long>(r0) * lda + p0;
+        g.lda = lda;
+        g.B = Wb + cc;  // B(k, j) = W(cc + j, k)
+        g.ldb = ldw;
+        g.C = A + static_cast<long>(r0) * lda + p0 + cc;
+        g.ldc = lda;
+        g.M = n_pad - r0;
+        g.N = 128;
+        g.K = cc + 128;
+        g.set = true;
+        g.role = k::GemmRole::PanelX;
+        k::gemm_f64(s, g);
+    }
+}
+
 // ---------------------------------------------------------------------------
-// FP64 Cholesky: blocked right-looking with one panel of look-ahead.
+// Cholesky: blocked right-looking with one panel of look-ahead.
 //
-// Outer panels of width Bo are factored on the `panel` stream in 64-wide
-// steps (fused POTRF+TRSM, then an MFMA update of the rest of the panel).
-// On the `main` stream the trailing update of step k first refreshes panel
-// k+1 only, releases it to the panel stream, and then updates the remaining
-// lower trailing matrix while panel k+1 is being factored.
+// A panel of width Bo is factored in two parts. Its diagonal block (Bo x Bo)
+// is factored in 64-wide steps and inverted on the `panel` stream; the rows
+// below it are then one set of MFMA products with that inverse on the
+// `trail` stream, followed by a k-major copy of the panel that feeds the
+// trailing update as the A^T operand. The latency-bound diagonal-block work
+// is a few small launches per panel, so it slots in beside the update.
+// Step k on `trail`: refresh panel k+1's columns, release its diagonal block
+// to `panel`, update the rest of the trailing matrix while that block is
+// being factored, then solve panel k+1's rows.
 void cholesky_fp64(Context &c, double *A, long lda, int n_pad, double *Winv) { cholesky(c, A, lda, n_pad, Winv, false); }
 
 void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mixed)
 {
     require(n_pad % 128 == 0 && lda >= n_pad && lda % 2 == 0, "cholesky: bad layout");
-    hipStream_t U = c.main, P = use_lookahead() ? c.panel : c.main;
+    const bool ahead = use_lookahead();
+    hipStream_t T = ahead ? c.trail : c.main, P = ahead ? c.panel : c.main;
     const int Bo = outer_block(n_pad);
     const int nblk = static_cast<int>(ceil_div(n_pad, Bo));
-    k::set_int(U, c.info.get(), INT_MAX);
-    // Mixed precision: FP32 k-major copies of the factored panel's trailing
-    // rows, double buffered so panel k+1 can be converted while the trailing
-    // update of step k still reads panel k.
-    DevBuf<float> p32[2];
+    k::set_int(c.main, c.info.get(), INT_MAX);
+    // k-major copy of the factored panel's rows below its diagonal block
+    // (FP32 for mixed precision), the diagonal block's inverse and scratch
+    DevBuf<float> p32;
+    DevBuf<double> p64;
     if (mixed)
-        for (auto &b : p32)
-            b = DevBuf<float>(c, static_cast<std::size_t>(Bo) * n_pad);
+        p32 = DevBuf<float>(c, static_cast<std::size_t>(Bo) * n_pad);
+    else
+        p64 = DevBuf<double>(c, static_cast<std::size_t>(Bo) * n_pad);
+    DevBuf<double> Wb(c, static_cast<std::size_t>(Bo) * Bo), tmp(c, static_cast<std::size_t>(Bo / 2) * (Bo / 2) + 256 * 256);
 
     std::vector<hipEvent_t> used;
     auto ev = [&]()
@@ -294,127 +464,68 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
         used.push_back(e);
         return e;
     };
+    auto hand_over = [&](hipStream_t from, hipStream_t to)
     {
-    Timed whole(c, U, "cholesky");
-    hipEvent_t start = ev();
-    GP_HIP(hipEventRecord(start, U));
-    GP_HIP(hipStreamWaitEvent(P, start, 0));
-
-    for (int kb = 0; kb < nblk; kb++)
+        if (from == to)
+            return;
+        hipEvent_t e = ev();
+        GP_HIP(hipEventRecord(e, from));
+        GP_HIP(hipStreamWaitEvent(to, e, 0));
+    };
+    auto diagonal = [&](int p0, int pe)
     {
-        const int p0 = kb * Bo;
-        const int pe = std::min(n_pad, p0 + Bo);
+        Timed t(c, P, "panel");
+        factor_diagonal_block(c, P, A, lda, p0, pe, Winv);
+        if (pe < n_pad)
         {
-            Timed t(c, P, "panel");
-            for (int c0 = p0; c0 < pe; c0 += 64)
-            {
-                double *w = Winv + static_cast<long>(c0 / 64) * 64 * 64;
-                k::potrf64_inv(P, A, lda, c0, w, c.info.get());
-                if (c0 + 64 >= n_pad)
-                    continue;
-                const long o = static_cast<long>(c0 + 64) * lda;
-                {
-                    // rows below: X = A W^T, in place
-                    k::GemmArgs g{};
-                    g.A = A + o + c0;
-                    g.lda = lda;
-                    g.B = w;
-                    g.ldb = 64;
-                    g.C = A + o + c0;
-                    g.ldc = lda;
-                    g.M = n_pad - c0 - 64;
-                    g.N = 64;
-                    g.K = 64;
-                    g.b_trans = true;
-                    g.set = true;
-                    g.role = k::GemmRole::PanelSolve;
-                    k::gemm_f64(P, g);
-                }
-                if (c0 + 64 < pe)
-                {
-                    k::GemmArgs g{};
-                    g.A = A + o + c0;
-                    g.lda = lda;
-                    g.B = A + o + c0;
-                    g.ldb = lda;
-                    g.C = A + o + c0 + 64;
-                    g.ldc = lda;
-                    g.M = n_pad - c0 - 64;
-                    g.N = pe - c0 - 64;
-                    g.K = 64;
-                    g.b_trans = true;
-                    g.lower = true;
-                    g.diag_off = 0;
-                    g.role = k::GemmRole::Panel;
-                    k::gemm_f64(P, g);
-                }
-            }
+            k::fill(P, Wb.get(), static_cast<long>(Bo) * Bo, 0.0);
+            invert_diagonal_block(P, A, lda, Winv, p0, pe - p0, Wb.get(), Bo, tmp.get());
         }
-        if (mixed && pe < n_pad)
-            k::panel_to_f32t(P, A, lda, pe, p0, n_pad - pe, pe - p0, p32[kb & 1].get(), n_pad);
-        hipEvent_t panel_done = ev();
-        GP_HIP(hipEventRecord(panel_done, P));
-        if (pe >= n_pad)
-        {
-            GP_HIP(hipStreamWaitEvent(U, panel_done, 0));
-            break;
-        }
-        GP_HIP(hipStreamWaitEvent(U, panel_done, 0));
-        const int q1 = std::min(n_pad, pe + Bo);
+    };
+    auto rows_below = [&](int p0, int pe)
+    {
+        Timed t(c, T, "panel_rows");
+        solve_panel_rows(T, A, lda, p0, pe - p0, pe, n_pad, Wb.get(), Bo);
         if (mixed)
-        {
-            Timed t(c, U, "syrk_trailing");
-            k::gemm_f32_trailing(U, p32[kb & 1].get(), n_pad, 0, 0, A + static_cast<long>(pe) * lda + pe, lda, n_pad - pe,
-                                 q1 - pe, pe - p0, 0);
-        }
+            k::panel_to_f32t(T, A, lda, pe, p0, n_pad - pe, pe - p0, p32.get(), n_pad);
         else
+            k::panel_to_f64t(T, A, lda, pe, p0, n_pad - pe, pe - p0, p64.get(), n_pad);
+    };
+    auto trailing = [&](long off, double *C, int M, int N, int K)
+    {
+        Timed t(c, T, "syrk_trailing");
+        if (mixed)
+            k::gemm_f32_trailing(T, p32.get(), n_pad, off, off, C, lda, M, N, K, 0);
+        else
+            trailing_kmajor(T, p64.get(), n_pad, off, C, lda, M, N, K);
+    };
+    {
+        Timed whole(c, c.main, "cholesky");
+        hand_over(c.main, P);
+        hand_over(c.main, T);
         {
-            Timed t(c, U, "syrk_trailing");
-            const long o = static_cast<long>(pe) * lda;
-            k::GemmArgs g{};
-            g.A = A + o + p0;
-            g.lda = lda;
-            g.B = A + o + p0;
-            g.ldb = lda;
-            g.C = A + o + pe;
-            g.ldc = lda;
-            g.M = n_pad - pe;
-            g.N = q1 - pe;
-            g.K = pe - p0;
-            g.b_trans = true;
-            g.lower = true;
-            g.role = k::GemmRole::Trailing;
-            k::gemm_f64(U, g);
+            const int pe = std::min(n_pad, Bo);
+            diagonal(0, pe);
+            hand_over(P, T);
+            if (pe < n_pad)
+                rows_below(0, pe);
         }
-        hipEvent_t next_ready = ev();
-        GP_HIP(hipEventRecord(next_ready, U));
-        GP_HIP(hipStreamWaitEvent(P, next_ready, 0));
-        if (q1 < n_pad && mixed)
+        for (int kb = 0; kb + 1 < nblk; kb++)
         {
-            Timed t(c, U, "syrk_trailing");
-            k::gemm_f32_trailing(U, p32[kb & 1].get(), n_pad, q1 - pe, q1 - pe, A + static_cast<long>(q1) * lda + q1, lda,
-                                 n_pad - q1, n_pad - q1, pe - p0, 0);
+            const int p0 = kb * Bo, pe = std::min(n_pad, p0 + Bo);
+            const int q1 = std::min(n_pad, pe + Bo);
+            // panel k+1's columns first, then hand its diagonal block over
+            trailing(0, A + static_cast<long>(pe) * lda + pe, n_pad - pe, q1 - pe, pe - p0);
+            hand_over(T, P);
+            diagonal(pe, q1);
+            if (q1 < n_pad)
+                trailing(q1 - pe, A + static_cast<long>(q1) * lda + q1, n_pad - q1, n_pad - q1, pe - p0);
+            hand_over(P, T);
+            if (q1 < n_pad)
+                rows_below(pe, q1);
         }
-        else if (q1 < n_pad)
-        {
-            Timed t(c, U, "syrk_trailing");
-            const long o = static_cast<long>(q1) * lda;
-            k::GemmArgs g{};
-            g.A = A + o + p0;
-            g.lda = lda;
-            g.B = A + o + p0;
-            g.ldb = lda;
-            g.C = A + o + q1;
-            g.ldc = lda;
-            g.M = n_pad - q1;
-            g.N = n_pad - q1;
-            g.K = pe - p0;
-            g.b_trans = true;
-            g.lower = true;
-            g.role = k::GemmRole::Trailing;
-            k::gemm_f64(U, g);
-        }
-    }
+        hand_over(T, c.main);
+        hand_over(P, c.main);
     }
     c.sync();
     for (auto e : used)
@@ -485,6 +596,107 @@ void trsm_lower_left(Context &c, const double *L, long lda, const double *Winv, 
     g.role = k::GemmRole::Solve;
     k::gemm_f64(c.main, g);
     trsm_lower_left(c, L, lda, Winv, B, ldb, ncols, mid, b, tri_rhs);
+}
+
+// out = L^-1 B, blocked right-looking in 256-wide steps: the diagonal step
+// out_J = W256_J B_J uses the k-major 256-block inverses, the update
+// B[below] -= L[below, J] out_J reads a k-major copy of L's column panel, so
+// both products take whole 1 KiB rows of their A operand. B is
+// overwritten. With tri_rhs, B is lower triangular (B = I for L^-1) and each
+// step's columns stop at the end of its block. With `staged`, out is a
+// single 256-row block reused by every step (only the squares are wanted);
+// colsq, when given, accumulates the column sums of squares of out over
+// rows, for columns < mt.
+void trsm_blocked(Context &c, const double *L, long lda, const double *Winv, int n_pad, double *B, long ldb, int ncols, bool tri_rhs,
+                  double *out, long ldo, bool staged, double *colsq, int mt)
+{
+    constexpr int NBK = 256;
+    hipStream_t s = c.main;
+    const int nblk = static_cast<int>(ceil_div(n_pad, NBK));
+    DevBuf<double> Wt(c, static_cast<std::size_t>(nblk) * NBK * NBK);
+    DevBuf<double> Pt(c, static_cast<std::size_t>(NBK) * n_pad);
+    k::invert_blocks256(s, L, lda, Winv, 0, n_pad, Wt.get(), NBK, static_cast<long>(NBK) * NBK);
+    for (int J = 0; J < nblk; J++)
+    {
+        const int j0 = J * NBK, nb = std::min(NBK, n_pad - j0), j1 = j0 + nb;
+        const int cols = tri_rhs ? std::min(ncols, j1) : ncols;
+        double *oJ = staged ? out : out + static_cast<long>(j0) * ldo;
+        {
+            k::GemmArgs g{};
+            g.A = Wt.get() + static_cast<long>(J) * NBK * NBK;
+            g.lda = NBK;
+            g.a_trans = true;
+            g.B = B + static_cast<long>(j0) * ldb;
+            g.ldb = ldb;
+            g.C = oJ;
+            g.ldc = ldo;
+            g.M = nb;
+            g.N = cols;
+            g.K = nb;
+            g.set = true;
+            g.role = k::GemmRole::LeafT;
+            k::gemm_f64(s, g);
+        }
+        if (colsq)
+            k::colsq_accumulate(s, oJ, ldo, nb, mt, colsq);
+        if (j1 >= n_pad)
+            break;
+        k::panel_to_f64t(s, L, lda, j1, j0, n_pad - j1, nb, Pt.get(), n_pad);
+        k::GemmArgs u{};
+        u.A = Pt.get();
+        u.lda = n_pad;
+        u.a_trans = true;
+        u.B = oJ;
+        u.ldb = ldo;
+        u.C = B + static_cast<long>(j1) * ldb;
+        u.ldc = ldb;
+        u.M = n_pad - j1;
+        u.N = cols;
+        u.K = nb;
+        u.role = k::GemmRole::SolveT;
+        k::gemm_f64(s, u);
+    }
+}
+
+// Column sums of squares of V = L^-1 K*^T for a mixed-precision model: the
+// same 256-step blocked solve as trsm_blocked, with FP32 copies of the block
+// inverses, of L's column panels and of K*^T on FP32 matrix cores; the
+// squares are summed in FP64.
+static void variance_colsq_f32(Context &c, const FactorView &m, const double *z, int mt, int m_pad, double *colsq)
+{
+    constexpr int NBK = 256;
+    hipStream_t s = c.main;
+    const int n_pad = m.n_pad;
+    const int nblk = static_cast<int>(ceil_div(n_pad, NBK));
+    const std::size_t wcount = static_cast<std::size_t>(nblk) * NBK * NBK;
+    DevBuf<float> Wt32(c, wcount);
+    {
+        DevBuf<double> Wt(c, wcount);
+        k::invert_blocks256(s, m.L, m.lda, m.Winv, 0, n_pad, Wt.get(), NBK, static_cast<long>(NBK) * NBK);
+        k::to_f32(s, Wt.get(), Wt32.get(), static_cast<long>(wcount));
+        c.sync();  // Wt goes back to the pool
+    }
+    DevBuf<float> B(c, static_cast<std::size_t>(n_pad) * m_pad), V(c, static_cast<std::size_t>(NBK) * m_pad),
+        Pt(c, static_cast<std::size_t>(NBK) * n_pad);
+    {
+        Timed t(c, s, "assemble");
+        k::assemble_cross_t_f32(s, m.x, z, m.r, m.sek, B.get(), m_pad, m.n, n_pad, mt, m_pad);
+    }
+    Timed t(c, s, "trsm");
+    k::fill(s, colsq, m_pad, 0.0);
+    for (int J = 0; J < nblk; J++)
+    {
+        const int j0 = J * NBK, nb = std::min(NBK, n_pad - j0), j1 = j0 + nb;
+        k::gemm_f32_tn(s, Wt32.get() + static_cast<long>(J) * NBK * NBK, NBK, B.get() + static_cast<long>(j0) * m_pad, m_pad,
+                       V.get(), m_pad, nb, m_pad, nb, true);
+        k::colsq_accumulate_f32(s, V.get(), m_pad, nb, mt, colsq);
+        if (j1 >= n_pad)
+            break;
+        k::panel_to_f32t(s, m.L, m.lda, j1, j0, n_pad - j1, nb, Pt.get(), n_pad);
+        k::gemm_f32_tn(s, Pt.get(), n_pad, V.get(), m_pad, B.get() + static_cast<long>(j1) * m_pad, m_pad, n_pad - j1, m_pad, nb,
+                       false);
+    }
+    c.sync();
 }
 
 // ---------------------------------------------------------------------------
@@ -621,7 +833,16 @@ void predict_view(const FactorView &m, const double *z, std::size_t z_len, int m
         k::predict_mean(c.main, m.x, dz.get(), m.alpha, m.r, m.sek, m.n, mt, partial.get(), splits, dmean.get());
     }
     DevBuf<double> dvar;
-    if (var)
+    if (var && m.mixed)
+    {
+        const int m_pad = static_cast<int>(round_up(mt, 64));
+        DevBuf<double> colsq(c, m_pad);
+        dvar = DevBuf<double>(c, mt);
+        variance_colsq_f32(c, m, dz.get(), mt, m_pad, colsq.get());
+        k::variance_finish(c.main, colsq.get(), m.sek.vertical, mt, dvar.get());
+        GP_HIP(hipMemcpyAsync(var, dvar.get(), sizeof(double) * mt, hipMemcpyDefault, c.main));
+    }
+    else if (var)
     {
         const int m_pad = static_cast<int>(round_up(mt, 64));
         const long ldb = m_pad;
@@ -634,11 +855,10 @@ void predict_view(const FactorView &m, const double *z, std::size_t z_len, int m
         }
         {
             Timed t(c, c.main, "trsm");
-            trsm_lower_left(c, m.L, m.lda, m.Winv, B.get(), ldb, m_pad, 0, m.n_pad, false);
+            DevBuf<double> V(c, static_cast<std::size_t>(256) * ldb);
+            k::fill(c.main, colsq.get(), m_pad, 0.0);
+            trsm_blocked(c, m.L, m.lda, m.Winv, m.n_pad, B.get(), ldb, m_pad, false, V.get(), ldb, true, colsq.get(), mt);
         }
-        const int splits = k::colsq_splits(m.n, mt);
-        DevBuf<double> cpart(c, static_cast<std::size_t>(splits) * mt);
-        k::column_sumsq(c.main, B.get(), ldb, m.n, mt, cpart.get(), splits, colsq.get());
         k::variance_finish(c.main, colsq.get(), m.sek.vertical, mt, dvar.get());
         GP_HIP(hipMemcpyAsync(var, dvar.get(), sizeof(double) * mt, hipMemcpyDefault, c.main));
     }
@@ -648,12 +868,56 @@ void predict_view(const FactorView &m, const double *z, std::size_t z_len, int m
 
 void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean, double *var)
 {
-    predict_view(FactorView{ m.ctx, m.x.get(), m.n, m.n_pad, m.r, m.lda, m.sek, m.L.get(), m.Winv.get(), m.alpha.get() }, z,
-                 z_len, mt, mean, var);
+    predict_view(FactorView{ m.ctx, m.x.get(), m.n, m.n_pad, m.r, m.lda, m.sek, m.L.get(), m.Winv.get(), m.alpha.get(),
+                             m.precision == GPRAT_PRECISION_MIXED },
+                 z, z_len, mt, mean, var);
 }
 
 // ---------------------------------------------------------------------------
 // Batched fit of many GPs
+
+// One persistent launch over GPs whose device buffers are already filled
+// (series and zero-padded targets); losses[g] receives GP g's loss and the
+// result's entry g the first non-positive pivot column of GP g (INT_MAX if
+// its matrix was positive definite).
+std::vector<int> batch_run(Context &c, const std::vector<k::BatchGP> &gps, double *losses_host)
+{
+    const int count = static_cast<int>(gps.size());
+    require(count >= 1, "batch: need at least one GP");
+    c.make_current();
+    // largest GPs first, so the persistent workgroups finish together
+    std::vector<int> order(count);
+    for (int g = 0; g < count; g++)
+        order[g] = g;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int z) { return gps[a].n > gps[z].n; });
+    DevBuf<double> desc(c, (sizeof(k::BatchGP) * count + 7) / 8);
+    DevBuf<int> dorder(c, count), info(c, count), counter(c, 1);
+    DevBuf<double> losses(c, count);
+    GP_HIP(hipMemcpyAsync(desc.get(), gps.data(), sizeof(k::BatchGP) * count, hipMemcpyDefault, c.main));
+    GP_HIP(hipMemcpyAsync(dorder.get(), order.data(), sizeof(int) * count, hipMemcpyDefault, c.main));
+    hipDeviceProp_t prop;
+    GP_HIP(hipGetDeviceProperties(&prop, c.device));
+    const int workgroups = std::max(1, std::min(count, prop.multiProcessorCount));
+    {
+        Timed t(c, c.main, "batch_fit");
+        k::batch_fit(c.main, reinterpret_cast<const k::BatchGP *>(desc.get()), dorder.get(), count, counter.get(), losses.get(),
+                     info.get(), workgroups);
+    }
+    std::vector<int> hinfo(count);
+    GP_HIP(hipMemcpyAsync(hinfo.data(), info.get(), sizeof(int) * count, hipMemcpyDefault, c.main));
+    GP_HIP(hipMemcpyAsync(losses_host, losses.get(), sizeof(double) * count, hipMemcpyDefault, c.main));
+    c.sync();
+    return hinfo;
+}
+
+static void check_desc(const gprat_gp_desc &d, int g)
+{
+    require(d.n >= 1 && d.n_reg >= 1, "batch: GP " + std::to_string(g) + ": n and n_reg must be >= 1");
+    require(d.train_in != nullptr && d.train_in_len >= static_cast<std::size_t>(d.n) + d.n_reg - 1,
+            "batch: GP " + std::to_string(g) + ": training input must hold n + n_reg - 1 values");
+    require(d.train_out != nullptr && d.train_out_len >= static_cast<std::size_t>(d.n),
+            "batch: GP " + std::to_string(g) + ": training output must hold n values");
+}
 
 void batch_fit(Context &c, Batch &b, int count, const gprat_gp_desc *gps, int precision)
 {
@@ -664,94 +928,69 @@ void batch_fit(Context &c, Batch &b, int count, const gprat_gp_desc *gps, int pr
     b.count = count;
     b.gps.resize(count);
     // arena layout: [inputs of all GPs][factors, inverses, alpha of all GPs]
+    std::vector<long> x_off(count), y_off(count), l_off(count), w_off(count), a_off(count);
     long in_total = 0, out_total = 0;
     for (int g = 0; g < count; g++)
     {
-        const gprat_gp_desc &d = gps[g];
-        require(d.n >= 1 && d.n_reg >= 1, "batch: GP " + std::to_string(g) + ": n and n_reg must be >= 1");
-        require(d.train_in != nullptr && d.train_in_len >= static_cast<std::size_t>(d.n) + d.n_reg - 1,
-                "batch: GP " + std::to_string(g) + ": training input must hold n + n_reg - 1 values");
-        require(d.train_out != nullptr && d.train_out_len >= static_cast<std::size_t>(d.n),
-                "batch: GP " + std::to_string(g) + ": training output must hold n values");
-        const long np = round_up(d.n, 64);
-        k::BatchGP &q = b.gps[g];
-        q.n = d.n;
-        q.n_pad = static_cast<int>(np);
-        q.n_reg = d.n_reg;
-        q.lengthscale = d.theta[0];
-        q.vertical = d.theta[1];
-        q.noise = d.theta[2];
-        q.x_off = in_total;
-        in_total += round_up(d.n + d.n_reg - 1, 2);
-        q.y_off = in_total;
+        check_desc(gps[g], g);
+        const long np = round_up(gps[g].n, 64);
+        x_off[g] = in_total;
+        in_total += round_up(gps[g].n + gps[g].n_reg - 1, 2);
+        y_off[g] = in_total;
         in_total += np;
     }
     for (int g = 0; g < count; g++)
     {
-        k::BatchGP &q = b.gps[g];
-        const long np = q.n_pad;
-        q.l_off = in_total + out_total;
+        const long np = round_up(gps[g].n, 64);
+        l_off[g] = in_total + out_total;
         out_total += np * np;
-        q.w_off = in_total + out_total;
+        w_off[g] = in_total + out_total;
         out_total += np * 64;
-        q.a_off = in_total + out_total;
+        a_off[g] = in_total + out_total;
         out_total += np;
     }
     std::vector<double> staging(static_cast<std::size_t>(in_total), 0.0);
     for (int g = 0; g < count; g++)
     {
-        const k::BatchGP &q = b.gps[g];
-        std::copy(gps[g].train_in, gps[g].train_in + q.n + q.n_reg - 1, staging.begin() + q.x_off);
-        std::copy(gps[g].train_out, gps[g].train_out + q.n, staging.begin() + q.y_off);
+        std::copy(gps[g].train_in, gps[g].train_in + gps[g].n + gps[g].n_reg - 1, staging.begin() + x_off[g]);
+        std::copy(gps[g].train_out, gps[g].train_out + gps[g].n, staging.begin() + y_off[g]);
     }
-    // largest GPs first, so the persistent workgroups finish together
-    std::vector<int> order(count);
-    for (int g = 0; g < count; g++)
-        order[g] = g;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int z) { return b.gps[a].n > b.gps[z].n; });
-
     b.arena = DevBuf<double>(c, static_cast<std::size_t>(in_total + out_total));
-    b.desc = DevBuf<double>(c, (sizeof(k::BatchGP) * count + 7) / 8);
-    b.order = DevBuf<int>(c, count);
-    b.info = DevBuf<int>(c, count);
-    b.counter = DevBuf<int>(c, 1);
-    b.losses = DevBuf<double>(c, count);
-    GP_HIP(hipMemcpyAsync(b.arena.get(), staging.data(), staging.size() * sizeof(double), hipMemcpyDefault, c.main));
-    GP_HIP(hipMemcpyAsync(b.desc.get(), b.gps.data(), sizeof(k::BatchGP) * count, hipMemcpyDefault, c.main));
-    GP_HIP(hipMemcpyAsync(b.order.get(), order.data(), sizeof(int) * count, hipMemcpyDefault, c.main));
-    hipDeviceProp_t prop;
-    GP_HIP(hipGetDeviceProperties(&prop, c.device));
-    const int workgroups = std::max(1, std::min(count, prop.multiProcessorCount));
+    double *base = b.arena.get();
+    for (int g = 0; g < count; g++)
     {
-        Timed t(c, c.main, "batch_fit");
-        k::batch_fit(c.main, reinterpret_cast<const k::BatchGP *>(b.desc.get()), b.order.get(), count, b.counter.get(),
-                     b.arena.get(), b.losses.get(), b.info.get(), workgroups);
+        const gprat_gp_desc &d = gps[g];
+        k::BatchGP &q = b.gps[g];
+        q = k::BatchGP{};
+        q.x = base + x_off[g];
+        q.y = base + y_off[g];
+        q.L = base + l_off[g];
+        q.W = base + w_off[g];
+        q.alpha = base + a_off[g];
+        q.n = d.n;
+        q.n_pad = static_cast<int>(round_up(d.n, 64));
+        q.n_reg = d.n_reg;
+        q.lengthscale = d.theta[0];
+        q.vertical = d.theta[1];
+        q.noise = d.theta[2];
     }
-    std::vector<int> info(count);
-    GP_HIP(hipMemcpyAsync(info.data(), b.info.get(), sizeof(int) * count, hipMemcpyDefault, c.main));
-    c.sync();
+    GP_HIP(hipMemcpyAsync(base, staging.data(), staging.size() * sizeof(double), hipMemcpyDefault, c.main));
+    b.losses.assign(count, 0.0);
+    const std::vector<int> info = batch_run(c, b.gps, b.losses.data());
     for (int g = 0; g < count; g++)
         if (info[g] != INT_MAX)
-            throw Error(GPRAT_ERR_NOT_SPD, "batch: GP " + std::to_string(g) +
-                                               ": covariance matrix is not positive definite (first non-positive pivot at column " +
-                                               std::to_string(info[g]) + ")");
+            throw Error(GPRAT_ERR_NOT_SPD, "batch: GP " + std::to_string(g)
+                                               + ": covariance matrix is not positive definite (first non-positive pivot at column "
+                                               + std::to_string(info[g]) + ")");
 }
 
-void batch_losses(Batch &b, double *losses)
-{
-    Context &c = *b.ctx;
-    c.make_current();
-    GP_HIP(hipMemcpyAsync(losses, b.losses.get(), sizeof(double) * b.count, hipMemcpyDefault, c.main));
-    c.sync();
-}
+void batch_losses(Batch &b, double *losses) { std::copy(b.losses.begin(), b.losses.end(), losses); }
 
 void batch_predict(Batch &b, int index, const double *z, std::size_t z_len, int mt, double *mean, double *var)
 {
     require(index >= 0 && index < b.count, "batch: GP index out of range");
     const k::BatchGP &q = b.gps[index];
-    double *base = b.arena.get();
-    predict_view(FactorView{ b.ctx, base + q.x_off, q.n, q.n_pad, q.n_reg, q.n_pad, SEK{ q.lengthscale, q.vertical, q.noise },
-                             base + q.l_off, base + q.w_off, base + q.a_off },
+    predict_view(FactorView{ b.ctx, q.x, q.n, q.n_pad, q.n_reg, q.n_pad, SEK{ q.lengthscale, q.vertical, q.noise }, q.L, q.W, q.alpha },
                  z, z_len, mt, mean, var);
 }
 
@@ -814,10 +1053,14 @@ void loss_gradient(Context &c,
     DevBuf<double> W(c, elems);
     DevBuf<double> Kinv(c, elems);
     k::fill(c.main, W.get(), static_cast<long>(elems), 0.0);
-    k::set_diagonal(c.main, W.get(), m.lda, m.n_pad, 1.0);
+    {
+        // the identity right-hand side lives in Kinv until the product overwrites it
+        k::fill(c.main, Kinv.get(), static_cast<long>(elems), 0.0);
+        k::set_diagonal(c.main, Kinv.get(), m.lda, m.n_pad, 1.0);
+    }
     {
         Timed tm(c, c.main, "inverse");
-        trsm_lower_left(c, m.L.get(), m.lda, m.Winv.get(), W.get(), m.lda, m.n_pad, 0, m.n_pad, true);
+        trsm_blocked(c, m.L.get(), m.lda, m.Winv.get(), m.n_pad, Kinv.get(), m.lda, m.n_pad, true, W.get(), m.lda, false, nullptr, 0);
         k::GemmArgs g{};
         g.A = W.get();
         g.lda = m.lda;

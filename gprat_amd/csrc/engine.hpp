@@ -23,9 +23,12 @@ class DevicePool
     void *get(std::size_t bytes);
     void put(void *p);
     void release_cached();
+    // Free cached blocks until at most `keep` bytes stay cached.
+    void trim(std::size_t keep);
     std::size_t cached_bytes() const { return cached_; }
 
   private:
+    std::mutex mu_;
     int device_;
     std::multimap<std::size_t, void *> free_;
     std::map<void *, std::size_t> live_;
@@ -78,8 +81,9 @@ struct Context
 
     int device;
     int n_streams;
-    hipStream_t main = nullptr;   // trailing updates, solves, predictions
-    hipStream_t panel = nullptr;  // Cholesky panel factorisation (look-ahead)
+    hipStream_t main = nullptr;   // solves, predictions, everything outside the Cholesky
+    hipStream_t panel = nullptr;  // Cholesky diagonal blocks (look-ahead); == main with one stream
+    hipStream_t trail = nullptr;  // Cholesky trailing updates and panel rows (== main)
     DevicePool pool;
     std::recursive_mutex mu;
 
@@ -136,6 +140,7 @@ struct FactorView
     const double *L;
     const double *Winv;
     const double *alpha;
+    bool mixed = false;  // variance solve in FP32 products (mixed-precision models)
 };
 
 // Many GPs fitted by one persistent launch; factors stay in one arena.
@@ -143,12 +148,10 @@ struct Batch
 {
     Context *ctx = nullptr;
     int count = 0;
-    std::vector<k::BatchGP> gps;
+    std::vector<k::BatchGP> gps;  // device addresses inside the arena
     DevBuf<double> arena;
-    DevBuf<double> desc;  // device copy of gps
-    DevBuf<int> order, info, counter;
-    DevBuf<double> losses;
-    std::size_t bytes() const { return arena.bytes() + desc.bytes() + losses.bytes(); }
+    std::vector<double> losses;
+    std::size_t bytes() const { return arena.bytes(); }
 };
 
 // Algorithms (engine.cpp)
@@ -160,6 +163,8 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
 void refine_alpha(Model &m, double *work);
 void solve_spd(Context &c, const double *L, long lda, const double *Winv, int n_pad, const double *y, double *work, double *out);
 void trsm_lower_left(Context &c, const double *L, long lda, const double *Winv, double *B, long ldb, int ncols, int a, int b, bool tri_rhs);
+void trsm_blocked(Context &c, const double *L, long lda, const double *Winv, int n_pad, double *B, long ldb, int ncols, bool tri_rhs,
+                  double *out, long ldo, bool staged, double *colsq, int mt);
 
 void factor(Context &c, Model &m, const double *x, std::size_t x_len, int n, int r, const double theta[3], int precision);
 void attach_targets(Model &m, const double *y, std::size_t y_len);
@@ -175,6 +180,7 @@ void fit(Context &c,
          int precision);
 void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean, double *var);
 void predict_view(const FactorView &m, const double *z, std::size_t z_len, int mt, double *mean, double *var);
+std::vector<int> batch_run(Context &c, const std::vector<k::BatchGP> &gps, double *losses_host);
 void batch_fit(Context &c, Batch &b, int count, const gprat_gp_desc *gps, int precision);
 void batch_losses(Batch &b, double *losses);
 void batch_predict(Batch &b, int index, const double *z, std::size_t z_len, int mt, double *mean, double *var);

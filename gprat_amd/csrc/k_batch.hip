@@ -1,11 +1,19 @@
 // Batched fit of many independent GPs of uneven size in one persistent
 // launch. Each workgroup repeatedly takes the next GP from a device counter
 // (GPs are ordered largest first) and, without leaving the workgroup,
-// assembles its covariance, factors it with 64-wide blocks (diagonal block
-// factor + inverse, then 64x64x64 FP64 MFMA block products for the panel
-// solve and the trailing update), solves alpha = K^-1 y and writes the loss.
-// Every workgroup exits as soon as the counter passes the batch size.
+// assembles its covariance, factors it, solves alpha = K^-1 y and writes
+// the loss. Every workgroup exits as soon as the counter passes the batch
+// size.
+//
+// The factorisation is left-looking over 64-wide block columns: block
+// (i, j) of column j accumulates sum_{k<j} L_ik L_jk^T on FP64 MFMA in
+// registers, with the next pair of blocks already on its way from memory
+// (register prefetch) while the current pair is multiplied; the sum is then
+// subtracted from K_ij once and the block is either factored (i == j:
+// Cholesky factor and inverse of the 64x64 block in LDS) or multiplied by
+// the diagonal block's inverse, which stays in LDS for the whole column.
 #include "block64.hpp"
+#include "block_mfma.hpp"
 #include "kernels.hpp"
 
 #include <climits>
@@ -16,11 +24,12 @@ namespace gpamd::k
 namespace
 {
 
-typedef double v4d __attribute__((ext_vector_type(4)));
+using blk::NB;
+using blk::THREADS;
+using blk::XS;
+using blk::v4d;
 
-constexpr int NB = 64;
-constexpr int THREADS = 256;
-constexpr int XS = NB + 2;  // LDS row stride of staged 64x64 blocks (bank-spread MFMA reads)
+typedef double d2 __attribute__((ext_vector_type(2)));
 
 struct Stage
 {
@@ -39,83 +48,73 @@ union Shared
     Factor fa;
 };
 
-// Copy a 64x64 row-major block (leading dimension ld) into LDS stride XS.
-__device__ __forceinline__ void stage_block(double *dst, const double *__restrict__ src, long ld)
+// One thread's share of a 64x64 row-major block: 8 pairs of doubles, row
+// e >> 5 and columns (e & 31) * 2 for e = tid + 256 s.
+struct BlockRegs
 {
-    for (int e = threadIdx.x; e < NB * NB / 2; e += THREADS)
-    {
-        const int r = e >> 5, c = (e & 31) * 2;
-        const double2 v = *reinterpret_cast<const double2 *>(src + static_cast<long>(r) * ld + c);
-        dst[r * XS + c] = v.x;
-        dst[r * XS + c + 1] = v.y;
-    }
-}
+    d2 v[8];
 
-// acc (this wavefront's 32x32 quarter of a 64x64 block) = sum_k A[r][k] B[c][k]
-// with A and B staged in LDS. Wavefront w owns rows 32*(w>>1).., columns 32*(w&1)..
-__device__ __forceinline__ void mfma_nt64(const double *A, const double *B, v4d (&acc)[2][2])
-{
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int lm = lane & 15, lk = lane >> 4;
-    const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-            acc[i][j] = v4d{ 0.0, 0.0, 0.0, 0.0 };
-#pragma unroll 4
-    for (int k0 = 0; k0 < NB; k0 += 4)
+    __device__ __forceinline__ void load(const double *__restrict__ src, long ld)
     {
-        double a[2], b[2];
 #pragma unroll
-        for (int i = 0; i < 2; i++)
-            a[i] = A[(r0 + i * 16 + lm) * XS + k0 + lk];
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-            b[j] = B[(c0 + j * 16 + lm) * XS + k0 + lk];
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-#pragma unroll
-            for (int j = 0; j < 2; j++)
-                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int s = 0; s < 8; s++)
+        {
+            const int e = threadIdx.x + THREADS * s;
+            v[s] = *reinterpret_cast<const d2 *>(src + static_cast<long>(e >> 5) * ld + (e & 31) * 2);
+        }
     }
-}
 
-// C (global, 64x64 block, ld) = acc  or  C -= acc, optionally only on or
-// below the block's diagonal. f64 16x16x4 accumulator layout: lane holds
-// column lane%16, rows lane/16 + 4e.
-__device__ __forceinline__ void store_acc(double *C, long ld, const v4d (&acc)[2][2], bool set, bool lower)
+    __device__ __forceinline__ void store(double *dst) const
+    {
+#pragma unroll
+        for (int s = 0; s < 8; s++)
+        {
+            const int e = threadIdx.x + THREADS * s;
+            *reinterpret_cast<d2 *>(dst + (e >> 5) * XS + (e & 31) * 2) = v[s];
+        }
+    }
+};
+
+// acc = sum_{k < kn} L[i, k] L[j, k]^T over 64-blocks of block rows i and j
+// (block (r, k) at L + r*64*np + k*64), streamed through the LDS stage with
+// the next pair prefetched into registers.
+__device__ __forceinline__ void accumulate_row_products(const double *L, long np, int ib, int jb, int kn, Stage &st, v4d (&acc)[2][2])
 {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int lm = lane & 15, lk = lane >> 4;
-    const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-#pragma unroll
-            for (int e = 0; e < 4; e++)
-            {
-                const int r = r0 + i * 16 + lk + 4 * e, c = c0 + j * 16 + lm;
-                if (lower && c > r)
-                    continue;
-                double *dst = C + static_cast<long>(r) * ld + c;
-                if (set)
-                    *dst = acc[i][j][e];
-                else
-                    *dst -= acc[i][j][e];
-            }
+    blk::zero(acc);
+    if (kn == 0)
+        return;
+    const double *ai = L + static_cast<long>(ib) * NB * np;
+    const double *bj = L + static_cast<long>(jb) * NB * np;
+    BlockRegs ra, rb;
+    ra.load(ai, np);
+    rb.load(bj, np);
+    for (int kb = 0; kb < kn; kb++)
+    {
+        __syncthreads();  // every wavefront is done with the previous pair
+        ra.store(st.a);
+        rb.store(st.b);
+        __syncthreads();
+        if (kb + 1 < kn)
+        {
+            ra.load(ai + (kb + 1) * NB, np);
+            rb.load(bj + (kb + 1) * NB, np);
+        }
+        // keep the prefetch ahead of the products
+        __builtin_amdgcn_sched_barrier(0);
+        blk::mfma_nt_acc(st.a, st.b, acc);
+    }
+    __syncthreads();
 }
 
 __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict__ gps,
                                                         const int *__restrict__ order,
                                                         int count,
                                                         int *__restrict__ counter,
-                                                        double *__restrict__ arena,
                                                         double *__restrict__ losses,
                                                         int *__restrict__ info)
 {
     __shared__ Shared sh;
+    __shared__ double wjj[NB * XS];  // inverse of the current diagonal block, rows = B^T rows
     __shared__ int next;
     __shared__ double red[2][THREADS];
     const int tid = threadIdx.x;
@@ -130,11 +129,11 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
             break;
         const int gi = order[slot];
         const BatchGP g = gps[gi];
-        const double *x = arena + g.x_off;
-        const double *y = arena + g.y_off;
-        double *L = arena + g.l_off;
-        double *Wg = arena + g.w_off;
-        double *alpha = arena + g.a_off;
+        const double *x = g.x;
+        const double *y = g.y;
+        double *L = g.L;
+        double *Wg = g.W;
+        double *alpha = g.alpha;
         const long np = g.n_pad;
         const int nblk = static_cast<int>(np / NB);
         const double scale = -0.5 / (g.lengthscale * g.lengthscale);
@@ -145,65 +144,57 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
         for (long i = wv; i < np; i += THREADS / 64)
             for (long j = lane; j < ((i >> 6) + 1) * 64; j += 64)
             {
-            double val;
-            if (i < g.n && j < g.n)
-            {
-                val = g.vertical * exp(scale * window_sqdist(x, i, x, j, g.n_reg));
-                if (i == j)
-                    val += g.noise;
-            }
-            else
-            {
-                val = (i == j) ? 1.0 : 0.0;
-            }
-            L[i * np + j] = val;
+                double val;
+                if (i < g.n && j < g.n)
+                {
+                    val = g.vertical * exp(scale * window_sqdist(x, i, x, j, g.n_reg));
+                    if (i == j)
+                        val += g.noise;
+                }
+                else
+                {
+                    val = (i == j) ? 1.0 : 0.0;
+                }
+                L[i * np + j] = val;
             }
         __syncthreads();
 
-        // 2. blocked right-looking Cholesky
-        for (int kb = 0; kb < nblk; kb++)
+        // 2. left-looking blocked Cholesky
+        for (int jb = 0; jb < nblk; jb++)
         {
-            double *Lkk = L + static_cast<long>(kb) * NB * np + kb * NB;
-            for (int e = tid; e < NB * NB; e += THREADS)
+            for (int ib = jb; ib < nblk; ib++)
             {
-                const int r = e / NB, c = e % NB;
-                sh.fa.D[r][c] = (c <= r) ? Lkk[static_cast<long>(r) * np + c] : 0.0;
-            }
-            __syncthreads();
-            factor_block64(sh.fa.D, sh.fa.W, sh.fa.S, info + gi, kb * NB);
-            double *Wkk = Wg + static_cast<long>(kb) * NB * NB;
-            for (int e = tid; e < NB * NB; e += THREADS)
-            {
-                const int r = e / NB, c = e % NB;
-                if (c <= r)
-                    Lkk[static_cast<long>(r) * np + c] = sh.fa.D[r][c];
-                Wkk[e] = (c <= r) ? sh.fa.W[r][c] : 0.0;
-            }
-            __syncthreads();
-            // panel solve: X_ib = A_ib W_kk^T (in place)
-            for (int ib = kb + 1; ib < nblk; ib++)
-            {
-                double *Aik = L + static_cast<long>(ib) * NB * np + kb * NB;
-                stage_block(sh.st.a, Aik, np);
-                stage_block(sh.st.b, Wkk, NB);
-                __syncthreads();
                 v4d acc[2][2];
-                mfma_nt64(sh.st.a, sh.st.b, acc);
-                __syncthreads();
-                store_acc(Aik, np, acc, true, false);
-                __syncthreads();
-            }
-            // trailing update: C_ij -= X_i X_j^T for kb < j <= i
-            for (int ib = kb + 1; ib < nblk; ib++)
-            {
-                stage_block(sh.st.a, L + static_cast<long>(ib) * NB * np + kb * NB, np);
-                for (int jb = kb + 1; jb <= ib; jb++)
+                accumulate_row_products(L, np, ib, jb, jb, sh.st, acc);
+                double *Cij = L + static_cast<long>(ib) * NB * np + jb * NB;
+                if (ib == jb)
                 {
-                    stage_block(sh.st.b, L + static_cast<long>(jb) * NB * np + kb * NB, np);
+                    // D = K_jj - acc (lower), factor + inverse in LDS
+                    blk::for_each(acc,
+                                  [&](int r, int c, double v)
+                                  { sh.fa.D[r][c] = (c <= r) ? Cij[static_cast<long>(r) * np + c] - v : 0.0; });
                     __syncthreads();
-                    v4d acc[2][2];
-                    mfma_nt64(sh.st.a, sh.st.b, acc);
-                    store_acc(L + static_cast<long>(ib) * NB * np + jb * NB, np, acc, false, jb == ib);
+                    factor_block64(sh.fa.D, sh.fa.W, sh.fa.S, info + gi, jb * NB);
+                    double *Wjj = Wg + static_cast<long>(jb) * NB * NB;
+                    for (int e = tid; e < NB * NB; e += THREADS)
+                    {
+                        const int r = e / NB, c = e % NB;
+                        const double w = (c <= r) ? sh.fa.W[r][c] : 0.0;
+                        if (c <= r)
+                            Cij[static_cast<long>(r) * np + c] = sh.fa.D[r][c];
+                        Wjj[e] = w;
+                        wjj[r * XS + c] = w;
+                    }
+                    __syncthreads();
+                }
+                else
+                {
+                    // L_ij = (K_ij - acc) W_jj^T
+                    blk::for_each(acc, [&](int r, int c, double v) { sh.st.a[r * XS + c] = Cij[static_cast<long>(r) * np + c] - v; });
+                    __syncthreads();
+                    v4d xb[2][2];
+                    blk::mfma_nt(sh.st.a, wjj, xb);
+                    blk::store_acc(Cij, np, xb, true, false);
                     __syncthreads();
                 }
             }
@@ -211,29 +202,36 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
 
         // 3. alpha: forward then backward block substitution with W
         double *xs = red[0];
+        double *part = red[1];
         for (long i = tid; i < np; i += THREADS)
             alpha[i] = (i < g.n) ? y[i] : 0.0;
         __syncthreads();
+        const int row = tid & 63, quarter = tid >> 6;
         for (int J = 0; J < nblk; J++)
         {
             const double *W = Wg + static_cast<long>(J) * NB * NB;
             const long base = static_cast<long>(J) * NB;
-            if (tid < NB)
             {
+                // x_J = W_JJ alpha_J: four partial sums per row, fixed order
                 double s = 0.0;
-                for (int k = 0; k < NB; k++)
-                    s += W[tid * NB + k] * alpha[base + k];
-                xs[tid] = s;
+#pragma unroll 4
+                for (int k = 0; k < 16; k++)
+                    s += W[row * NB + quarter * 16 + k] * alpha[base + quarter * 16 + k];
+                part[quarter * 64 + row] = s;
             }
             __syncthreads();
             if (tid < NB)
+            {
+                xs[tid] = (part[tid] + part[64 + tid]) + (part[128 + tid] + part[192 + tid]);
                 alpha[base + tid] = xs[tid];
+            }
+            __syncthreads();
             for (long i = base + NB + tid; i < np; i += THREADS)
             {
-                const double *row = L + i * np + base;
+                const double *lr = L + i * np + base;
                 double s = 0.0;
                 for (int k = 0; k < NB; k++)
-                    s += row[k] * xs[k];
+                    s += lr[k] * xs[k];
                 alpha[i] -= s;
             }
             __syncthreads();
@@ -242,16 +240,21 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
         {
             const double *W = Wg + static_cast<long>(J) * NB * NB;
             const long base = static_cast<long>(J) * NB;
-            if (tid < NB)
             {
+                // x_J = W_JJ^T alpha_J
                 double s = 0.0;
-                for (int k = 0; k < NB; k++)
-                    s += W[k * NB + tid] * alpha[base + k];
-                xs[tid] = s;
+#pragma unroll 4
+                for (int k = 0; k < 16; k++)
+                    s += W[(quarter * 16 + k) * NB + row] * alpha[base + quarter * 16 + k];
+                part[quarter * 64 + row] = s;
             }
             __syncthreads();
             if (tid < NB)
+            {
+                xs[tid] = (part[tid] + part[64 + tid]) + (part[128 + tid] + part[192 + tid]);
                 alpha[base + tid] = xs[tid];
+            }
+            __syncthreads();
             for (long jcol = tid; jcol < base; jcol += THREADS)
             {
                 double s = 0.0;
@@ -302,15 +305,15 @@ __global__ void k_set_ints(int *p, int count, int value)
 
 }  // namespace
 
-void batch_fit(hipStream_t s, const BatchGP *gps, const int *order, int count, int *counter, double *arena, double *losses,
-               int *info, int workgroups)
+void batch_fit(hipStream_t s, const BatchGP *gps, const int *order, int count, int *counter, double *losses, int *info,
+               int workgroups)
 {
     require(count >= 1 && workgroups >= 1, "batch_fit: bad shape");
     hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(1), 0, s, counter, 1, 0);
     GP_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_set_ints, dim3(ceil_div(count, 256)), dim3(256), 0, s, info, count, INT_MAX);
     GP_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_batch_fit, dim3(workgroups), dim3(THREADS), 0, s, gps, order, count, counter, arena, losses, info);
+    hipLaunchKernelGGL(k_batch_fit, dim3(workgroups), dim3(THREADS), 0, s, gps, order, count, counter, losses, info);
     GP_CHECK_LAUNCH();
 }
 

@@ -48,12 +48,13 @@ __global__ __launch_bounds__(THREADS) void k_assemble_lower(const double *__rest
     }
 }
 
+template <class T>
 __global__ __launch_bounds__(THREADS) void k_assemble_cross_t(const double *__restrict__ x,
                                                                const double *__restrict__ z,
                                                                int r,
                                                                double scale,
                                                                double v,
-                                                               double *__restrict__ B,
+                                                               T *__restrict__ B,
                                                                long ldb,
                                                                int n,
                                                                int m)
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(THREADS) void k_assemble_cross_t(const double *__re
         double val = 0.0;
         if (i < n && j < m)
             val = se(scale, v, window_sqdist(z, j, x, i, r));  // (test - train), as gp_algorithms.cpp:113-140
-        B[i * ldb + j] = val;
+        B[i * ldb + j] = static_cast<T>(val);
     }
 }
 
@@ -276,12 +277,23 @@ void assemble_lower(hipStream_t s, const double *x, int r, SEK sek, double *A, l
     GP_CHECK_LAUNCH();
 }
 
-void assemble_cross_t(hipStream_t s, const double *x, const double *z, int r, SEK sek, double *B, long ldb, int n, int n_pad, int m, int m_pad)
+template <class T>
+static void assemble_cross_t_impl(hipStream_t s, const double *x, const double *z, int r, SEK sek, T *B, long ldb, int n, int n_pad, int m, int m_pad)
 {
     require(n_pad % TB == 0 && m_pad % TB == 0 && n <= n_pad && m <= m_pad && ldb >= m_pad, "assemble_cross_t: bad shape");
     dim3 grid(m_pad / TB, n_pad / TB);
-    hipLaunchKernelGGL(k_assemble_cross_t, grid, dim3(THREADS), 0, s, x, z, r, sek.scale(), sek.vertical, B, ldb, n, m);
+    hipLaunchKernelGGL(k_assemble_cross_t<T>, grid, dim3(THREADS), 0, s, x, z, r, sek.scale(), sek.vertical, B, ldb, n, m);
     GP_CHECK_LAUNCH();
+}
+
+void assemble_cross_t(hipStream_t s, const double *x, const double *z, int r, SEK sek, double *B, long ldb, int n, int n_pad, int m, int m_pad)
+{
+    assemble_cross_t_impl<double>(s, x, z, r, sek, B, ldb, n, n_pad, m, m_pad);
+}
+
+void assemble_cross_t_f32(hipStream_t s, const double *x, const double *z, int r, SEK sek, float *B, long ldb, int n, int n_pad, int m, int m_pad)
+{
+    assemble_cross_t_impl<float>(s, x, z, r, sek, B, ldb, n, n_pad, m, m_pad);
 }
 
 int mean_splits(int n, int m)

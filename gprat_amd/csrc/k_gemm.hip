@@ -21,6 +21,9 @@ namespace
 {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+// staging registers of the global->LDS copy: a native vector type, so they
+// stay in registers (the HIP double2 struct ends up in scratch here)
+typedef double d2 __attribute__((ext_vector_type(2)));
 
 constexpr int BM = 128;
 constexpr int BN = 128;
@@ -79,14 +82,14 @@ struct Slab
         }
     }
 
-    __device__ __forceinline__ void load(const double *__restrict__ origin, double2 (&reg)[4]) const
+    __device__ __forceinline__ void load(const double *__restrict__ origin, d2 (&reg)[4]) const
     {
 #pragma unroll
         for (int s = 0; s < 4; s++)
-            reg[s] = *reinterpret_cast<const double2 *>(origin + off[s]);
+            reg[s] = *reinterpret_cast<const d2 *>(origin + off[s]);
     }
 
-    __device__ __forceinline__ void store(double *S, const double2 (&reg)[4]) const
+    __device__ __forceinline__ void store(double *S, const d2 (&reg)[4]) const
     {
 #pragma unroll
         for (int s = 0; s < 4; s++)
@@ -98,7 +101,7 @@ struct Slab
             }
             else
             {
-                *reinterpret_cast<double2 *>(S + lds[s]) = reg[s];
+                *reinterpret_cast<d2 *>(S + lds[s]) = reg[s];
             }
         }
     }
@@ -171,7 +174,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
         const double *ak = a0 + (AT ? kbeg * p.lda : kbeg);
         const double *bk = b0 + (BT ? kbeg : kbeg * p.ldb);
 
-        double2 ra[4], rb[4];
+        d2 ra[4], rb[4];
         sa.load(ak, ra);
         sb.load(bk, rb);
         sa.store(As[0], ra);
@@ -186,7 +189,12 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
             bk += b_step;
             sa.load(ak, ra);
             sb.load(bk, rb);
+            // keep the next slab's loads ahead of this slab's products: left
+            // to itself the scheduler sinks them to the LDS stores and every
+            // slab then waits out a full memory latency
+            __builtin_amdgcn_sched_barrier(0);
             mfma_slab(As[buf], Bs[buf], wr, wc, lm, lk, acc);
+            __builtin_amdgcn_sched_barrier(0);
             sa.store(As[buf ^ 1], ra);
             sb.store(Bs[buf ^ 1], rb);
             __syncthreads();
@@ -236,15 +244,24 @@ void gemm_f64(hipStream_t s, const GemmArgs &g)
               g.kstart_off, g.set ? 1 : 0 };
     dim3 grid(static_cast<unsigned>(ceil_div(g.N, BN)), static_cast<unsigned>(ceil_div(g.M, BM)));
     require(grid.y <= 65535, "gemm_f64: M too large for one launch");
-    // Instantiated combinations: (N,T) trailing/panel, (N,N) solve, (T,N) inverse.
+    // Instantiated combinations: (N,T) trailing/panel, (N,N) solve, (T,N) inverse
+    // and the updates fed by k-major panel copies.
     if (!g.a_trans && g.b_trans && g.role == GemmRole::Trailing)
         hipLaunchKernelGGL((k_gemm_f64<false, true, 0>), grid, dim3(THREADS), 0, s, p);
     else if (!g.a_trans && g.b_trans && g.role == GemmRole::PanelSolve)
         hipLaunchKernelGGL((k_gemm_f64<false, true, 4>), grid, dim3(THREADS), 0, s, p);
     else if (!g.a_trans && g.b_trans)
         hipLaunchKernelGGL((k_gemm_f64<false, true, 1>), grid, dim3(THREADS), 0, s, p);
+    else if (!g.a_trans && !g.b_trans && g.role == GemmRole::PanelX)
+        hipLaunchKernelGGL((k_gemm_f64<false, false, 8>), grid, dim3(THREADS), 0, s, p);
     else if (!g.a_trans && !g.b_trans)
         hipLaunchKernelGGL((k_gemm_f64<false, false, 2>), grid, dim3(THREADS), 0, s, p);
+    else if (g.a_trans && !g.b_trans && g.role == GemmRole::TrailingT)
+        hipLaunchKernelGGL((k_gemm_f64<true, false, 5>), grid, dim3(THREADS), 0, s, p);
+    else if (g.a_trans && !g.b_trans && g.role == GemmRole::SolveT)
+        hipLaunchKernelGGL((k_gemm_f64<true, false, 6>), grid, dim3(THREADS), 0, s, p);
+    else if (g.a_trans && !g.b_trans && g.role == GemmRole::LeafT)
+        hipLaunchKernelGGL((k_gemm_f64<true, false, 7>), grid, dim3(THREADS), 0, s, p);
     else if (g.a_trans && !g.b_trans)
         hipLaunchKernelGGL((k_gemm_f64<true, false, 3>), grid, dim3(THREADS), 0, s, p);
     else

@@ -1,4 +1,5 @@
-// FP32 matrix-core trailing update for the mixed-precision factorisation.
+// FP32 matrix-core products for the mixed-precision path: the trailing
+// update of the factorisation and the variance solve.
 //
 // The factored FP64 panel is first copied to FP32 in k-major order
 // (P[k][row], `panel_to_f32t`), so both operands of C -= A B^T are read as
@@ -15,27 +16,36 @@ namespace
 {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+// staging registers of the global->LDS copy: a native vector type, so they
+// stay in registers (the HIP float4 struct ends up in scratch here)
+typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int BM = 128;
 constexpr int BK = 32;
 constexpr int KS = BM + 16;  // floats per LDS k-row: the 4 k-slots of a read hit disjoint banks
 constexpr int THREADS = 256;
 
+// C (op)= A B with A(i, k) = A[k * lda + i] and B(k, j) = B[k * ldb + j]
+// (both operands k-major, as the panel copies are), FP32 products and sums.
+template <class TC>
 struct Params32
 {
-    const float *P;  // k-major panel copy
-    long ldp;
-    long a_off, b_off;  // row offsets of the A and B operands inside P
-    double *C;
+    const float *A;
+    long lda;
+    const float *B;
+    long ldb;
+    TC *C;
     long ldc;
     int M, N, K;
+    int lower;  // only i + diag_off >= j
     long diag_off;
+    int set;  // C = acc instead of C -= acc
 };
 
-// Element offsets (within one slab) of this thread's four float4 loads,
-// clamped to the operand's last row pair; rows past the edge only feed
+// Element offsets (within one slab) of this thread's four 16-byte loads,
+// clamped to the operand's last group of four; rows past the edge only feed
 // accumulators the epilogue never stores.
-__device__ __forceinline__ void slab_offsets(long ldp, int rows, int tid, unsigned (&off)[4], int (&lds)[4])
+__device__ __forceinline__ void slab_offsets(long ld, int rows, int tid, unsigned (&off)[4], int (&lds)[4])
 {
 #pragma unroll
     for (int s = 0; s < 4; s++)
@@ -43,7 +53,7 @@ __device__ __forceinline__ void slab_offsets(long ldp, int rows, int tid, unsign
         const int idx = tid + THREADS * s;
         const int kk = idx >> 5;         // 0..31
         const int col = (idx & 31) * 4;  // 0..124
-        off[s] = static_cast<unsigned>(kk * ldp + min(col, rows - 4));
+        off[s] = static_cast<unsigned>(kk * ld + min(col, rows - 4));
         lds[s] = kk * KS + col;
     }
 }
@@ -68,14 +78,17 @@ __device__ __forceinline__ void mfma_slab32(const float *As, const float *Bs, in
     }
 }
 
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_f32_trailing(Params32 p)
+// ROLE only separates profiler names: 0 trailing update (FP64 C), 1 TRSM
+// diagonal step, 2 TRSM update.
+template <class TC, int ROLE>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_f32_tn(Params32<TC> p)
 {
     __shared__ float As[2][BK * KS];
     __shared__ float Bs[2][BK * KS];
 
     const long r0 = static_cast<long>(blockIdx.y) * BM;
     const long c0 = static_cast<long>(blockIdx.x) * BM;
-    if (r0 + (BM - 1) + p.diag_off < c0)
+    if (p.lower && r0 + (BM - 1) + p.diag_off < c0)
         return;  // tile entirely above the diagonal
     const int rows_a = static_cast<int>(min(static_cast<long>(BM), p.M - r0));
     const int rows_b = static_cast<int>(min(static_cast<long>(BM), p.N - c0));
@@ -94,24 +107,24 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2))) vo
 
     unsigned oa[4], ob[4];
     int la[4], lb[4];
-    slab_offsets(p.ldp, rows_a, tid, oa, la);
-    slab_offsets(p.ldp, rows_b, tid, ob, lb);
-    const float *ak = p.P + p.a_off + r0;
-    const float *bk = p.P + p.b_off + c0;
-    const long step = BK * p.ldp;
+    slab_offsets(p.lda, rows_a, tid, oa, la);
+    slab_offsets(p.ldb, rows_b, tid, ob, lb);
+    const float *ak = p.A + r0;
+    const float *bk = p.B + c0;
+    const long astep = BK * p.lda, bstep = BK * p.ldb;
 
-    float4 ra[4], rb[4];
+    f4 ra[4], rb[4];
 #pragma unroll
     for (int s = 0; s < 4; s++)
     {
-        ra[s] = *reinterpret_cast<const float4 *>(ak + oa[s]);
-        rb[s] = *reinterpret_cast<const float4 *>(bk + ob[s]);
+        ra[s] = *reinterpret_cast<const f4 *>(ak + oa[s]);
+        rb[s] = *reinterpret_cast<const f4 *>(bk + ob[s]);
     }
 #pragma unroll
     for (int s = 0; s < 4; s++)
     {
-        *reinterpret_cast<float4 *>(&As[0][la[s]]) = ra[s];
-        *reinterpret_cast<float4 *>(&Bs[0][lb[s]]) = rb[s];
+        *reinterpret_cast<f4 *>(&As[0][la[s]]) = ra[s];
+        *reinterpret_cast<f4 *>(&Bs[0][lb[s]]) = rb[s];
     }
     __syncthreads();
 
@@ -119,20 +132,23 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2))) vo
     const int steps = p.K / BK;
     for (int t = 1; t < steps; t++)
     {
-        ak += step;
-        bk += step;
+        ak += astep;
+        bk += bstep;
 #pragma unroll
         for (int s = 0; s < 4; s++)
         {
-            ra[s] = *reinterpret_cast<const float4 *>(ak + oa[s]);
-            rb[s] = *reinterpret_cast<const float4 *>(bk + ob[s]);
+            ra[s] = *reinterpret_cast<const f4 *>(ak + oa[s]);
+            rb[s] = *reinterpret_cast<const f4 *>(bk + ob[s]);
         }
+        // the next slab's loads stay ahead of this slab's products
+        __builtin_amdgcn_sched_barrier(0);
         mfma_slab32(As[buf], Bs[buf], wr, wc, lm, lk, acc);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int s = 0; s < 4; s++)
         {
-            *reinterpret_cast<float4 *>(&As[buf ^ 1][la[s]]) = ra[s];
-            *reinterpret_cast<float4 *>(&Bs[buf ^ 1][lb[s]]) = rb[s];
+            *reinterpret_cast<f4 *>(&As[buf ^ 1][la[s]]) = ra[s];
+            *reinterpret_cast<f4 *>(&Bs[buf ^ 1][lb[s]]) = rb[s];
         }
         __syncthreads();
         buf ^= 1;
@@ -153,24 +169,29 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2))) vo
                 if (rr >= rows_a || cc >= rows_b)
                     continue;
                 const long gi = r0 + rr, gj = c0 + cc;
-                if (gi + p.diag_off < gj)
+                if (p.lower && gi + p.diag_off < gj)
                     continue;
-                p.C[gi * p.ldc + gj] -= static_cast<double>(acc[i][j][e]);
+                TC *dst = p.C + gi * p.ldc + gj;
+                if (p.set)
+                    *dst = static_cast<TC>(acc[i][j][e]);
+                else
+                    *dst -= static_cast<TC>(acc[i][j][e]);
             }
 }
 
-// P[k][r] = (float) A[(r0 + r) * lda + p0 + k] for r < R, k < W, through a
-// 64x64 LDS tile so both the FP64 reads and the FP32 writes are coalesced.
-__global__ __launch_bounds__(THREADS) void k_panel_to_f32t(const double *__restrict__ A, long lda, long r0, long p0, int R, int W,
-                                                            float *__restrict__ P, long ldp)
+// P[k][r] = (T) A[(r0 + r) * lda + p0 + k] for r < R, k < W, through a
+// 64x64 LDS tile so both the FP64 reads and the k-major writes are coalesced.
+template <class T>
+__global__ __launch_bounds__(THREADS) void k_panel_to_t(const double *__restrict__ A, long lda, long r0, long p0, int R, int W,
+                                                         T *__restrict__ P, long ldp)
 {
-    __shared__ float t[64][65];
+    __shared__ T t[64][65];
     const long rb = static_cast<long>(blockIdx.x) * 64, kb = static_cast<long>(blockIdx.y) * 64;
     const int c = threadIdx.x & 63, q = threadIdx.x >> 6;
     for (int rr = q; rr < 64; rr += 4)
     {
         const long r = rb + rr, k = kb + c;
-        t[rr][c] = (r < R && k < W) ? static_cast<float>(A[(r0 + r) * lda + p0 + k]) : 0.0f;
+        t[rr][c] = (r < R && k < W) ? static_cast<T>(A[(r0 + r) * lda + p0 + k]) : T(0);
     }
     __syncthreads();
     for (int kk = q; kk < 64; kk += 4)
@@ -183,29 +204,57 @@ __global__ __launch_bounds__(THREADS) void k_panel_to_f32t(const double *__restr
 
 }  // namespace
 
-void panel_to_f32t(hipStream_t s, const double *A, long lda, long r0, long p0, int R, int W, float *P, long ldp)
+template <class T>
+static void panel_to_t(hipStream_t s, const double *A, long lda, long r0, long p0, int R, int W, T *P, long ldp)
 {
-    require(R >= 0 && W >= 0 && ldp >= R, "panel_to_f32t: bad shape");
+    require(R >= 0 && W >= 0 && ldp >= R, "panel copy: bad shape");
     if (R == 0 || W == 0)
         return;
     dim3 grid(static_cast<unsigned>(ceil_div(R, 64)), static_cast<unsigned>(ceil_div(W, 64)));
-    hipLaunchKernelGGL(k_panel_to_f32t, grid, dim3(THREADS), 0, s, A, lda, r0, p0, R, W, P, ldp);
+    require(grid.y <= 65535, "panel copy: panel too wide");
+    hipLaunchKernelGGL(k_panel_to_t<T>, grid, dim3(THREADS), 0, s, A, lda, r0, p0, R, W, P, ldp);
+    GP_CHECK_LAUNCH();
+}
+
+void panel_to_f32t(hipStream_t s, const double *A, long lda, long r0, long p0, int R, int W, float *P, long ldp)
+{
+    panel_to_t<float>(s, A, lda, r0, p0, R, W, P, ldp);
+}
+
+void panel_to_f64t(hipStream_t s, const double *A, long lda, long r0, long p0, int R, int W, double *P, long ldp)
+{
+    panel_to_t<double>(s, A, lda, r0, p0, R, W, P, ldp);
+}
+
+template <class TC, int ROLE>
+static void launch_f32_tn(hipStream_t s, const Params32<TC> &p)
+{
+    if (p.M <= 0 || p.N <= 0)
+        return;
+    require(p.K % BK == 0, "FP32 product: K must be a multiple of 32");
+    require(p.M % 4 == 0 && p.N % 4 == 0 && p.lda % 4 == 0 && p.ldb % 4 == 0
+                && reinterpret_cast<uintptr_t>(p.A) % 16 == 0 && reinterpret_cast<uintptr_t>(p.B) % 16 == 0,
+            "FP32 product: operand rows must come in aligned groups of four");
+    require(static_cast<long>(BK) * std::max(p.lda, p.ldb) + BM < (1L << 31), "FP32 product: operand too tall");
+    dim3 grid(static_cast<unsigned>(ceil_div(p.N, BM)), static_cast<unsigned>(ceil_div(p.M, BM)));
+    require(grid.y <= 65535, "FP32 product: M too large for one launch");
+    hipLaunchKernelGGL((k_gemm_f32_tn<TC, ROLE>), grid, dim3(THREADS), 0, s, p);
     GP_CHECK_LAUNCH();
 }
 
 void gemm_f32_trailing(hipStream_t s, const float *P, long ldp, long a_off, long b_off, double *C, long ldc, int M, int N, int K, long diag_off)
 {
-    if (M <= 0 || N <= 0)
-        return;
-    require(K % BK == 0, "gemm_f32_trailing: K must be a multiple of 32");
-    require(M % 4 == 0 && N % 4 == 0 && ldp % 4 == 0 && a_off % 4 == 0 && b_off % 4 == 0,
-            "gemm_f32_trailing: rows must come in float4 groups");
-    require(static_cast<long>(BK) * ldp + BM < (1L << 31), "gemm_f32_trailing: panel too tall");
-    Params32 p{ P, ldp, a_off, b_off, C, ldc, M, N, K, diag_off };
-    dim3 grid(static_cast<unsigned>(ceil_div(N, BM)), static_cast<unsigned>(ceil_div(M, BM)));
-    require(grid.y <= 65535, "gemm_f32_trailing: M too large for one launch");
-    hipLaunchKernelGGL(k_gemm_f32_trailing, grid, dim3(THREADS), 0, s, p);
-    GP_CHECK_LAUNCH();
+    Params32<double> p{ P + a_off, ldp, P + b_off, ldp, C, ldc, M, N, K, 1, diag_off, 0 };
+    launch_f32_tn<double, 0>(s, p);
+}
+
+void gemm_f32_tn(hipStream_t s, const float *A, long lda, const float *B, long ldb, float *C, long ldc, int M, int N, int K, bool set)
+{
+    Params32<float> p{ A, lda, B, ldb, C, ldc, M, N, K, 0, 0, set ? 1 : 0 };
+    if (set)
+        launch_f32_tn<float, 1>(s, p);
+    else
+        launch_f32_tn<float, 2>(s, p);
 }
 
 }  // namespace gpamd::k

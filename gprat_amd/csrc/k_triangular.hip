@@ -3,6 +3,7 @@
 // loss reduction. All reductions run in a fixed order, so results do not
 // depend on scheduling.
 #include "block64.hpp"
+#include "block_mfma.hpp"
 #include "kernels.hpp"
 
 namespace gpamd::k
@@ -250,7 +251,130 @@ __global__ void k_fill(double *__restrict__ p, long count, double value)
 
 __global__ void k_set_int(int *p, int value) { *p = value; }
 
+// One workgroup per 256-wide diagonal block: the 64-wide diagonal
+// inverses are placed first, then the off-diagonal 64-blocks by increasing
+// distance d = i - j, each from blocks of smaller distance:
+//   S = sum_{t=j}^{i-1} L_it W_tj,   W_ij = -W_ii S.
+// Block b covers rows row0 + 256 b .. (at most 256, up to row0 + rows) and is
+// written k-major (W(r, c) at out[c * ldw + r], out = Wt + b * block_stride),
+// zero above its diagonal and past its last row; a 64-block of W read as
+// B^T rows is then a plain strided 64x64 read.
+__global__ __launch_bounds__(THREADS) void k_invert_blocks256(const double *__restrict__ L, long lda, const double *__restrict__ Winv,
+                                                               long row0, long rows, double *Wt, long ldw, long block_stride)
+{
+    __shared__ double a[blk::NB * blk::XS];
+    __shared__ double b[blk::NB * blk::XS];
+    const int tid = threadIdx.x;
+    const long j0 = row0 + static_cast<long>(blockIdx.x) * 256;
+    const int q = static_cast<int>(min(4L, (row0 + rows - j0) / NB));
+    const long jb = j0 / NB;  // first 64-block of this 256-block
+    double *out = Wt + static_cast<long>(blockIdx.x) * block_stride;
+    for (int e = tid; e < 256 * 256; e += THREADS)
+    {
+        const int c = e >> 8, r = e & 255;
+        const int bi = r >> 6, bj = c >> 6;
+        double v = 0.0;
+        if (bi == bj && bi < q)
+            v = Winv[(jb + bi) * NB * NB + (r & 63) * NB + (c & 63)];
+        out[c * ldw + r] = v;
+    }
+    __syncthreads();
+    for (int d = 1; d < q; d++)
+        for (int j = 0; j + d < q; j++)
+        {
+            const int i = j + d;
+            blk::v4d acc[2][2];
+            blk::zero(acc);
+            for (int t = j; t < i; t++)
+            {
+                blk::stage_block(a, L + (j0 + NB * i) * lda + j0 + NB * t, lda);
+                // rows c of the staged block: W(64t + k, 64j + c) over k
+                blk::stage_block(b, out + static_cast<long>(NB * j) * ldw + NB * t, ldw);
+                __syncthreads();
+                blk::mfma_nt_acc(a, b, acc);
+                __syncthreads();
+            }
+            blk::stage_block(a, Winv + (jb + i) * NB * NB, NB);
+            blk::for_each(acc, [&](int r, int c, double v) { b[c * blk::XS + r] = v; });
+            __syncthreads();
+            blk::v4d w[2][2];
+            blk::mfma_nt(a, b, w);
+            blk::for_each(w, [&](int r, int c, double v) { out[static_cast<long>(NB * j + c) * ldw + NB * i + r] = -v; });
+            __syncthreads();
+        }
+}
+
+// colsq[j] += sum_{i<rows} V[i][j]^2: 64 columns per workgroup, four row
+// groups summed separately and combined in a fixed order.
+template <class T>
+__global__ __launch_bounds__(THREADS) void k_colsq_accumulate(const T *__restrict__ V, long ldv, int rows, int m, double *__restrict__ colsq)
+{
+    __shared__ double part[4][64];
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const long j = static_cast<long>(blockIdx.x) * 64 + c;
+    double s = 0.0;
+    if (j < m)
+        for (int i = g; i < rows; i += 4)
+        {
+            const double v = static_cast<double>(V[i * ldv + j]);
+            s += v * v;
+        }
+    part[g][c] = s;
+    __syncthreads();
+    if (g == 0 && j < m)
+        colsq[j] += (part[0][c] + part[1][c]) + (part[2][c] + part[3][c]);
+}
+
+__global__ void k_to_f32(const double *__restrict__ src, float *__restrict__ dst, long count)
+{
+    const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < count)
+        dst[i] = static_cast<float>(src[i]);
+}
+
 }  // namespace
+
+void invert_blocks256(hipStream_t s, const double *L, long lda, const double *Winv, long row0, long rows, double *Wt, long ldw,
+                      long block_stride)
+{
+    require(row0 % NB == 0 && rows % NB == 0 && rows > 0 && lda >= row0 + rows && lda % 2 == 0 && ldw % 2 == 0 && ldw >= 256,
+            "invert_blocks256: bad shape");
+    require(reinterpret_cast<uintptr_t>(L) % 16 == 0 && reinterpret_cast<uintptr_t>(Wt) % 16 == 0,
+            "invert_blocks256: operands must be 16-byte aligned");
+    hipLaunchKernelGGL(k_invert_blocks256, dim3(static_cast<unsigned>(ceil_div(rows, 256L))), dim3(THREADS), 0, s, L, lda, Winv, row0,
+                       rows, Wt, ldw, block_stride);
+    GP_CHECK_LAUNCH();
+}
+
+template <class T>
+static void colsq_accumulate_impl(hipStream_t s, const T *V, long ldv, int rows, int m, double *colsq)
+{
+    require(rows >= 0 && m >= 0 && ldv >= m, "colsq_accumulate: bad shape");
+    if (rows == 0 || m == 0)
+        return;
+    hipLaunchKernelGGL(k_colsq_accumulate<T>, dim3(static_cast<unsigned>(ceil_div(m, 64))), dim3(THREADS), 0, s, V, ldv, rows, m,
+                       colsq);
+    GP_CHECK_LAUNCH();
+}
+
+void colsq_accumulate(hipStream_t s, const double *V, long ldv, int rows, int m, double *colsq)
+{
+    colsq_accumulate_impl<double>(s, V, ldv, rows, m, colsq);
+}
+
+void colsq_accumulate_f32(hipStream_t s, const float *V, long ldv, int rows, int m, double *colsq)
+{
+    colsq_accumulate_impl<float>(s, V, ldv, rows, m, colsq);
+}
+
+void to_f32(hipStream_t s, const double *src, float *dst, long count)
+{
+    require(count >= 0, "to_f32: negative count");
+    if (count == 0)
+        return;
+    hipLaunchKernelGGL(k_to_f32, dim3(static_cast<unsigned>(ceil_div(count, 256L))), dim3(THREADS), 0, s, src, dst, count);
+    GP_CHECK_LAUNCH();
+}
 
 void potrf64_inv(hipStream_t s, double *A, long lda, int c0, double *w, int *info)
 {

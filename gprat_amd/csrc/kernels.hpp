@@ -29,6 +29,19 @@ void assemble_cross_t(hipStream_t s,
                       int m,
                       int m_pad);
 
+// The same in FP32 (values computed in FP64, rounded once).
+void assemble_cross_t_f32(hipStream_t s,
+                          const double *x,
+                          const double *z,
+                          int r,
+                          SEK sek,
+                          float *B,
+                          long ldb,
+                          int n,
+                          int n_pad,
+                          int m,
+                          int m_pad);
+
 // mean[j] = sum_i k(test j, train i) alpha[i], K* generated on the fly.
 // `partial` holds splits * m doubles; returns nothing until the reduction
 // kernel (same stream) has written mean.
@@ -78,6 +91,10 @@ enum class GemmRole
     Solve,     // TRSM coupling update
     Inverse,   // W^T W
     PanelSolve,  // panel rows times the diagonal-block inverse
+    TrailingT,   // Cholesky trailing update from a k-major panel copy
+    SolveT,      // TRSM update from a k-major panel copy
+    LeafT,       // TRSM diagonal step: k-major block inverse times the right-hand side
+    PanelX,      // Cholesky panel rows times the diagonal block's inverse
 };
 
 struct GemmArgs
@@ -108,29 +125,36 @@ struct GemmArgs
 };
 void gemm_f64(hipStream_t s, const GemmArgs &g);
 
-// ---- FP32 matrix-core trailing update (k_gemm32.hip) -----------------------
+// ---- FP32 matrix-core trailing update, k-major panel copies (k_gemm32.hip) --
 
 // P[k][r] = (float) A[(r0 + r) * lda + p0 + k], r < R, k < W (k-major copy).
 void panel_to_f32t(hipStream_t s, const double *A, long lda, long r0, long p0, int R, int W, float *P, long ldp);
+// The same copy in FP64: the A^T operand layout of gemm_f64 (a_trans).
+void panel_to_f64t(hipStream_t s, const double *A, long lda, long r0, long p0, int R, int W, double *P, long ldp);
 // C[i][j] -= sum_k P[k][a_off + i] P[k][b_off + j] (FP32 products and sums),
 // for i < M, j < N, i + diag_off >= j; K % 32 == 0.
 void gemm_f32_trailing(hipStream_t s, const float *P, long ldp, long a_off, long b_off, double *C, long ldc, int M, int N, int K, long diag_off);
+// C (= or -=) A B over FP32 operands, A(i,k) = A[k*lda + i], B(k,j) = B[k*ldb + j],
+// FP32 C; K % 32 == 0, M, N, lda, ldb multiples of 4, A and B 16-byte aligned.
+void gemm_f32_tn(hipStream_t s, const float *A, long lda, const float *B, long ldb, float *C, long ldc, int M, int N, int K, bool set);
 
 // ---- batched fit of many GPs (k_batch.hip) --------------------------------
 
-// One GP of a batch: offsets (in doubles) of its series, targets, factor,
-// diagonal-block inverses and alpha inside the batch arena.
+// One GP of a batch: device addresses of its series (n + n_reg - 1),
+// targets (n_pad, zero padded), factor (n_pad x n_pad), diagonal-block
+// inverses (n_pad x 64) and alpha (n_pad); n_pad = n rounded up to 64.
 struct BatchGP
 {
-    long x_off, y_off, l_off, w_off, a_off;
+    const double *x, *y;
+    double *L, *W, *alpha;
     int n, n_pad, n_reg, pad_;
     double lengthscale, vertical, noise;
 };
 // Persistent fit of `count` GPs (visited in `order`, largest first) by
 // `workgroups` workgroups; losses[g] gets GP g's loss, info[g] its first
 // non-positive pivot column (INT_MAX if none).
-void batch_fit(hipStream_t s, const BatchGP *gps, const int *order, int count, int *counter, double *arena, double *losses,
-               int *info, int workgroups);
+void batch_fit(hipStream_t s, const BatchGP *gps, const int *order, int count, int *counter, double *losses, int *info,
+               int workgroups);
 
 // ---- triangular family (k_triangular.hip) ---------------------------------
 
@@ -146,6 +170,22 @@ void potrf64_inv(hipStream_t s, double *A, long lda, int c0, double *w, int *inf
 // run J descending.
 void solve_lower_step(hipStream_t s, const double *L, long lda, const double *Winv, double *rhs, double *out, int n_pad, int j);
 void solve_upper_step(hipStream_t s, const double *L, long lda, const double *Winv, double *rhs, double *out, int n_pad, int j);
+
+// Inverses of the 256-wide diagonal blocks of L covering rows
+// [row0, row0 + rows) (the last block may be narrower), built from the
+// 64-wide inverses Winv by W_ij = -W_ii sum_{t=j}^{i-1} L_it W_tj over
+// 64-blocks. Block b is stored k-major at Wt + b * block_stride with leading
+// dimension ldw (W(r, c) at [c * ldw + r]), zero above the diagonal and past
+// the block's last row within its 256 x 256 square.
+void invert_blocks256(hipStream_t s, const double *L, long lda, const double *Winv, long row0, long rows, double *Wt, long ldw,
+                      long block_stride);
+
+// colsq[j] += sum_{i<rows} V[i][j]^2 for j < m (fixed summation order;
+// FP64 sums for FP32 V too).
+void colsq_accumulate(hipStream_t s, const double *V, long ldv, int rows, int m, double *colsq);
+void colsq_accumulate_f32(hipStream_t s, const float *V, long ldv, int rows, int m, double *colsq);
+// dst[i] = (float) src[i]
+void to_f32(hipStream_t s, const double *src, float *dst, long count);
 
 // colsq[j] = sum_{i<n} V[i][j]^2 for j < m (deterministic two-pass);
 // `part` holds colsq_splits(n, m) * m doubles.

@@ -41,7 +41,8 @@ enum
     GPRAT_ERR_OUT_OF_MEMORY = 3,
     GPRAT_ERR_NOT_SPD = 4,
     GPRAT_ERR_NO_DEVICE = 5,
-    GPRAT_ERR_INTERNAL = 6
+    GPRAT_ERR_INTERNAL = 6,
+    GPRAT_ERR_PENDING = 7 /* gprat_server_wait: the request has not finished yet */
 };
 
 /* Factorisation precision for resident models. */
@@ -49,7 +50,8 @@ enum
 {
     GPRAT_PRECISION_FP64 = 0,
     /* FP64 panel (POTRF/TRSM) and log-det, FP32-MFMA trailing updates,
-     * FP64 iterative refinement of alpha and of the variance solves. */
+     * FP64 iterative refinement of alpha; the variance solves run in FP64
+     * against the mixed-precision factor. */
     GPRAT_PRECISION_MIXED = 1
 };
 
@@ -254,14 +256,85 @@ int gprat_batch_predict(gprat_batch *batch, int index, const double *test_in, si
                         double *var_out);
 int gprat_batch_release(gprat_batch *batch);
 
+/* ---- Serving: request loop with continuous batching ---------------------
+ * No counterpart in the reference, whose API is synchronous and keeps
+ * nothing on the device between calls (core/src/gprat_c.cpp:108-152,
+ * core/src/gpu/gp_functions.cu:81-87). A server owns `slots` worker threads,
+ * each with its own stream, so admitted requests overlap on the GPU. A
+ * request is admitted when resident factors plus the working memory of all
+ * admitted requests stay within `hbm_budget`; idle factors are evicted least
+ * recently used first and rebuilt from a host copy of their data when a
+ * predict needs them again. Fits with n <= batch_max_n that are waiting
+ * together run as one persistent batched launch (FP64); larger fits use
+ * `precision`. Every kernel reduces in a fixed order, so a result does not
+ * depend on what else was in flight. */
+
+typedef struct gprat_server gprat_server;
+
+typedef struct gprat_server_config
+{
+    int device;
+    int slots;          /* requests in flight at once, 1..16 */
+    size_t hbm_budget;  /* bytes; 0 = 70% of the device's memory */
+    int precision;      /* GPRAT_PRECISION_FP64 or GPRAT_PRECISION_MIXED for single-GP fits */
+    int batch_max_n;    /* fits with n <= batch_max_n are batched (0: never) */
+    int batch_max;      /* at most this many fits per batched launch (>= 1) */
+} gprat_server_config;
+
+typedef struct gprat_request_result
+{
+    int status;         /* GPRAT_OK or the request's error (message in `error`) */
+    int kind;           /* 0 fit, 1 predict */
+    long long model;    /* model id (a fit's own ticket) */
+    double loss;        /* fit: loss at theta */
+    double queue_ms;    /* submit -> admitted */
+    double service_ms;  /* admitted -> finished */
+    int refit;          /* predict: the evicted factor was rebuilt first */
+    int batched;        /* fit: GPs in its batched launch (0: single-GP path) */
+    char error[256];
+} gprat_request_result;
+
+typedef struct gprat_server_stats
+{
+    long long submitted, completed, failed;
+    long long fits, predicts, batched_fits, batch_launches;
+    long long evictions, refits, resident_models;
+    size_t resident_bytes, reserved_bytes, peak_bytes, budget;
+} gprat_server_stats;
+
+int gprat_server_create(const gprat_server_config *config, gprat_server **out);
+/* Runs every queued request to completion, then frees all models. */
+int gprat_server_destroy(gprat_server *server);
+/* Queue a fit (inputs copied before return). *ticket is also the model id. */
+int gprat_server_submit_fit(gprat_server *server, const gprat_gp_desc *gp, long long *ticket);
+/* Queue a predict against a model (its fit may still be queued). mean_out
+ * (m values) and var_out (m values, or NULL) are written when the request
+ * finishes and must stay valid until gprat_server_wait has returned it. */
+int gprat_server_submit_predict(gprat_server *server,
+                                long long model,
+                                const double *test_in,
+                                size_t test_in_len,
+                                int m,
+                                double *mean_out,
+                                double *var_out,
+                                long long *ticket);
+/* Wait up to timeout_ms (< 0: no limit) for a request. Returns GPRAT_OK with
+ * *result filled (the request's own status in result->status) and retires
+ * the ticket, or GPRAT_ERR_PENDING on timeout. */
+int gprat_server_wait(gprat_server *server, long long ticket, double timeout_ms, gprat_request_result *result);
+/* Drop a model; its factor is freed once no admitted request uses it. */
+int gprat_server_release_model(gprat_server *server, long long model);
+int gprat_server_get_stats(gprat_server *server, gprat_server_stats *out);
+
 /* ---- Measurement hooks ------------------------------------------------- */
 
 /* When enabled, the library brackets every launch of its named kernels with
  * HIP events on the launching stream and accumulates their durations. */
 int gprat_context_set_profiling(gprat_context *ctx, int enable);
 /* Launch count and summed duration (ms) for one kernel family since
- * profiling was enabled. Families: "syrk_trailing", "gemm_panel",
- * "potrf_trsm", "assemble", "trsm", "solve", "mean". */
+ * profiling was enabled. Families: "cholesky" (whole factorisation),
+ * "panel" (diagonal blocks), "panel_rows", "syrk_trailing", "assemble",
+ * "solve", "trsm", "mean", "refine", "inverse", "gradient", "batch_fit". */
 int gprat_context_kernel_stats(gprat_context *ctx, const char *family, long *launches, double *total_ms);
 
 #ifdef __cplusplus

@@ -52,6 +52,25 @@ def test_status_codes_and_errors_without_device():
     assert lib.gprat_model_release(None) == capi.OK
 
 
+def test_server_arguments_without_device():
+    from gprat_amd import capi
+    lib = capi.lib()
+    h = ctypes.c_void_p()
+    bad = capi.ServerConfig(0, 0, 0, capi.PRECISION_MIXED, 2048, 256)  # zero slots
+    assert lib.gprat_server_create(ctypes.byref(bad), ctypes.byref(h)) == capi.ERR_ARGUMENT
+    assert "slots" in lib.gprat_last_error().decode()
+    n = ctypes.c_int(-1)
+    assert lib.gprat_device_count(ctypes.byref(n)) == capi.OK
+    if n.value == 0:
+        cfg = capi.ServerConfig(0, 4, 0, capi.PRECISION_MIXED, 2048, 256)
+        assert lib.gprat_server_create(ctypes.byref(cfg), ctypes.byref(h)) == capi.ERR_NO_DEVICE
+    r = capi.RequestResult()
+    assert lib.gprat_server_wait(None, 1, 0.0, ctypes.byref(r)) == capi.ERR_ARGUMENT
+    assert lib.gprat_server_destroy(None) == capi.ERR_ARGUMENT
+    t = ctypes.c_longlong()
+    assert lib.gprat_server_submit_fit(None, None, ctypes.byref(t)) == capi.ERR_ARGUMENT
+
+
 def test_facade_api_surface(oracle):
     import gprat_amd as gprat
     for name in ["GP", "GP_data", "AdamParams", "SEKParams", "compute_train_tiles", "compute_train_tile_size",

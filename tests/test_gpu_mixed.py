@@ -1,12 +1,12 @@
-"""Mixed precision (FP32 matrix-core trailing updates, FP64 panels, FP64
-iterative refinement of alpha) against the FP64 path on the GPU.
+"""Mixed precision (FP32 matrix-core trailing updates, FP64 diagonal blocks,
+FP64 iterative refinement of alpha; variance solve on FP32 matrix cores)
+against the FP64 path on the GPU.
 
-Bar (BASELINE): predictive mean within 1e-8 (normwise) of FP64. The
-variance and the loss come from the reduced-accuracy factor without
-refinement; their errors are bounded here at the measured levels: variance
-1e-5 relative (normwise) and loss 1e-6 absolute (the loss is normalised per
-training point and sits near zero, so its relative error is inflated by
-cancellation)."""
+Bar: predictive mean within 1e-8 (normwise) of FP64. The variance and the
+loss come from the reduced-accuracy factor without refinement; their errors
+are bounded here at the measured levels: variance 1e-5 relative (normwise)
+and loss 1e-6 absolute (the loss is normalised per training point and sits
+near zero, so its relative error is inflated by cancellation)."""
 import numpy as np
 import pytest
 
@@ -23,7 +23,8 @@ def ctx():
     c.close()
 
 
-@pytest.mark.parametrize("n,r,m,noise", [(4096, 8, 1024, 0.1), (16384, 8, 4096, 0.1), (32768, 16, 2048, 0.05)])
+@pytest.mark.parametrize("n,r,m,noise", [(4096, 8, 1024, 0.1), (16384, 8, 4096, 0.1), (32768, 16, 2048, 0.05),
+                                         (65536, 16, 8192, 0.05)])
 def test_mixed_vs_fp64(ctx, n, r, m, noise):
     from gprat_amd import capi, datagen
     theta = (1.0, 1.0, noise)
