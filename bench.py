@@ -230,8 +230,8 @@ def serving_leg(capi, count=192, budget=200 << 30, slots=4):
       instead, where nothing stays on the device and every predict refits.
     * saturated: the whole trace queued at t = 0 on the server (mixed precision
       for single-GP fits, batched FP64 launches for n <= 2048, 200 GB budget).
-    * poisson: open-loop Poisson arrivals at 70% of the saturated rate;
-      latency = completion - arrival.
+    * poisson: open-loop Poisson arrivals at 50% and 80% of the saturated
+      rate; latency = completion - arrival.
     """
     import threading
     reqs = serving_trace(count)
@@ -291,9 +291,14 @@ def serving_leg(capi, count=192, budget=200 << 30, slots=4):
     with capi.Server(0, slots=slots, hbm_budget=budget, precision=capi.PRECISION_MIXED) as srv:
         run(srv, None)  # warm-up: device pools and code paths
         sat_total, sat_lat, sat_bad = run(srv, None)
-        rate = 0.7 * len(reqs) / sat_total
-        gaps = np.random.default_rng(11).exponential(1.0 / rate, len(reqs))
-        po_total, po_lat, po_bad = run(srv, np.cumsum(gaps) - gaps[0])
+        poisson, po_bad = [], 0
+        for frac in (0.5, 0.8):
+            rate = frac * len(reqs) / sat_total
+            gaps = np.random.default_rng(11).exponential(1.0 / rate, len(reqs))
+            po_total, po_lat, bad = run(srv, np.cumsum(gaps) - gaps[0])
+            po_bad += bad
+            poisson.append({"offered_req_s": rate, "fraction_of_saturated": frac, "achieved_req_s": len(reqs) / po_total,
+                            "p50_ms": percentile(po_lat, 50) * 1e3, "p99_ms": percentile(po_lat, 99) * 1e3})
         stats = srv.stats()
     n_fit = sum(r[0] == "fit" for r in reqs)
     return {
@@ -309,10 +314,7 @@ def serving_leg(capi, count=192, budget=200 << 30, slots=4):
         "speedup_vs_sequential_refit": refit_total / sat_total,
         "saturated_p50_ms": percentile(sat_lat, 50) * 1e3,
         "saturated_p99_ms": percentile(sat_lat, 99) * 1e3,
-        "poisson_rate_req_s": rate,
-        "poisson_req_s": len(reqs) / po_total,
-        "poisson_p50_ms": percentile(po_lat, 50) * 1e3,
-        "poisson_p99_ms": percentile(po_lat, 99) * 1e3,
+        "poisson": poisson,
         "sequential_p50_ms": percentile(times, 50) * 1e3,
         "sequential_p99_ms": percentile(times, 99) * 1e3,
         "failed": sat_bad + po_bad,
@@ -388,6 +390,10 @@ def main():
     }
     chol_ms = fams["cholesky"][1] / max(fams["cholesky"][0], 1)
 
+    calib = ctx.calibrate()
+    roofline["peak_measured"] = calib["fp64_tflops"]
+    roofline["frac_of_measured"] = (achieved / calib["fp64_tflops"]) if achieved else None
+
     chol = None
     if args.chol_n > 0:
         chol = cholesky_rate(capi, ctx, args.chol_n)
@@ -427,6 +433,7 @@ def main():
         "batched_2048_gps": batched,
         "serving": serving,
         "phase_ms_per_step": {k: v[1] / args.steps for k, v in fams.items()},
+        "device_calibration": calib,
         "roofline": roofline,
     }
     if not args.no_cpu_baseline:

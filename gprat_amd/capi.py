@@ -81,6 +81,7 @@ SIGNATURES = {
     "gprat_server_release_model": [_VP, _LL],
     "gprat_server_get_stats": [_VP, C.POINTER(ServerStats)],
     "gprat_context_set_profiling": [_VP, C.c_int],
+    "gprat_device_calibrate": [_VP, _SZ, _D, _D, _D],
     "gprat_context_kernel_stats": [_VP, C.c_char_p, C.POINTER(C.c_long), _D],
 }
 _RESTYPE = {"gprat_last_error": C.c_char_p, "gprat_version": C.c_char_p}
@@ -217,6 +218,12 @@ class Context:
 
     def synchronize(self):
         _check(lib().gprat_context_synchronize(self.handle))
+
+    def calibrate(self, copy_bytes: int = 0):
+        """Measured FP64 / FP32 matrix-core TFLOP/s and copy GB/s of this device."""
+        f64, f32, bw = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().gprat_device_calibrate(self.handle, copy_bytes, C.byref(f64), C.byref(f32), C.byref(bw)))
+        return {"fp64_tflops": f64.value, "fp32_tflops": f32.value, "copy_gbs": bw.value}
 
     # -- device buffers ------------------------------------------------------
     def device_array(self, host: np.ndarray) -> "DeviceArray":

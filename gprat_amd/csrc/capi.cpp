@@ -730,3 +730,49 @@ extern "C" int gprat_server_get_stats(gprat_server *server, gprat_server_stats *
             out->budget = s.budget;
         });
 }
+
+// ---------------------------------------------------------------------------
+// Calibration
+
+extern "C" int gprat_device_calibrate(gprat_context *ctx, size_t copy_bytes, double *fp64_tflops, double *fp32_tflops, double *copy_gbs)
+{
+    return guarded(
+        [&]
+        {
+            auto &c = ctx_ref(ctx);
+            gpamd::require(fp64_tflops != nullptr && fp32_tflops != nullptr && copy_gbs != nullptr, "calibrate: null output");
+            std::lock_guard<std::recursive_mutex> g(c.mu);
+            c.make_current();
+            hipDeviceProp_t prop;
+            GP_HIP(hipGetDeviceProperties(&prop, c.device));
+            const int wgs = 8 * prop.multiProcessorCount;
+            gpamd::DevBuf<double> out(c, static_cast<std::size_t>(wgs) * 256);
+            hipEvent_t a, b;
+            GP_HIP(hipEventCreate(&a));
+            GP_HIP(hipEventCreate(&b));
+            auto time_ms = [&](auto &&launch)
+            {
+                launch();  // warm-up
+                GP_HIP(hipEventRecord(a, c.main));
+                launch();
+                GP_HIP(hipEventRecord(b, c.main));
+                GP_HIP(hipEventSynchronize(b));
+                float ms = 0.0f;
+                GP_HIP(hipEventElapsedTime(&ms, a, b));
+                return static_cast<double>(ms);
+            };
+            long flops = 0;
+            double ms = time_ms([&] { flops = gpamd::k::mfma_loop(c.main, true, out.get(), wgs, 1000); });
+            *fp64_tflops = flops / (ms * 1e-3) / 1e12;
+            ms = time_ms([&] { flops = gpamd::k::mfma_loop(c.main, false, out.get(), wgs, 2000); });
+            *fp32_tflops = flops / (ms * 1e-3) / 1e12;
+            const std::size_t bytes = copy_bytes ? (copy_bytes / 16) * 16 : (std::size_t(4) << 30);
+            gpamd::DevBuf<double> src(c, bytes / 8), dst(c, bytes / 8);
+            gpamd::k::fill(c.main, src.get(), static_cast<long>(bytes / 8), 1.0);
+            ms = time_ms([&] { gpamd::k::copy_f64(c.main, src.get(), dst.get(), static_cast<long>(bytes / 8), 4 * prop.multiProcessorCount); });
+            *copy_gbs = 2.0 * bytes / (ms * 1e-3) / 1e9;
+            GP_HIP(hipEventDestroy(a));
+            GP_HIP(hipEventDestroy(b));
+            c.sync();
+        });
+}

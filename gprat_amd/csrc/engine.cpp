@@ -135,6 +135,7 @@ template class DevBuf<int>;
 Context::Context(int dev, int ns) : device(dev), n_streams(ns), pool(dev)
 {
     make_current();
+    GP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     GP_HIP(hipStreamCreateWithFlags(&main, hipStreamNonBlocking));
     panel = trail = main;
     // one stream: the panel factorisation runs in line (no look-ahead)
@@ -285,9 +286,21 @@ static int outer_block(int n_pad)
     return n_pad <= 4096 ? 256 : 512;
 }
 
+// Workgroup slots (out of two per CU) that look-ahead launches on the
+// trailing stream leave free, so the panel stream's small kernels start at
+// once instead of queueing behind a saturating update (no stream
+// priorities); GPRAT_TRAILING_RESERVE overrides.
+static int trailing_reserve()
+{
+    if (const char *e = std::getenv("GPRAT_TRAILING_RESERVE"))
+        return std::max(0, std::atoi(e));
+    return 16;
+}
+
 // C[i][j] -= sum_k P[k][off + i] P[k][off + j] for i < M, j < N, i >= j, with
 // P the k-major copy of a factored panel (leading dimension ldp).
-static void trailing_kmajor(hipStream_t s, const double *P, long ldp, long off, double *C, long ldc, int M, int N, int K)
+static void trailing_kmajor(hipStream_t s, const double *P, long ldp, long off, double *C, long ldc, int M, int N, int K,
+                            int max_wgs)
 {
     k::GemmArgs g{};
     g.A = P + off;
@@ -303,6 +316,7 @@ static void trailing_kmajor(hipStream_t s, const double *P, long ldp, long off, 
     g.b_trans = false;
     g.lower = true;
     g.role = k::GemmRole::TrailingT;
+    g.max_wgs = max_wgs;
     k::gemm_f64(s, g);
 }
 
@@ -403,7 +417,8 @@ static void invert_diagonal_block(hipStream_t s, const double *A, long lda, cons
 // launch from right to left: X[:, c:c+128] needs A[:, 0:c+128] only (W is
 // lower triangular) and every launch has a single column tile, so no
 // workgroup reads what another writes.
-static void solve_panel_rows(hipStream_t s, double *A, long lda, int p0, int w, int r0, int n_pad, const double *Wb, long ldw)
+static void solve_panel_rows(hipStream_t s, double *A, long lda, int p0, int w, int r0, int n_pad, const double *Wb, long ldw,
+                             int max_wgs)
 {
     if (r0 >= n_pad)
         return;
@@ -421,6 +436,7 @@ static void solve_panel_rows(hipStream_t s, double *A, long lda, int p0, int w, 
         g.K = cc + 128;
         g.set = true;
         g.role = k::GemmRole::PanelX;
+        g.max_wgs = max_wgs;
         k::gemm_f64(s, g);
     }
 }
@@ -439,11 +455,14 @@ static void solve_panel_rows(hipStream_t s, double *A, long lda, int p0, int w, 
 // being factored, then solve panel k+1's rows.
 void cholesky_fp64(Context &c, double *A, long lda, int n_pad, double *Winv) { cholesky(c, A, lda, n_pad, Winv, false); }
 
-void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mixed)
+int panel_width(int n_pad) { return outer_block(n_pad); }
+
+void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mixed, double *Wpanel)
 {
     require(n_pad % 128 == 0 && lda >= n_pad && lda % 2 == 0, "cholesky: bad layout");
-    const bool ahead = use_lookahead();
+    const bool ahead = use_lookahead() && c.panel != c.main;
     hipStream_t T = ahead ? c.trail : c.main, P = ahead ? c.panel : c.main;
+    const int cap = ahead ? std::max(1, 2 * c.cus - trailing_reserve()) : 0;
     const int Bo = outer_block(n_pad);
     const int nblk = static_cast<int>(ceil_div(n_pad, Bo));
     k::set_int(c.main, c.info.get(), INT_MAX);
@@ -455,7 +474,12 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
         p32 = DevBuf<float>(c, static_cast<std::size_t>(Bo) * n_pad);
     else
         p64 = DevBuf<double>(c, static_cast<std::size_t>(Bo) * n_pad);
-    DevBuf<double> Wb(c, static_cast<std::size_t>(Bo) * Bo), tmp(c, static_cast<std::size_t>(Bo / 2) * (Bo / 2) + 256 * 256);
+    // inverses of the panels' diagonal blocks: kept (Wpanel, block kb at
+    // kb * Bo * Bo) or one scratch block reused by every panel
+    DevBuf<double> Wscratch, tmp(c, static_cast<std::size_t>(Bo / 2) * (Bo / 2) + 256 * 256);
+    if (!Wpanel)
+        Wscratch = DevBuf<double>(c, static_cast<std::size_t>(Bo) * Bo);
+    auto Wblock = [&](int p0) { return Wpanel ? Wpanel + static_cast<long>(p0 / Bo) * Bo * Bo : Wscratch.get(); };
 
     std::vector<hipEvent_t> used;
     auto ev = [&]()
@@ -476,16 +500,16 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
     {
         Timed t(c, P, "panel");
         factor_diagonal_block(c, P, A, lda, p0, pe, Winv);
-        if (pe < n_pad)
+        if (pe < n_pad || Wpanel)
         {
-            k::fill(P, Wb.get(), static_cast<long>(Bo) * Bo, 0.0);
-            invert_diagonal_block(P, A, lda, Winv, p0, pe - p0, Wb.get(), Bo, tmp.get());
+            k::fill(P, Wblock(p0), static_cast<long>(Bo) * Bo, 0.0);
+            invert_diagonal_block(P, A, lda, Winv, p0, pe - p0, Wblock(p0), Bo, tmp.get());
         }
     };
     auto rows_below = [&](int p0, int pe)
     {
         Timed t(c, T, "panel_rows");
-        solve_panel_rows(T, A, lda, p0, pe - p0, pe, n_pad, Wb.get(), Bo);
+        solve_panel_rows(T, A, lda, p0, pe - p0, pe, n_pad, Wblock(p0), Bo, cap);
         if (mixed)
             k::panel_to_f32t(T, A, lda, pe, p0, n_pad - pe, pe - p0, p32.get(), n_pad);
         else
@@ -495,9 +519,9 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
     {
         Timed t(c, T, "syrk_trailing");
         if (mixed)
-            k::gemm_f32_trailing(T, p32.get(), n_pad, off, off, C, lda, M, N, K, 0);
+            k::gemm_f32_trailing(T, p32.get(), n_pad, off, off, C, lda, M, N, K, 0, cap);
         else
-            trailing_kmajor(T, p64.get(), n_pad, off, C, lda, M, N, K);
+            trailing_kmajor(T, p64.get(), n_pad, off, C, lda, M, N, K, cap);
     };
     {
         Timed whole(c, c.main, "cholesky");
@@ -598,24 +622,31 @@ void trsm_lower_left(Context &c, const double *L, long lda, const double *Winv, 
     trsm_lower_left(c, L, lda, Winv, B, ldb, ncols, mid, b, tri_rhs);
 }
 
-// out = L^-1 B, blocked right-looking in 256-wide steps: the diagonal step
-// out_J = W256_J B_J uses the k-major 256-block inverses, the update
-// B[below] -= L[below, J] out_J reads a k-major copy of L's column panel, so
-// both products take whole 1 KiB rows of their A operand. B is
-// overwritten. With tri_rhs, B is lower triangular (B = I for L^-1) and each
-// step's columns stop at the end of its block. With `staged`, out is a
-// single 256-row block reused by every step (only the squares are wanted);
-// colsq, when given, accumulates the column sums of squares of out over
-// rows, for columns < mt.
-void trsm_blocked(Context &c, const double *L, long lda, const double *Winv, int n_pad, double *B, long ldb, int ncols, bool tri_rhs,
-                  double *out, long ldo, bool staged, double *colsq, int mt)
+// out = L^-1 B, blocked right-looking in NBK-wide steps: the diagonal step
+// out_J = W_J B_J uses the k-major inverse of the diagonal block (the
+// factorisation's panel inverses when given, NBK = their width, else
+// 256-wide inverses built here), the update B[below] -= L[below, J] out_J
+// reads a k-major copy of L's column panel, so both products take whole
+// 1 KiB rows of their A operand. B is overwritten. With tri_rhs, B is lower
+// triangular (B = I for L^-1) and each step's columns stop at the end of
+// its block. With `staged`, out is a single NBK-row block reused by every
+// step (only the squares are wanted); colsq, when given, accumulates the
+// column sums of squares of out over rows, for columns < mt.
+void trsm_blocked(Context &c, const double *L, long lda, const double *Winv, const double *Wpanel, int wblock, int n_pad, double *B,
+                  long ldb, int ncols, bool tri_rhs, double *out, long ldo, bool staged, double *colsq, int mt)
 {
-    constexpr int NBK = 256;
     hipStream_t s = c.main;
+    const int NBK = Wpanel ? wblock : 256;
+    require(NBK % 64 == 0 && NBK > 0, "trsm_blocked: bad block width");
     const int nblk = static_cast<int>(ceil_div(n_pad, NBK));
-    DevBuf<double> Wt(c, static_cast<std::size_t>(nblk) * NBK * NBK);
+    DevBuf<double> Wt;
+    if (!Wpanel)
+    {
+        Wt = DevBuf<double>(c, static_cast<std::size_t>(nblk) * NBK * NBK);
+        k::invert_blocks256(s, L, lda, Winv, 0, n_pad, Wt.get(), NBK, static_cast<long>(NBK) * NBK);
+    }
+    const double *W = Wpanel ? Wpanel : Wt.get();
     DevBuf<double> Pt(c, static_cast<std::size_t>(NBK) * n_pad);
-    k::invert_blocks256(s, L, lda, Winv, 0, n_pad, Wt.get(), NBK, static_cast<long>(NBK) * NBK);
     for (int J = 0; J < nblk; J++)
     {
         const int j0 = J * NBK, nb = std::min(NBK, n_pad - j0), j1 = j0 + nb;
@@ -623,7 +654,7 @@ void trsm_blocked(Context &c, const double *L, long lda, const double *Winv, int
         double *oJ = staged ? out : out + static_cast<long>(j0) * ldo;
         {
             k::GemmArgs g{};
-            g.A = Wt.get() + static_cast<long>(J) * NBK * NBK;
+            g.A = W + static_cast<long>(J) * NBK * NBK;
             g.lda = NBK;
             g.a_trans = true;
             g.B = B + static_cast<long>(j0) * ldb;
@@ -664,12 +695,15 @@ void trsm_blocked(Context &c, const double *L, long lda, const double *Winv, int
 // squares are summed in FP64.
 static void variance_colsq_f32(Context &c, const FactorView &m, const double *z, int mt, int m_pad, double *colsq)
 {
-    constexpr int NBK = 256;
     hipStream_t s = c.main;
+    const int NBK = m.Wpanel ? m.wblock : 256;
     const int n_pad = m.n_pad;
     const int nblk = static_cast<int>(ceil_div(n_pad, NBK));
     const std::size_t wcount = static_cast<std::size_t>(nblk) * NBK * NBK;
     DevBuf<float> Wt32(c, wcount);
+    if (m.Wpanel)
+        k::to_f32(s, m.Wpanel, Wt32.get(), static_cast<long>(wcount));
+    else
     {
         DevBuf<double> Wt(c, wcount);
         k::invert_blocks256(s, m.L, m.lda, m.Winv, 0, n_pad, Wt.get(), NBK, static_cast<long>(NBK) * NBK);
@@ -727,12 +761,14 @@ void factor(Context &c, Model &m, const double *x, std::size_t x_len, int n, int
     m.x = DevBuf<double>(c, xl);
     m.L = DevBuf<double>(c, static_cast<std::size_t>(m.n_pad) * m.lda);
     m.Winv = DevBuf<double>(c, static_cast<std::size_t>(m.n_pad) * 64);
+    m.wblock = panel_width(m.n_pad);
+    m.Wpanel = DevBuf<double>(c, static_cast<std::size_t>(ceil_div(m.n_pad, m.wblock)) * m.wblock * m.wblock);
     upload(c, m.x.get(), x, xl);
     {
         Timed t(c, c.main, "assemble");
         k::assemble_lower(c.main, m.x.get(), r, m.sek, m.L.get(), m.lda, n, m.n_pad);
     }
-    cholesky(c, m.L.get(), m.lda, m.n_pad, m.Winv.get(), precision == GPRAT_PRECISION_MIXED);
+    cholesky(c, m.L.get(), m.lda, m.n_pad, m.Winv.get(), precision == GPRAT_PRECISION_MIXED, m.Wpanel.get());
 }
 
 // FP64 iterative refinement of alpha against a factor of reduced accuracy:
@@ -855,9 +891,10 @@ void predict_view(const FactorView &m, const double *z, std::size_t z_len, int m
         }
         {
             Timed t(c, c.main, "trsm");
-            DevBuf<double> V(c, static_cast<std::size_t>(256) * ldb);
+            DevBuf<double> V(c, static_cast<std::size_t>(m.Wpanel ? m.wblock : 256) * ldb);
             k::fill(c.main, colsq.get(), m_pad, 0.0);
-            trsm_blocked(c, m.L, m.lda, m.Winv, m.n_pad, B.get(), ldb, m_pad, false, V.get(), ldb, true, colsq.get(), mt);
+            trsm_blocked(c, m.L, m.lda, m.Winv, m.Wpanel, m.wblock, m.n_pad, B.get(), ldb, m_pad, false, V.get(), ldb, true, colsq.get(),
+                         mt);
         }
         k::variance_finish(c.main, colsq.get(), m.sek.vertical, mt, dvar.get());
         GP_HIP(hipMemcpyAsync(var, dvar.get(), sizeof(double) * mt, hipMemcpyDefault, c.main));
@@ -869,7 +906,7 @@ void predict_view(const FactorView &m, const double *z, std::size_t z_len, int m
 void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean, double *var)
 {
     predict_view(FactorView{ m.ctx, m.x.get(), m.n, m.n_pad, m.r, m.lda, m.sek, m.L.get(), m.Winv.get(), m.alpha.get(),
-                             m.precision == GPRAT_PRECISION_MIXED },
+                             m.precision == GPRAT_PRECISION_MIXED, m.Wpanel.get(), m.wblock },
                  z, z_len, mt, mean, var);
 }
 
@@ -1060,7 +1097,8 @@ void loss_gradient(Context &c,
     }
     {
         Timed tm(c, c.main, "inverse");
-        trsm_blocked(c, m.L.get(), m.lda, m.Winv.get(), m.n_pad, Kinv.get(), m.lda, m.n_pad, true, W.get(), m.lda, false, nullptr, 0);
+        trsm_blocked(c, m.L.get(), m.lda, m.Winv.get(), m.Wpanel.get(), m.wblock, m.n_pad, Kinv.get(), m.lda, m.n_pad, true, W.get(),
+                     m.lda, false, nullptr, 0);
         k::GemmArgs g{};
         g.A = W.get();
         g.lda = m.lda;

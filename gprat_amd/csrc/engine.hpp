@@ -81,6 +81,7 @@ struct Context
 
     int device;
     int n_streams;
+    int cus = 0;  // compute units of the device
     hipStream_t main = nullptr;   // solves, predictions, everything outside the Cholesky
     hipStream_t panel = nullptr;  // Cholesky diagonal blocks (look-ahead); == main with one stream
     hipStream_t trail = nullptr;  // Cholesky trailing updates and panel rows (== main)
@@ -125,8 +126,10 @@ struct Model
     DevBuf<double> y;      // targets, zero padded to n_pad
     DevBuf<double> L;      // n_pad x lda, lower triangle holds the factor
     DevBuf<double> Winv;   // inverses of the 64x64 diagonal blocks of L (n_pad/64 x 64 x 64)
+    DevBuf<double> Wpanel; // k-major inverses of the factorisation's wblock-wide diagonal blocks (may be empty)
+    int wblock = 0;
     DevBuf<double> alpha;  // K^-1 y (n_pad)
-    std::size_t bytes() const { return x.bytes() + y.bytes() + L.bytes() + Winv.bytes() + alpha.bytes(); }
+    std::size_t bytes() const { return x.bytes() + y.bytes() + L.bytes() + Winv.bytes() + Wpanel.bytes() + alpha.bytes(); }
 };
 
 // Raw view of a factored GP (resident model or one GP of a batch).
@@ -140,7 +143,9 @@ struct FactorView
     const double *L;
     const double *Winv;
     const double *alpha;
-    bool mixed = false;  // variance solve in FP32 products (mixed-precision models)
+    bool mixed = false;               // variance solve in FP32 products (mixed-precision models)
+    const double *Wpanel = nullptr;   // k-major wblock-wide diagonal-block inverses, if kept
+    int wblock = 0;
 };
 
 // Many GPs fitted by one persistent launch; factors stay in one arena.
@@ -159,12 +164,15 @@ long padded(long n);
 long leading_dim(long n_pad);
 void cholesky_fp64(Context &c, double *A, long lda, int n_pad, double *Winv);
 // mixed: FP32 matrix-core trailing updates (FP64 panels and diagonal blocks).
-void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mixed);
+// Wpanel (optional): receives the k-major inverses of the panels' diagonal
+// blocks, panel_width(n_pad) wide, block k at Wpanel + k * width^2.
+void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mixed, double *Wpanel = nullptr);
+int panel_width(int n_pad);
 void refine_alpha(Model &m, double *work);
 void solve_spd(Context &c, const double *L, long lda, const double *Winv, int n_pad, const double *y, double *work, double *out);
 void trsm_lower_left(Context &c, const double *L, long lda, const double *Winv, double *B, long ldb, int ncols, int a, int b, bool tri_rhs);
-void trsm_blocked(Context &c, const double *L, long lda, const double *Winv, int n_pad, double *B, long ldb, int ncols, bool tri_rhs,
-                  double *out, long ldo, bool staged, double *colsq, int mt);
+void trsm_blocked(Context &c, const double *L, long lda, const double *Winv, const double *Wpanel, int wblock, int n_pad, double *B,
+                  long ldb, int ncols, bool tri_rhs, double *out, long ldo, bool staged, double *colsq, int mt);
 
 void factor(Context &c, Model &m, const double *x, std::size_t x_len, int n, int r, const double theta[3], int precision);
 void attach_targets(Model &m, const double *y, std::size_t y_len);

@@ -128,16 +128,10 @@ __device__ __forceinline__ void mfma_slab(const double *As, const double *Bs, in
     }
 }
 
-template <bool AT, bool BT, int ROLE>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gemm_f64(Params p)
+// One 128x128 tile of C at (r0, c0).
+template <bool AT, bool BT>
+__device__ __forceinline__ void gemm_tile(const Params &p, long r0, long c0, double (*As)[BK * LDS_STRIDE], double (*Bs)[BK * LDS_STRIDE])
 {
-    __shared__ double As[2][BK * LDS_STRIDE];
-    __shared__ double Bs[2][BK * LDS_STRIDE];
-
-    const long r0 = static_cast<long>(blockIdx.y) * BM;
-    const long c0 = static_cast<long>(blockIdx.x) * BN;
-    if (p.lower && r0 + (BM - 1) + p.diag_off < c0)
-        return;  // tile entirely above the diagonal
     long kbeg = 0;
     if (p.k_from_tile)
     {
@@ -227,6 +221,39 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
             }
 }
 
+// One workgroup per tile (grid x = column tiles, y = row tiles).
+template <bool AT, bool BT, int ROLE>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gemm_f64(Params p)
+{
+    __shared__ double As[2][BK * LDS_STRIDE];
+    __shared__ double Bs[2][BK * LDS_STRIDE];
+    const long r0 = static_cast<long>(blockIdx.y) * BM;
+    const long c0 = static_cast<long>(blockIdx.x) * BN;
+    if (p.lower && r0 + (BM - 1) + p.diag_off < c0)
+        return;  // tile entirely above the diagonal
+    gemm_tile<AT, BT>(p, r0, c0, As, Bs);
+}
+
+// Capped launch: G workgroups loop over the row-major tiles (w, w + G, ...),
+// so the launch leaves workgroup slots free for kernels on other streams.
+template <bool AT, bool BT, int ROLE>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gemm_f64_capped(Params p)
+{
+    __shared__ double As[2][BK * LDS_STRIDE];
+    __shared__ double Bs[2][BK * LDS_STRIDE];
+    const long gx = (p.N + BN - 1) / BN;
+    const long tiles = gx * ((p.M + BM - 1) / BM);
+    for (long t = blockIdx.x; t < tiles; t += gridDim.x)
+    {
+        const long r0 = (t / gx) * BM;
+        const long c0 = (t % gx) * BN;
+        if (p.lower && r0 + (BM - 1) + p.diag_off < c0)
+            continue;  // tile entirely above the diagonal
+        __syncthreads();  // the previous tile's last slab has been read by every wavefront
+        gemm_tile<AT, BT>(p, r0, c0, As, Bs);
+    }
+}
+
 }  // namespace
 
 void gemm_f64(hipStream_t s, const GemmArgs &g)
@@ -244,6 +271,15 @@ void gemm_f64(hipStream_t s, const GemmArgs &g)
               g.kstart_off, g.set ? 1 : 0 };
     dim3 grid(static_cast<unsigned>(ceil_div(g.N, BN)), static_cast<unsigned>(ceil_div(g.M, BM)));
     require(grid.y <= 65535, "gemm_f64: M too large for one launch");
+    const long tiles = static_cast<long>(grid.x) * grid.y;
+    if (g.max_wgs > 0 && g.max_wgs < tiles)
+    {
+        // only the trailing update runs capped
+        require(g.a_trans && !g.b_trans && g.role == GemmRole::TrailingT, "gemm_f64: capped launch for this role");
+        hipLaunchKernelGGL((k_gemm_f64_capped<true, false, 5>), dim3(static_cast<unsigned>(g.max_wgs)), dim3(THREADS), 0, s, p);
+        GP_CHECK_LAUNCH();
+        return;
+    }
     // Instantiated combinations: (N,T) trailing/panel, (N,N) solve, (T,N) inverse
     // and the updates fed by k-major panel copies.
     if (!g.a_trans && g.b_trans && g.role == GemmRole::Trailing)

@@ -40,6 +40,7 @@ struct Params32
     int lower;  // only i + diag_off >= j
     long diag_off;
     int set;  // C = acc instead of C -= acc
+    int max_wgs;  // > 0: at most this many workgroups, looping over tiles
 };
 
 // Element offsets (within one slab) of this thread's four 16-byte loads,
@@ -78,18 +79,10 @@ __device__ __forceinline__ void mfma_slab32(const float *As, const float *Bs, in
     }
 }
 
-// ROLE only separates profiler names: 0 trailing update (FP64 C), 1 TRSM
-// diagonal step, 2 TRSM update.
-template <class TC, int ROLE>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_f32_tn(Params32<TC> p)
+// One 128x128 tile of C at (r0, c0).
+template <class TC>
+__device__ __forceinline__ void gemm_tile32(const Params32<TC> &p, long r0, long c0, float (*As)[BK * KS], float (*Bs)[BK * KS])
 {
-    __shared__ float As[2][BK * KS];
-    __shared__ float Bs[2][BK * KS];
-
-    const long r0 = static_cast<long>(blockIdx.y) * BM;
-    const long c0 = static_cast<long>(blockIdx.x) * BM;
-    if (p.lower && r0 + (BM - 1) + p.diag_off < c0)
-        return;  // tile entirely above the diagonal
     const int rows_a = static_cast<int>(min(static_cast<long>(BM), p.M - r0));
     const int rows_b = static_cast<int>(min(static_cast<long>(BM), p.N - c0));
 
@@ -179,6 +172,27 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2))) vo
             }
 }
 
+// ROLE only separates profiler names: 0 trailing update (FP64 C), 1 TRSM
+// diagonal step, 2 TRSM update. Tiles are numbered row-major; a launch with
+// fewer workgroups than tiles loops over them (tile w, w + G, ...).
+template <class TC, int ROLE>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_f32_tn(Params32<TC> p)
+{
+    __shared__ float As[2][BK * KS];
+    __shared__ float Bs[2][BK * KS];
+    const long gx = (p.N + BM - 1) / BM;
+    const long tiles = gx * ((p.M + BM - 1) / BM);
+    for (long t = blockIdx.x; t < tiles; t += gridDim.x)
+    {
+        const long r0 = (t / gx) * BM;
+        const long c0 = (t % gx) * BM;
+        if (p.lower && r0 + (BM - 1) + p.diag_off < c0)
+            continue;  // tile entirely above the diagonal
+        __syncthreads();  // the previous tile's last slab has been read by every wavefront
+        gemm_tile32<TC>(p, r0, c0, As, Bs);
+    }
+}
+
 // P[k][r] = (T) A[(r0 + r) * lda + p0 + k] for r < R, k < W, through a
 // 64x64 LDS tile so both the FP64 reads and the k-major writes are coalesced.
 template <class T>
@@ -236,21 +250,23 @@ static void launch_f32_tn(hipStream_t s, const Params32<TC> &p)
                 && reinterpret_cast<uintptr_t>(p.A) % 16 == 0 && reinterpret_cast<uintptr_t>(p.B) % 16 == 0,
             "FP32 product: operand rows must come in aligned groups of four");
     require(static_cast<long>(BK) * std::max(p.lda, p.ldb) + BM < (1L << 31), "FP32 product: operand too tall");
-    dim3 grid(static_cast<unsigned>(ceil_div(p.N, BM)), static_cast<unsigned>(ceil_div(p.M, BM)));
-    require(grid.y <= 65535, "FP32 product: M too large for one launch");
+    const long tiles = ceil_div(static_cast<long>(p.N), static_cast<long>(BM)) * ceil_div(static_cast<long>(p.M), static_cast<long>(BM));
+    require(tiles < (1L << 31), "FP32 product: too many tiles for one launch");
+    dim3 grid(static_cast<unsigned>(p.max_wgs > 0 ? std::min<long>(tiles, p.max_wgs) : tiles));
     hipLaunchKernelGGL((k_gemm_f32_tn<TC, ROLE>), grid, dim3(THREADS), 0, s, p);
     GP_CHECK_LAUNCH();
 }
 
-void gemm_f32_trailing(hipStream_t s, const float *P, long ldp, long a_off, long b_off, double *C, long ldc, int M, int N, int K, long diag_off)
+void gemm_f32_trailing(hipStream_t s, const float *P, long ldp, long a_off, long b_off, double *C, long ldc, int M, int N, int K, long diag_off,
+                       int max_wgs)
 {
-    Params32<double> p{ P + a_off, ldp, P + b_off, ldp, C, ldc, M, N, K, 1, diag_off, 0 };
+    Params32<double> p{ P + a_off, ldp, P + b_off, ldp, C, ldc, M, N, K, 1, diag_off, 0, max_wgs };
     launch_f32_tn<double, 0>(s, p);
 }
 
 void gemm_f32_tn(hipStream_t s, const float *A, long lda, const float *B, long ldb, float *C, long ldc, int M, int N, int K, bool set)
 {
-    Params32<float> p{ A, lda, B, ldb, C, ldc, M, N, K, 0, 0, set ? 1 : 0 };
+    Params32<float> p{ A, lda, B, ldb, C, ldc, M, N, K, 0, 0, set ? 1 : 0, 0 };
     if (set)
         launch_f32_tn<float, 1>(s, p);
     else

@@ -122,6 +122,9 @@ struct GemmArgs
     // true: C = acc; false: C -= acc.
     bool set;
     GemmRole role;
+    // > 0: at most this many workgroups, each looping over tiles, so the
+    // launch leaves workgroup slots free for kernels on other streams.
+    int max_wgs;
 };
 void gemm_f64(hipStream_t s, const GemmArgs &g);
 
@@ -133,7 +136,9 @@ void panel_to_f32t(hipStream_t s, const double *A, long lda, long r0, long p0, i
 void panel_to_f64t(hipStream_t s, const double *A, long lda, long r0, long p0, int R, int W, double *P, long ldp);
 // C[i][j] -= sum_k P[k][a_off + i] P[k][b_off + j] (FP32 products and sums),
 // for i < M, j < N, i + diag_off >= j; K % 32 == 0.
-void gemm_f32_trailing(hipStream_t s, const float *P, long ldp, long a_off, long b_off, double *C, long ldc, int M, int N, int K, long diag_off);
+// max_wgs > 0 caps the launch (workgroups loop over tiles).
+void gemm_f32_trailing(hipStream_t s, const float *P, long ldp, long a_off, long b_off, double *C, long ldc, int M, int N, int K, long diag_off,
+                       int max_wgs = 0);
 // C (= or -=) A B over FP32 operands, A(i,k) = A[k*lda + i], B(k,j) = B[k*ldb + j],
 // FP32 C; K % 32 == 0, M, N, lda, ldb multiples of 4, A and B 16-byte aligned.
 void gemm_f32_tn(hipStream_t s, const float *A, long lda, const float *B, long ldb, float *C, long ldc, int M, int N, int K, bool set);
@@ -210,5 +215,14 @@ void fill(hipStream_t s, double *p, long count, double value);
 void set_diagonal(hipStream_t s, double *A, long lda, int n, double value);
 // Store one int (used to reset the pivot-failure flag without a memset node).
 void set_int(hipStream_t s, int *p, int value);
+
+// ---- calibration (k_calibrate.hip) ---------------------------------------
+
+// Back-to-back MFMAs from registers on `workgroups` x 256 threads (FP64
+// 16x16x4 or FP32 16x16x4); returns the flops of the launch. out receives one
+// double per thread.
+long mfma_loop(hipStream_t s, bool fp64, double *out, int workgroups, int iters);
+// dst[i] = src[i], i < count (count even, 16-byte aligned pointers).
+void copy_f64(hipStream_t s, const double *src, double *dst, long count, int workgroups);
 
 }  // namespace gpamd::k

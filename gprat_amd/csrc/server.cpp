@@ -197,7 +197,8 @@ std::size_t Server::fit_need(const Entry &e) const
         // and refinement vectors, the refinement's partial sums
         temp = 8 * (2 * 512 * np + 8 * np + static_cast<std::size_t>(k::mean_splits(e.n, e.n)) * n);
     }
-    const std::size_t resident = 8 * (np * ld + np * 64 + 2 * np + n + r);
+    // factor, 64-block inverses, panel inverses (at most 512 wide), vectors
+    const std::size_t resident = 8 * (np * ld + np * 64 + (e.batched_path ? 0 : np * 512) + 2 * np + n + r);
     return resident + temp + (std::size_t(1) << 20);
 }
 
@@ -208,6 +209,18 @@ std::size_t Server::predict_need(const Entry &e, int m) const
     const std::size_t blocks = (np + 255) / 256;
     const std::size_t splits = static_cast<std::size_t>(k::mean_splits(e.n, m));
     return 8 * (np * mp + 256 * mp + 4 * mp + blocks * 65536 + 256 * np + splits * m + m + e.r) + (std::size_t(1) << 20);
+}
+
+// Estimated flops of a request: the factorisation n^3/3 for a fit (or a
+// predict that rebuilds an evicted factor) plus n^2 m for a predict's
+// variance solve.
+double Server::cost(const Request &r, const Entry &e) const
+{
+    const double n = e.n;
+    double f = (r.kind == 0 || !e.model) ? n * n * n / 3.0 : 0.0;
+    if (r.kind == 1)
+        f += n * n * r.m;
+    return f;
 }
 
 // Fits the budget, evicting idle resident factors least recently used first.
@@ -294,6 +307,12 @@ bool Server::pick(Job &job)
         {
             need = fit_need(e);
         }
+        const bool heavy = cost(r, e) > cfg_.heavy_flops && !(r.kind == 0 && e.batched_path);
+        if (heavy && heavy_in_flight_ >= std::max(1, cfg_.slots / 2))
+        {
+            ++it;  // keeps its place; short requests behind it may start
+            continue;
+        }
         if (!admit(need))
         {
             if (in_flight_ > 0)
@@ -316,6 +335,8 @@ bool Server::pick(Job &job)
         const auto now = Clock::now();
         r.need = need;
         r.admitted = now;
+        r.heavy = heavy;
+        heavy_in_flight_ += heavy ? 1 : 0;
         reserved_ += need;
         if (r.kind == 1)
         {
@@ -518,7 +539,7 @@ void Server::run(Context &c, Job &job, std::vector<RequestResult> &res)
                 res[0].refit = 1;
             }
             predict_view(FactorView{ &c, m->x.get(), m->n, m->n_pad, m->r, m->lda, m->sek, m->L.get(), m->Winv.get(), m->alpha.get(),
-                                     m->precision == GPRAT_PRECISION_MIXED },
+                                     m->precision == GPRAT_PRECISION_MIXED, m->Wpanel.get(), m->wblock },
                          r0.z.data(), r0.z.size(), r0.m, r0.mean, r0.var);
         }
     }
@@ -554,6 +575,7 @@ void Server::finish(Job &job, std::vector<RequestResult> &res)
     {
         Request &r = job.reqs[i];
         reserved_ -= r.need;
+        heavy_in_flight_ -= r.heavy ? 1 : 0;
         auto mit = models_.find(r.model);
         Entry &e = *mit->second;
         auto b = job.built.find(e.id);

@@ -35,6 +35,10 @@ struct ServerConfig
     int precision = GPRAT_PRECISION_MIXED;
     int batch_max_n = 2048;  // fits with n <= batch_max_n run in batched launches
     int batch_max = 256;     // fits per batched launch at most
+    // Requests estimated above heavy_flops run on at most max(1, slots / 2)
+    // slots at once: one of them already fills the device, and the other
+    // slots stay free for short requests.
+    double heavy_flops = 2e11;
 };
 
 struct RequestResult
@@ -113,6 +117,7 @@ class Server
         Clock::time_point submitted, admitted;
         std::size_t need = 0;
         bool rebuild = false;
+        bool heavy = false;
     };
     struct Job
     {
@@ -128,6 +133,7 @@ class Server
     void build_batched(Context &c, Job &job, std::vector<Entry *> &es, std::vector<RequestResult *> &res);
     void finish(Job &job, std::vector<RequestResult> &res);
     void complete(const Request &r, RequestResult res);
+    double cost(const Request &r, const Entry &e) const;
     std::size_t fit_need(const Entry &e) const;
     std::size_t predict_need(const Entry &e, int m) const;
 
@@ -145,6 +151,7 @@ class Server
     unsigned long long clock_ = 0;
     std::size_t resident_ = 0, reserved_ = 0;
     int in_flight_ = 0;
+    int heavy_in_flight_ = 0;
     bool stop_ = false;
     ServerStats st_;
 };

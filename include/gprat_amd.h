@@ -337,6 +337,12 @@ int gprat_context_set_profiling(gprat_context *ctx, int enable);
  * "solve", "trsm", "mean", "refine", "inverse", "gradient", "batch_fit". */
 int gprat_context_kernel_stats(gprat_context *ctx, const char *family, long *launches, double *total_ms);
 
+/* Measured ceilings of this device, to report rates against: FP64 and FP32
+ * matrix-core TFLOP/s (16x16x4 MFMAs from registers on every CU) and the
+ * device-to-device copy bandwidth in GB/s (read + write bytes over a
+ * `copy_bytes` copy; 0 selects 4 GiB). No counterpart in the reference. */
+int gprat_device_calibrate(gprat_context *ctx, size_t copy_bytes, double *fp64_tflops, double *fp32_tflops, double *copy_gbs);
+
 #ifdef __cplusplus
 }
 #endif
