@@ -68,6 +68,20 @@ def trailing_update_flops(n_pad: int):
     return out
 
 
+def measured_traffic():
+    """HBM bytes per launch of the trailing update at this workload, from the
+    committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE per
+    dispatch, each scaled by what the counter reads for a 1 GiB copy);
+    None when that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "trailing_update_n16384_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def percentile(values, q):
     return float(np.percentile(np.asarray(values), q))
 
@@ -280,7 +294,8 @@ def serving_leg(capi, count=192, budget=200 << 30, slots=4):
         lat, bad = [], 0
         for i in range(len(reqs)):
             res = srv.wait(tickets[i])
-            lat.append(time.perf_counter() - submit_t[i])
+            # submission to completion, on the server's own clock
+            lat.append((res["queue_ms"] + res["service_ms"]) * 1e-3)
             bad += res["status"] != 0
         total = time.perf_counter() - t0
         for i, r in enumerate(reqs):
@@ -382,7 +397,7 @@ def main():
         "peak": FP64_MFMA_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": (achieved / FP64_MFMA_PEAK_TFLOPS) if achieved else None,
-        "traffic": None,
+        "traffic": measured_traffic(),
         "launches": launches,
         "launches_expected": expect,
         "avg_launch_ms": syrk_ms / launches if launches else None,

@@ -66,6 +66,7 @@ SIGNATURES = {
     "gprat_model_fit": [_VP, _D, _SZ, _D, _SZ, C.c_int, C.c_int, _D, C.c_int, C.POINTER(_VP)],
     "gprat_model_predict": [_VP, _D, _SZ, C.c_int, _D, _D],
     "gprat_model_loss": [_VP, _D],
+    "gprat_model_gradient": [_VP, _I, _D, _D],
     "gprat_model_bytes": [_VP, C.POINTER(_SZ)],
     "gprat_model_refinement": [_VP, _I, _D],
     "gprat_model_release": [_VP],
@@ -307,6 +308,14 @@ class Model:
         out = C.c_double(0.0)
         _check(lib().gprat_model_loss(self.handle, C.byref(out)))
         return out.value
+
+    def gradient(self, trainable=(1, 1, 1)):
+        """(loss, softplus-space gradient) from the model's own factor."""
+        tr = (C.c_int * 3)(*[int(bool(v)) for v in trainable])
+        loss = C.c_double(0.0)
+        grad = np.zeros(3)
+        _check(lib().gprat_model_gradient(self.handle, tr, C.byref(loss), _dptr(grad)))
+        return loss.value, grad
 
     def refinement(self):
         """(iterations, last relative correction) of the alpha refinement."""

@@ -18,10 +18,12 @@ namespace gpamd::k
 //  * inverse: the four 16x16 diagonal inverses are formed in parallel (one
 //    wavefront each, one column per lane), then the off-diagonal blocks by
 //    distance: W_ij = -W_ii * sum_{t=j}^{i-1} L_it W_tj.
-// On exit D holds L in its lower triangle and W = L^-1 (lower, zero above).
-// A non-positive pivot stores the smallest failing global column in *info.
+// On exit D holds L in its lower triangle and W = L^-1 (lower, zero above),
+// R the reciprocals of L's diagonal (divisions past the pivots are products
+// with them). A non-positive pivot stores the smallest failing global column
+// in *info.
 template <int NB = 64, int THREADS = 256>
-__device__ __forceinline__ void factor_block64(double (*D)[NB + 1], double (*W)[NB + 1], double (*S)[16][17], int *info, int col0)
+__device__ __forceinline__ void factor_block64(double (*D)[NB + 1], double (*W)[NB + 1], double (*S)[16][17], double *R, int *info, int col0)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 #pragma unroll
@@ -42,6 +44,8 @@ __device__ __forceinline__ void factor_block64(double (*D)[NB + 1], double (*W)[
                     atomicMin(info, col0 + o + j);
                 const double ljj = sqrt(djj);
                 const double inv = 1.0 / ljj;
+                if (lane == 0)
+                    R[o + j] = inv;
                 const double lij = (lane > j) ? r[j] * inv : (lane == j ? ljj : 0.0);
                 r[j] = lij;
 #pragma unroll
@@ -72,7 +76,7 @@ __device__ __forceinline__ void factor_block64(double (*D)[NB + 1], double (*W)[
 #pragma unroll
                 for (int q = 0; q < j; q++)
                     sum -= x[q] * D[o + j][o + q];
-                x[j] = sum / D[o + j][o + j];
+                x[j] = sum * R[o + j];
             }
 #pragma unroll
             for (int t = 0; t < 16; t++)
@@ -110,7 +114,7 @@ __device__ __forceinline__ void factor_block64(double (*D)[NB + 1], double (*W)[
 #pragma unroll
                 for (int t = 0; t < i; t++)
                     sum -= D[o + i][o + t] * wc[t];
-                wc[i] = sum / D[o + i][o + i];
+                wc[i] = sum * R[o + i];
             }
 #pragma unroll
             for (int i = 0; i < 16; i++)

@@ -15,15 +15,25 @@ constexpr int NB = 64;
 constexpr int THREADS = 256;
 constexpr int XS = NB + 2;  // LDS row stride of staged blocks (bank-spread MFMA reads)
 
-// Copy a 64x64 row-major block (leading dimension ld) into LDS stride XS.
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+// Copy a 64x64 row-major block (leading dimension ld) into LDS stride XS:
+// all eight 16-byte loads of a thread are issued before the first LDS
+// store, so the block costs one memory latency, not eight.
 __device__ __forceinline__ void stage_block(double *dst, const double *__restrict__ src, long ld)
 {
-    for (int e = threadIdx.x; e < NB * NB / 2; e += THREADS)
+    d2 v[8];
+#pragma unroll
+    for (int s = 0; s < 8; s++)
     {
-        const int r = e >> 5, c = (e & 31) * 2;
-        const double2 v = *reinterpret_cast<const double2 *>(src + static_cast<long>(r) * ld + c);
-        dst[r * XS + c] = v.x;
-        dst[r * XS + c + 1] = v.y;
+        const int e = threadIdx.x + THREADS * s;
+        v[s] = *reinterpret_cast<const d2 *>(src + static_cast<long>(e >> 5) * ld + (e & 31) * 2);
+    }
+#pragma unroll
+    for (int s = 0; s < 8; s++)
+    {
+        const int e = threadIdx.x + THREADS * s;
+        *reinterpret_cast<d2 *>(dst + (e >> 5) * XS + (e & 31) * 2) = v[s];
     }
 }
 
@@ -84,6 +94,23 @@ __device__ __forceinline__ void for_each(const v4d (&acc)[2][2], F f)
 #pragma unroll
             for (int e = 0; e < 4; e++)
                 f(r0 + i * 16 + lk + 4 * e, c0 + j * 16 + lm, acc[i][j][e]);
+}
+
+// The 16 entries of a 64x64 global block (leading dimension ld) at this
+// lane's accumulator positions, in for_each order; all loads are issued
+// before any is used.
+__device__ __forceinline__ void load_at_acc(const double *__restrict__ C, long ld, double (&v)[16])
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lm = lane & 15, lk = lane >> 4;
+    const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                v[(i * 2 + j) * 4 + e] = C[static_cast<long>(r0 + i * 16 + lk + 4 * e) * ld + c0 + j * 16 + lm];
 }
 
 // C (global 64x64 block, ld) = acc or C -= acc, optionally only on or below

@@ -467,6 +467,18 @@ int gprat_model_loss(gprat_model *model, double *loss_out)
         });
 }
 
+int gprat_model_gradient(gprat_model *model, const int trainable[3], double *loss_out, double grad_out[3])
+{
+    return guarded(
+        [&]
+        {
+            gpamd::require(trainable != nullptr && loss_out != nullptr && grad_out != nullptr, "model_gradient: null argument");
+            auto &md = model_ref(model);
+            std::lock_guard<std::recursive_mutex> g(md.ctx->mu);
+            gpamd::model_gradient(md, trainable, loss_out, grad_out);
+        });
+}
+
 int gprat_model_refinement(const gprat_model *model, int *iterations, double *last_correction)
 {
     return guarded(

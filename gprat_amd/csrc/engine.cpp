@@ -369,18 +369,18 @@ static void factor_diagonal_block(Context &c, hipStream_t s, double *A, long lda
 // of the factored diagonal block L[p0:p0+w, p0:p0+w]: the 256-wide diagonal
 // inverses first, then adjacent halves joined bottom-up by
 //   W_bt = -W_b L_bt W_t
-// (two MFMA products through `tmp`, at least w/2 x w/2 doubles). Wb must be
-// zero on entry (its strict upper part is never written).
+// (two MFMA products through `tmp`, at least w/2 x w/2 doubles). Every
+// entry of the w x w inverse is written, its upper part as zeros.
 static void invert_diagonal_block(hipStream_t s, const double *A, long lda, const double *Winv, int p0, int w, double *Wb, long ldw,
                                   double *tmp)
 {
-    k::invert_blocks256(s, A, lda, Winv, p0, w, Wb, ldw, 256 * ldw + 256);
+    k::invert_blocks256(s, A, lda, Winv, p0, w, Wb, ldw, 256 * ldw + 256, true);
     for (int size = 256; size < w; size *= 2)
         for (int t0 = 0; t0 + size < w; t0 += 2 * size)
         {
             const int sb = std::min(size, w - t0 - size);
             // tmp = -L_bt W_t  (sb x size, leading dimension size)
-            k::fill(s, tmp, static_cast<long>(sb) * size, 0.0);
+            k::fill_narrow(s, tmp, static_cast<long>(sb) * size, 0.0);
             k::GemmArgs g{};
             g.A = A + static_cast<long>(p0 + t0 + size) * lda + p0 + t0;
             g.lda = lda;
@@ -501,15 +501,12 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
         Timed t(c, P, "panel");
         factor_diagonal_block(c, P, A, lda, p0, pe, Winv);
         if (pe < n_pad || Wpanel)
-        {
-            k::fill(P, Wblock(p0), static_cast<long>(Bo) * Bo, 0.0);
             invert_diagonal_block(P, A, lda, Winv, p0, pe - p0, Wblock(p0), Bo, tmp.get());
-        }
     };
     auto rows_below = [&](int p0, int pe)
     {
         Timed t(c, T, "panel_rows");
-        solve_panel_rows(T, A, lda, p0, pe - p0, pe, n_pad, Wblock(p0), Bo, cap);
+        solve_panel_rows(T, A, lda, p0, pe - p0, pe, n_pad, Wblock(p0), Bo, 0);
         if (mixed)
             k::panel_to_f32t(T, A, lda, pe, p0, n_pad - pe, pe - p0, p32.get(), n_pad);
         else
@@ -1073,6 +1070,14 @@ void loss_gradient(Context &c,
 {
     Model m;
     fit(c, m, x, x_len, y, y_len, n, r, theta, GPRAT_PRECISION_FP64);
+    model_gradient(m, trainable, loss_out, grad);
+}
+
+void model_gradient(Model &m, const int trainable[3], double *loss_out, double grad[3])
+{
+    Context &c = *m.ctx;
+    c.make_current();
+    const int n = m.n, r = m.r;
     k::loss_terms(c.main, m.L.get(), m.lda, m.y.get(), m.alpha.get(), m.n, c.scalars.get());
     double t[3];
     GP_HIP(hipMemcpyAsync(t, c.scalars.get(), sizeof t, hipMemcpyDefault, c.main));

@@ -194,6 +194,9 @@ void batch_losses(Batch &b, double *losses);
 void batch_predict(Batch &b, int index, const double *z, std::size_t z_len, int mt, double *mean, double *var);
 double loss(Model &m);
 void cholesky_tiles(Model &m, int T, int b, double *out);
+// Loss and softplus-space gradient of a fitted model from its own factor
+// (FP64 or mixed precision).
+void model_gradient(Model &m, const int trainable[3], double *loss_out, double grad[3]);
 void loss_gradient(Context &c,
                    const double *x,
                    std::size_t x_len,

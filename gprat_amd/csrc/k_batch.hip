@@ -41,6 +41,7 @@ struct Factor
     double D[NB][NB + 1];
     double W[NB][NB + 1];
     double S[3][16][17];
+    double R[NB];
 };
 union Shared
 {
@@ -164,17 +165,24 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
         {
             for (int ib = jb; ib < nblk; ib++)
             {
+                double *Cij = L + static_cast<long>(ib) * NB * np + jb * NB;
+                double kij[16];
+                blk::load_at_acc(Cij, np, kij);  // in flight while the products run
                 v4d acc[2][2];
                 accumulate_row_products(L, np, ib, jb, jb, sh.st, acc);
-                double *Cij = L + static_cast<long>(ib) * NB * np + jb * NB;
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++)
+#pragma unroll
+                        for (int e = 0; e < 4; e++)
+                            acc[i][j][e] = kij[(i * 2 + j) * 4 + e] - acc[i][j][e];
                 if (ib == jb)
                 {
                     // D = K_jj - acc (lower), factor + inverse in LDS
-                    blk::for_each(acc,
-                                  [&](int r, int c, double v)
-                                  { sh.fa.D[r][c] = (c <= r) ? Cij[static_cast<long>(r) * np + c] - v : 0.0; });
+                    blk::for_each(acc, [&](int r, int c, double v) { sh.fa.D[r][c] = (c <= r) ? v : 0.0; });
                     __syncthreads();
-                    factor_block64(sh.fa.D, sh.fa.W, sh.fa.S, info + gi, jb * NB);
+                    factor_block64(sh.fa.D, sh.fa.W, sh.fa.S, sh.fa.R, info + gi, jb * NB);
                     double *Wjj = Wg + static_cast<long>(jb) * NB * NB;
                     for (int e = tid; e < NB * NB; e += THREADS)
                     {
@@ -190,7 +198,7 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
                 else
                 {
                     // L_ij = (K_ij - acc) W_jj^T
-                    blk::for_each(acc, [&](int r, int c, double v) { sh.st.a[r * XS + c] = Cij[static_cast<long>(r) * np + c] - v; });
+                    blk::for_each(acc, [&](int r, int c, double v) { sh.st.a[r * XS + c] = v; });
                     __syncthreads();
                     v4d xb[2][2];
                     blk::mfma_nt(sh.st.a, wjj, xb);
@@ -213,10 +221,17 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
             const long base = static_cast<long>(J) * NB;
             {
                 // x_J = W_JJ alpha_J: four partial sums per row, fixed order
-                double s = 0.0;
-#pragma unroll 4
+                double w[16], a[16];
+#pragma unroll
                 for (int k = 0; k < 16; k++)
-                    s += W[row * NB + quarter * 16 + k] * alpha[base + quarter * 16 + k];
+                {
+                    w[k] = W[row * NB + quarter * 16 + k];
+                    a[k] = alpha[base + quarter * 16 + k];
+                }
+                double s = 0.0;
+#pragma unroll
+                for (int k = 0; k < 16; k++)
+                    s += w[k] * a[k];
                 part[quarter * 64 + row] = s;
             }
             __syncthreads();
@@ -230,8 +245,17 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
             {
                 const double *lr = L + i * np + base;
                 double s = 0.0;
-                for (int k = 0; k < NB; k++)
-                    s += lr[k] * xs[k];
+                // 16 values of the row in flight at a time
+                for (int k0 = 0; k0 < NB; k0 += 16)
+                {
+                    d2 v[8];
+#pragma unroll
+                    for (int q = 0; q < 8; q++)
+                        v[q] = *reinterpret_cast<const d2 *>(lr + k0 + 2 * q);
+#pragma unroll
+                    for (int q = 0; q < 8; q++)
+                        s += v[q].x * xs[k0 + 2 * q] + v[q].y * xs[k0 + 2 * q + 1];
+                }
                 alpha[i] -= s;
             }
             __syncthreads();
@@ -242,10 +266,17 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
             const long base = static_cast<long>(J) * NB;
             {
                 // x_J = W_JJ^T alpha_J
-                double s = 0.0;
-#pragma unroll 4
+                double w[16], a[16];
+#pragma unroll
                 for (int k = 0; k < 16; k++)
-                    s += W[(quarter * 16 + k) * NB + row] * alpha[base + quarter * 16 + k];
+                {
+                    w[k] = W[(quarter * 16 + k) * NB + row];
+                    a[k] = alpha[base + quarter * 16 + k];
+                }
+                double s = 0.0;
+#pragma unroll
+                for (int k = 0; k < 16; k++)
+                    s += w[k] * a[k];
                 part[quarter * 64 + row] = s;
             }
             __syncthreads();
@@ -258,8 +289,16 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
             for (long jcol = tid; jcol < base; jcol += THREADS)
             {
                 double s = 0.0;
-                for (int k = 0; k < NB; k++)
-                    s += L[(base + k) * np + jcol] * xs[k];
+                for (int k0 = 0; k0 < NB; k0 += 16)
+                {
+                    double v[16];
+#pragma unroll
+                    for (int q = 0; q < 16; q++)
+                        v[q] = L[(base + k0 + q) * np + jcol];
+#pragma unroll
+                    for (int q = 0; q < 16; q++)
+                        s += v[q] * xs[k0 + q];
+                }
                 alpha[jcol] -= s;
             }
             __syncthreads();

@@ -2,7 +2,9 @@
 // device actually reaches, so measured rates can be reported against both
 // the vendor peak and a measured one. The MFMA loops run independent
 // accumulators back to back from registers (no memory traffic); the copy
-// streams 16-byte vectors through every CU.
+// streams 16-byte vectors through every CU. The loops are held to two
+// waves per SIMD: left free, the compiler moves every accumulator between
+// VGPRs and AGPRs on each iteration.
 #include "kernels.hpp"
 
 namespace gpamd::k
@@ -16,7 +18,7 @@ typedef double d2 __attribute__((ext_vector_type(2)));
 
 constexpr int CHAINS = 16;  // independent accumulators per wavefront
 
-__global__ __launch_bounds__(256) void k_mfma_f64_loop(double *out, int iters)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mfma_f64_loop(double *out, int iters)
 {
     v4d acc[CHAINS];
 #pragma unroll
@@ -36,7 +38,7 @@ __global__ __launch_bounds__(256) void k_mfma_f64_loop(double *out, int iters)
     out[static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x] = s;
 }
 
-__global__ __launch_bounds__(256) void k_mfma_f32_loop(double *out, int iters)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mfma_f32_loop(double *out, int iters)
 {
     v4f acc[CHAINS];
 #pragma unroll

@@ -14,13 +14,22 @@ namespace
 constexpr int NB = 64;        // diagonal block edge
 constexpr int THREADS = 256;
 
-// Lower triangle of the NB x NB block at (r0, r0) into LDS (strict upper zeroed).
+// Lower triangle of the NB x NB block at (r0, r0) into LDS (strict upper
+// zeroed); every load of a thread is issued before the first LDS store.
 __device__ __forceinline__ void load_lower_block(const double *__restrict__ A, long lda, long r0, double (*D)[NB + 1])
 {
-    for (int e = threadIdx.x; e < NB * NB; e += THREADS)
+    double v[NB * NB / THREADS];
+#pragma unroll
+    for (int s = 0; s < NB * NB / THREADS; s++)
     {
-        const int r = e / NB, c = e % NB;
-        D[r][c] = (c <= r) ? A[(r0 + r) * lda + r0 + c] : 0.0;
+        const int e = threadIdx.x + THREADS * s, r = e / NB, c = e % NB;
+        v[s] = (c <= r) ? A[(r0 + r) * lda + r0 + c] : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < NB * NB / THREADS; s++)
+    {
+        const int e = threadIdx.x + THREADS * s;
+        D[e / NB][e % NB] = v[s];
     }
 }
 
@@ -31,15 +40,12 @@ __global__ __launch_bounds__(THREADS) void k_potrf64_inv(double *__restrict__ A,
     __shared__ double D[NB][NB + 1];
     __shared__ double Wl[NB][NB + 1];
     __shared__ double S[3][16][17];
+    __shared__ double R[NB];
     const int tid = threadIdx.x;
     double *blk = A + static_cast<long>(c0) * lda + c0;
-    for (int e = tid; e < NB * NB; e += THREADS)
-    {
-        const int r = e / NB, c = e % NB;
-        D[r][c] = (c <= r) ? blk[static_cast<long>(r) * lda + c] : 0.0;
-    }
+    load_lower_block(A, lda, c0, D);
     __syncthreads();
-    factor_block64(D, Wl, S, info, c0);
+    factor_block64(D, Wl, S, R, info, c0);
     for (int e = tid; e < NB * NB; e += THREADS)
     {
         const int r = e / NB, c = e % NB;
@@ -260,7 +266,8 @@ __global__ void k_set_int(int *p, int value) { *p = value; }
 // zero above its diagonal and past its last row; a 64-block of W read as
 // B^T rows is then a plain strided 64x64 read.
 __global__ __launch_bounds__(THREADS) void k_invert_blocks256(const double *__restrict__ L, long lda, const double *__restrict__ Winv,
-                                                               long row0, long rows, double *Wt, long ldw, long block_stride)
+                                                               long row0, long rows, double *Wt, long ldw, long block_stride,
+                                                               int zero_above)
 {
     __shared__ double a[blk::NB * blk::XS];
     __shared__ double b[blk::NB * blk::XS];
@@ -269,14 +276,36 @@ __global__ __launch_bounds__(THREADS) void k_invert_blocks256(const double *__re
     const int q = static_cast<int>(min(4L, (row0 + rows - j0) / NB));
     const long jb = j0 / NB;  // first 64-block of this 256-block
     double *out = Wt + static_cast<long>(blockIdx.x) * block_stride;
+    // zero the off-diagonal 64-blocks (stores only) ...
     for (int e = tid; e < 256 * 256; e += THREADS)
     {
         const int c = e >> 8, r = e & 255;
-        const int bi = r >> 6, bj = c >> 6;
-        double v = 0.0;
-        if (bi == bj && bi < q)
-            v = Winv[(jb + bi) * NB * NB + (r & 63) * NB + (c & 63)];
-        out[c * ldw + r] = v;
+        if ((r >> 6) != (c >> 6) || (r >> 6) >= q)
+            out[c * ldw + r] = 0.0;
+    }
+    // ... and place the 64-wide inverses, transposed into the k-major
+    // layout through LDS, with every load of a block issued up front
+    for (int bi = 0; bi < q; bi++)
+    {
+        blk::stage_block(a, Winv + (jb + bi) * NB * NB, NB);
+        __syncthreads();
+        for (int e = tid; e < NB * NB; e += THREADS)
+        {
+            const int c = e >> 6, r = e & 63;
+            out[static_cast<long>(NB * bi + c) * ldw + NB * bi + r] = a[r * blk::XS + c];
+        }
+        __syncthreads();
+    }
+    if (zero_above)
+    {
+        // one inverse of the whole range (blocks along its diagonal): the
+        // rows above this block, in its columns, are the inverse's zero part
+        const long above = j0 - row0;
+        for (long e = tid; e < above * 256; e += THREADS)
+        {
+            const long c = e / above, r = e % above;
+            out[c * ldw + r - above] = 0.0;
+        }
     }
     __syncthreads();
     for (int d = 1; d < q; d++)
@@ -325,6 +354,12 @@ __global__ __launch_bounds__(THREADS) void k_colsq_accumulate(const T *__restric
         colsq[j] += (part[0][c] + part[1][c]) + (part[2][c] + part[3][c]);
 }
 
+__global__ void k_fill_strided(double *__restrict__ p, long count, double value)
+{
+    for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += static_cast<long>(gridDim.x) * blockDim.x)
+        p[i] = value;
+}
+
 __global__ void k_to_f32(const double *__restrict__ src, float *__restrict__ dst, long count)
 {
     const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -335,14 +370,15 @@ __global__ void k_to_f32(const double *__restrict__ src, float *__restrict__ dst
 }  // namespace
 
 void invert_blocks256(hipStream_t s, const double *L, long lda, const double *Winv, long row0, long rows, double *Wt, long ldw,
-                      long block_stride)
+                      long block_stride, bool zero_above)
 {
     require(row0 % NB == 0 && rows % NB == 0 && rows > 0 && lda >= row0 + rows && lda % 2 == 0 && ldw % 2 == 0 && ldw >= 256,
             "invert_blocks256: bad shape");
     require(reinterpret_cast<uintptr_t>(L) % 16 == 0 && reinterpret_cast<uintptr_t>(Wt) % 16 == 0,
             "invert_blocks256: operands must be 16-byte aligned");
+    require(!zero_above || block_stride == 256 * ldw + 256, "invert_blocks256: zero_above needs the diagonal layout");
     hipLaunchKernelGGL(k_invert_blocks256, dim3(static_cast<unsigned>(ceil_div(rows, 256L))), dim3(THREADS), 0, s, L, lda, Winv, row0,
-                       rows, Wt, ldw, block_stride);
+                       rows, Wt, ldw, block_stride, zero_above ? 1 : 0);
     GP_CHECK_LAUNCH();
 }
 
@@ -454,6 +490,16 @@ void add_vec(hipStream_t s, double *y, const double *x, long n)
     if (n <= 0)
         return;
     hipLaunchKernelGGL(k_add_vec, dim3(ceil_div(n, 256)), dim3(256), 0, s, y, x, n);
+    GP_CHECK_LAUNCH();
+}
+
+void fill_narrow(hipStream_t s, double *p, long count, double value)
+{
+    require(count >= 0, "fill_narrow: negative count");
+    if (count == 0)
+        return;
+    hipLaunchKernelGGL(k_fill_strided, dim3(static_cast<unsigned>(std::min<long>(ceil_div(count, 256L), 8))), dim3(THREADS), 0, s, p, count,
+                       value);
     GP_CHECK_LAUNCH();
 }
 

@@ -182,8 +182,11 @@ void solve_upper_step(hipStream_t s, const double *L, long lda, const double *Wi
 // 64-blocks. Block b is stored k-major at Wt + b * block_stride with leading
 // dimension ldw (W(r, c) at [c * ldw + r]), zero above the diagonal and past
 // the block's last row within its 256 x 256 square.
+// zero_above (only with block_stride = 256 * ldw + 256, i.e. one inverse of
+// the whole range stored with its blocks along the diagonal): each block
+// also zeroes the rows above it in its columns, the inverse's upper part.
 void invert_blocks256(hipStream_t s, const double *L, long lda, const double *Winv, long row0, long rows, double *Wt, long ldw,
-                      long block_stride);
+                      long block_stride, bool zero_above = false);
 
 // colsq[j] += sum_{i<rows} V[i][j]^2 for j < m (fixed summation order;
 // FP64 sums for FP32 V too).
@@ -211,6 +214,9 @@ void add_vec(hipStream_t s, double *y, const double *x, long n);
 
 // Fill a device vector.
 void fill(hipStream_t s, double *p, long count, double value);
+// The same with at most 8 workgroups, for streams that run beside a
+// saturating kernel (a wide launch would wait for free slots).
+void fill_narrow(hipStream_t s, double *p, long count, double value);
 // A[i][i] = value for i < n.
 void set_diagonal(hipStream_t s, double *A, long lda, int n, double value);
 // Store one int (used to reset the pivot-failure flag without a memset node).

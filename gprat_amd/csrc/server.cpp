@@ -308,7 +308,7 @@ bool Server::pick(Job &job)
             need = fit_need(e);
         }
         const bool heavy = cost(r, e) > cfg_.heavy_flops && !(r.kind == 0 && e.batched_path);
-        if (heavy && heavy_in_flight_ >= std::max(1, cfg_.slots / 2))
+        if (heavy && heavy_in_flight_ >= std::max(1, cfg_.slots - 1))
         {
             ++it;  // keeps its place; short requests behind it may start
             continue;

@@ -35,9 +35,8 @@ struct ServerConfig
     int precision = GPRAT_PRECISION_MIXED;
     int batch_max_n = 2048;  // fits with n <= batch_max_n run in batched launches
     int batch_max = 256;     // fits per batched launch at most
-    // Requests estimated above heavy_flops run on at most max(1, slots / 2)
-    // slots at once: one of them already fills the device, and the other
-    // slots stay free for short requests.
+    // Requests estimated above heavy_flops run on at most max(1, slots - 1)
+    // slots at once, so one slot always stays free for short requests.
     double heavy_flops = 2e11;
 };
 

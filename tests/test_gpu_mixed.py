@@ -46,3 +46,25 @@ def test_mixed_vs_fp64(ctx, n, r, m, noise):
     finally:
         f64.release()
         mix.release()
+
+
+def test_mixed_gradient_vs_fp64(ctx):
+    """Gradient from a mixed-precision factor (K^-1 = W^T W of the reduced-
+    accuracy factor, refined alpha) against the FP64 model's."""
+    from gprat_amd import capi, datagen
+    n, r, theta = 8192, 8, (1.0, 1.0, 0.1)
+    x, y, _ = datagen.gp_problem(n, 1, r)
+    f64 = capi.Model(ctx, x, y, n, r, theta, capi.PRECISION_FP64)
+    mix = capi.Model(ctx, x, y, n, r, theta, capi.PRECISION_MIXED)
+    try:
+        l64, g64 = f64.gradient()
+        lmx, gmx = mix.gradient()
+        ref_loss, ref_grad = ctx.loss_gradient(x, y, theta, 8, 1024, r)
+        assert abs(l64 - ref_loss) <= 1e-12 * abs(ref_loss)
+        assert np.max(np.abs(g64 - ref_grad)) <= 1e-12 * np.max(np.abs(ref_grad))
+        err = np.max(np.abs(gmx - g64)) / np.max(np.abs(g64))
+        print(f"mixed gradient rel err {err:.2e}: {gmx} vs {g64}")
+        assert err < 5e-3  # K^-1 carries the FP32 factor's error (no refinement for a full inverse)
+    finally:
+        f64.release()
+        mix.release()
