@@ -559,19 +559,48 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
                         + std::to_string(info) + ")");
 }
 
-// alpha = K^-1 y by the blocked forward and backward solves.
-void solve_spd(Context &c, const double *L, long lda, const double *Winv, int n_pad, const double *y, double *work, double *out)
+// alpha = K^-1 y by the blocked forward and backward solves. With the
+// factorisation's wide diagonal-block inverses (Wpanel, k-major, wblock
+// wide) each direction takes n_pad / wblock steps of two matrix-vector
+// kernels (the diagonal block's inverse, then the coupling to the rest);
+// otherwise 64-wide steps of one kernel each against Winv.
+void solve_spd(Context &c, const double *L, long lda, const double *Winv, int n_pad, const double *y, double *work, double *out,
+               const double *Wpanel, int wblock)
 {
     hipStream_t s = c.main;
     Timed t(c, s, "solve");
-    const int nb = n_pad / 64;
     // work <- y (rhs of the forward solve, updated in place)
     GP_HIP(hipMemcpyAsync(work, y, sizeof(double) * n_pad, hipMemcpyDeviceToDevice, s));
-    for (int j = 0; j < nb; j++)
-        k::solve_lower_step(s, L, lda, Winv, work, out, n_pad, j);
-    // out holds L^-1 y; use it as the rhs of the backward solve, result in work
-    for (int j = nb - 1; j >= 0; j--)
-        k::solve_upper_step(s, L, lda, Winv, out, work, n_pad, j);
+    if (Wpanel)
+    {
+        require(wblock % 64 == 0 && wblock > 0, "solve_spd: bad block width");
+        const int nblk = static_cast<int>(ceil_div(n_pad, wblock));
+        auto WJ = [&](int J) { return Wpanel + static_cast<long>(J) * wblock * wblock; };
+        for (int J = 0; J < nblk; J++)
+        {
+            const int j0 = J * wblock, nb = std::min(wblock, n_pad - j0), j1 = j0 + nb;
+            k::coldot(s, WJ(J), wblock, nb, nb, work + j0, out + j0, false);
+            if (j1 < n_pad)
+                k::rowdot(s, L + static_cast<long>(j1) * lda + j0, lda, nb, n_pad - j1, out + j0, work + j1, true);
+        }
+        // out holds z = L^-1 y, updated in place; x = L^-T z goes to work
+        for (int J = nblk - 1; J >= 0; J--)
+        {
+            const int j0 = J * wblock, nb = std::min(wblock, n_pad - j0);
+            k::rowdot(s, WJ(J), wblock, nb, nb, out + j0, work + j0, false);
+            if (j0 > 0)
+                k::coldot(s, L + static_cast<long>(j0) * lda, lda, nb, j0, work + j0, out, true);
+        }
+    }
+    else
+    {
+        const int nb = n_pad / 64;
+        for (int j = 0; j < nb; j++)
+            k::solve_lower_step(s, L, lda, Winv, work, out, n_pad, j);
+        // out holds L^-1 y; use it as the rhs of the backward solve, result in work
+        for (int j = nb - 1; j >= 0; j--)
+            k::solve_upper_step(s, L, lda, Winv, out, work, n_pad, j);
+    }
     GP_HIP(hipMemcpyAsync(out, work, sizeof(double) * n_pad, hipMemcpyDeviceToDevice, s));
 }
 
@@ -770,8 +799,9 @@ void factor(Context &c, Model &m, const double *x, std::size_t x_len, int n, int
 
 // FP64 iterative refinement of alpha against a factor of reduced accuracy:
 // r = y - K alpha with K regenerated from the series in FP64, delta from the
-// factor, alpha += delta, until the correction stops mattering
-// (max|delta| <= 1e-15 max|alpha|) or stops shrinking.
+// factor, alpha += delta, until the correction is at most 1e-12 max|alpha|
+// (the error left after it is smaller still, since every step contracts it)
+// or stops shrinking.
 void refine_alpha(Model &m, double *work)
 {
     Context &c = *m.ctx;
@@ -791,7 +821,7 @@ void refine_alpha(Model &m, double *work)
             k::predict_mean(s, m.x.get(), m.x.get(), m.alpha.get(), m.r, m.sek, n, n, partial.get(), splits, kx.get());
             k::residual_finish(s, m.y.get(), kx.get(), m.alpha.get(), m.sek.noise, n, r.get());
         }
-        solve_spd(c, m.L.get(), m.lda, m.Winv.get(), m.n_pad, r.get(), work, delta.get());
+        solve_spd(c, m.L.get(), m.lda, m.Winv.get(), m.n_pad, r.get(), work, delta.get(), m.Wpanel.get(), m.wblock);
         k::add_vec(s, m.alpha.get(), delta.get(), n);
         GP_HIP(hipMemcpyAsync(hd.data(), delta.get(), sizeof(double) * n, hipMemcpyDefault, s));
         GP_HIP(hipMemcpyAsync(ha.data(), m.alpha.get(), sizeof(double) * n, hipMemcpyDefault, s));
@@ -804,7 +834,7 @@ void refine_alpha(Model &m, double *work)
         }
         m.refine_iterations = it + 1;
         m.refine_last = amax > 0.0 ? dmax / amax : 0.0;
-        if (!(dmax <= 1e-15 * amax) && dmax < 0.5 * prev)
+        if (!(dmax <= 1e-12 * amax) && dmax < 0.5 * prev)
         {
             prev = dmax;
             continue;
@@ -823,7 +853,7 @@ void attach_targets(Model &m, const double *y, std::size_t y_len)
     DevBuf<double> work(c, m.n_pad);
     k::fill(c.main, m.y.get(), m.n_pad, 0.0);
     upload(c, m.y.get(), y, static_cast<std::size_t>(m.n));
-    solve_spd(c, m.L.get(), m.lda, m.Winv.get(), m.n_pad, m.y.get(), work.get(), m.alpha.get());
+    solve_spd(c, m.L.get(), m.lda, m.Winv.get(), m.n_pad, m.y.get(), work.get(), m.alpha.get(), m.Wpanel.get(), m.wblock);
     if (m.precision == GPRAT_PRECISION_MIXED)
         refine_alpha(m, work.get());
     c.sync();

@@ -169,7 +169,8 @@ void cholesky_fp64(Context &c, double *A, long lda, int n_pad, double *Winv);
 void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mixed, double *Wpanel = nullptr);
 int panel_width(int n_pad);
 void refine_alpha(Model &m, double *work);
-void solve_spd(Context &c, const double *L, long lda, const double *Winv, int n_pad, const double *y, double *work, double *out);
+void solve_spd(Context &c, const double *L, long lda, const double *Winv, int n_pad, const double *y, double *work, double *out,
+               const double *Wpanel = nullptr, int wblock = 0);
 void trsm_lower_left(Context &c, const double *L, long lda, const double *Winv, double *B, long ldb, int ncols, int a, int b, bool tri_rhs);
 void trsm_blocked(Context &c, const double *L, long lda, const double *Winv, const double *Wpanel, int wblock, int n_pad, double *B,
                   long ldb, int ncols, bool tri_rhs, double *out, long ldo, bool staged, double *colsq, int mt);

@@ -40,7 +40,6 @@ struct Factor
 {
     double D[NB][NB + 1];
     double W[NB][NB + 1];
-    double S[3][16][17];
     double R[NB];
 };
 union Shared
@@ -182,7 +181,7 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
                     // D = K_jj - acc (lower), factor + inverse in LDS
                     blk::for_each(acc, [&](int r, int c, double v) { sh.fa.D[r][c] = (c <= r) ? v : 0.0; });
                     __syncthreads();
-                    factor_block64(sh.fa.D, sh.fa.W, sh.fa.S, sh.fa.R, info + gi, jb * NB);
+                    factor_block64(sh.fa.D, sh.fa.W, sh.fa.R, info + gi, jb * NB);
                     double *Wjj = Wg + static_cast<long>(jb) * NB * NB;
                     for (int e = tid; e < NB * NB; e += THREADS)
                     {

@@ -106,6 +106,88 @@ __global__ __launch_bounds__(THREADS) void k_mean_partial(const double *__restri
     }
 }
 
+// Window lengths up to 16 known at compile time: the lane's test window is
+// held in registers for the whole chunk, the train index is wavefront-
+// uniform (so the train window and alpha arrive through scalar loads and
+// feed the FP64 VALU as scalar operands), and each wavefront walks its own
+// contiguous quarter of the chunk, two train points per step. The four
+// wavefront partials are combined in a fixed order, as above.
+template <int R>
+__global__ __launch_bounds__(THREADS) void k_mean_partial_w(const double *__restrict__ x,
+                                                             const double *__restrict__ z,
+                                                             const double *__restrict__ alpha,
+                                                             double scale,
+                                                             double v,
+                                                             int n,
+                                                             int m,
+                                                             int chunk,
+                                                             double *__restrict__ partial)
+{
+    __shared__ double red[THREADS];
+    const int lane = threadIdx.x % TB;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / TB);
+    const int j = blockIdx.x * TB + lane;
+    const int jc = j < m ? j : m - 1;  // lanes past the end read a valid window and discard the sum
+    double zw[R];
+#pragma unroll
+    for (int k = 0; k < R; k++)
+        zw[k] = z[jc + k];
+    const int c0 = blockIdx.y * chunk;
+    const int c1 = min(n, c0 + chunk);
+    const int share = (chunk + 3) / 4;
+    const int i0 = min(c1, c0 + w * share);
+    const int i1 = min(c1, i0 + share);
+    double acc0 = 0.0, acc1 = 0.0;
+    int i = i0;
+    for (; i + 1 < i1; i += 2)
+    {
+        double d0 = 0.0, d1 = 0.0;
+#pragma unroll
+        for (int k = 0; k < R; k++)
+        {
+            const double t0 = zw[k] - x[i + k];
+            const double t1 = zw[k] - x[i + 1 + k];
+            d0 = fma(t0, t0, d0);
+            d1 = fma(t1, t1, d1);
+        }
+        acc0 = fma(se(scale, v, d0), alpha[i], acc0);
+        acc1 = fma(se(scale, v, d1), alpha[i + 1], acc1);
+    }
+    if (i < i1)
+    {
+        double d0 = 0.0;
+#pragma unroll
+        for (int k = 0; k < R; k++)
+        {
+            const double t0 = zw[k] - x[i + k];
+            d0 = fma(t0, t0, d0);
+        }
+        acc0 = fma(se(scale, v, d0), alpha[i], acc0);
+    }
+    red[threadIdx.x] = acc0 + acc1;
+    __syncthreads();
+    if (w == 0 && j < m)
+    {
+        double s = red[lane];
+        for (int q = 1; q < THREADS / TB; q++)
+            s += red[q * TB + lane];
+        partial[static_cast<long>(blockIdx.y) * m + j] = s;
+    }
+}
+
+template <int R>
+void launch_mean_w(hipStream_t s, dim3 grid, const double *x, const double *z, const double *alpha, double scale, double v, int n, int m,
+                   int chunk, double *partial)
+{
+    hipLaunchKernelGGL(k_mean_partial_w<R>, grid, dim3(THREADS), 0, s, x, z, alpha, scale, v, n, m, chunk, partial);
+}
+
+using MeanLaunch = void (*)(hipStream_t, dim3, const double *, const double *, const double *, double, double, int, int, int, double *);
+constexpr MeanLaunch mean_w_table[16] = { launch_mean_w<1>,  launch_mean_w<2>,  launch_mean_w<3>,  launch_mean_w<4>,
+                                          launch_mean_w<5>,  launch_mean_w<6>,  launch_mean_w<7>,  launch_mean_w<8>,
+                                          launch_mean_w<9>,  launch_mean_w<10>, launch_mean_w<11>, launch_mean_w<12>,
+                                          launch_mean_w<13>, launch_mean_w<14>, launch_mean_w<15>, launch_mean_w<16> };
+
 __global__ __launch_bounds__(THREADS) void k_mean_reduce(const double *__restrict__ partial, int splits, int m, double *__restrict__ mean)
 {
     const long j = static_cast<long>(blockIdx.x) * THREADS + threadIdx.x;
@@ -306,10 +388,14 @@ int mean_splits(int n, int m)
 
 void predict_mean(hipStream_t s, const double *x, const double *z, const double *alpha, int r, SEK sek, int n, int m, double *partial, int splits, double *mean)
 {
-    require(n >= 1 && m >= 1 && splits >= 1, "predict_mean: bad shape");
+    require(n >= 1 && m >= 1 && splits >= 1 && r >= 1, "predict_mean: bad shape");
     const int chunk = static_cast<int>(ceil_div(n, splits));
+    require(n + r < (1 << 30) && m + r < (1 << 30), "predict_mean: sizes beyond 32-bit indexing");
     dim3 grid(ceil_div(m, TB), splits);
-    hipLaunchKernelGGL(k_mean_partial, grid, dim3(THREADS), 0, s, x, z, alpha, r, sek.scale(), sek.vertical, n, m, chunk, partial);
+    if (r <= 16)
+        mean_w_table[r - 1](s, grid, x, z, alpha, sek.scale(), sek.vertical, n, m, chunk, partial);
+    else
+        hipLaunchKernelGGL(k_mean_partial, grid, dim3(THREADS), 0, s, x, z, alpha, r, sek.scale(), sek.vertical, n, m, chunk, partial);
     GP_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_mean_reduce, dim3(ceil_div(m, THREADS)), dim3(THREADS), 0, s, partial, splits, m, mean);
     GP_CHECK_LAUNCH();

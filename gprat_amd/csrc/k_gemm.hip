@@ -129,6 +129,47 @@ __device__ __forceinline__ void mfma_slab(const double *As, const double *Bs, in
 }
 
 // One 128x128 tile of C at (r0, c0).
+// C (op)= acc over one wavefront's 64x64 quarter, element (i, j, e) at
+// cw + (16 i + 4 e) ldc + 16 j. C is read and written through the same
+// pointer, so the compiler keeps every element's load, wait and store in
+// program order; the loads of a 16-row strip are therefore all issued before
+// its first store, and the strip pays one memory latency instead of sixteen.
+// FULL: an interior tile (no edge, wholly inside the lower band), no
+// predicates.
+template <bool FULL>
+__device__ __forceinline__ void store_quarter(const Params &p, const v4d (&acc)[4][4], double *cw, int rb, int cb, int rows_a, int rows_b,
+                                              long r0, long c0)
+{
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+    {
+        bool ok[16];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+            {
+                const int rr = rb + i * 16 + 4 * e, cc = cb + j * 16;
+                ok[j * 4 + e] = FULL || ((rr < rows_a) & (cc < rows_b) & !(p.lower && r0 + rr + p.diag_off < c0 + cc));
+            }
+        double cv[16];
+        if (!p.set)
+        {
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    cv[j * 4 + e] = ok[j * 4 + e] ? cw[static_cast<long>(i * 16 + 4 * e) * p.ldc + j * 16] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                if (ok[j * 4 + e])
+                    cw[static_cast<long>(i * 16 + 4 * e) * p.ldc + j * 16] = p.set ? acc[i][j][e] : cv[j * 4 + e] - acc[i][j][e];
+    }
+}
+
 template <bool AT, bool BT>
 __device__ __forceinline__ void gemm_tile(const Params &p, long r0, long c0, double (*As)[BK * LDS_STRIDE], double (*Bs)[BK * LDS_STRIDE])
 {
@@ -199,26 +240,18 @@ __device__ __forceinline__ void gemm_tile(const Params &p, long r0, long c0, dou
 
     // Accumulator layout of the 16x16x4 f64 MFMA (measured on gfx950): lane
     // holds column (lane % 16) and rows (lane / 16) + 4 e, e = 0..3.
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int e = 0; e < 4; e++)
-            {
-                const int rr = wr * 64 + i * 16 + lk + 4 * e;
-                const int cc = wc * 64 + j * 16 + lm;
-                if (rr >= rows_a || cc >= rows_b)
-                    continue;
-                const long gi = r0 + rr, gj = c0 + cc;
-                if (p.lower && gi + p.diag_off < gj)
-                    continue;
-                double *dst = p.C + gi * p.ldc + gj;
-                if (p.set)
-                    *dst = acc[i][j][e];
-                else
-                    *dst -= acc[i][j][e];
-            }
+    // C is read and written through the same pointer, so the compiler keeps
+    // every element's load, wait and store in program order; the loads of a
+    // 16-row strip are therefore all issued before its first store, and the
+    // strip pays one memory latency instead of sixteen. Interior tiles (no
+    // edge, wholly on or below the diagonal band) skip the predicates.
+    const bool full = rows_a == BM && rows_b == BN && (!p.lower || r0 + p.diag_off >= c0 + BN - 1);
+    const int rb = wr * 64 + lk, cb = wc * 64 + lm;
+    double *cw = p.C + (r0 + rb) * p.ldc + c0 + cb;
+    if (full)
+        store_quarter<true>(p, acc, cw, rb, cb, rows_a, rows_b, r0, c0);
+    else
+        store_quarter<false>(p, acc, cw, rb, cb, rows_a, rows_b, r0, c0);
 }
 
 // One workgroup per tile (grid x = column tiles, y = row tiles).

@@ -80,6 +80,47 @@ __device__ __forceinline__ void mfma_slab32(const float *As, const float *Bs, in
 }
 
 // One 128x128 tile of C at (r0, c0).
+// C (op)= acc over one wavefront's 64x64 quarter, element (i, j, e) at
+// cw + (16 i + e) ldc + 16 j. C is read and written through the same
+// pointer, so the compiler keeps every element's load, wait and store in
+// program order; the loads of a 16-row strip are therefore all issued before
+// its first store (one memory latency per strip, not sixteen). FULL: an
+// interior tile (no edge, wholly inside the lower band), no predicates.
+template <bool FULL, class TC>
+__device__ __forceinline__ void store_quarter32(const Params32<TC> &p, const v4f (&acc)[4][4], TC *cw, int rb, int cb, int rows_a, int rows_b,
+                                                long r0, long c0)
+{
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+    {
+        bool ok[16];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+            {
+                const int rr = rb + i * 16 + e, cc = cb + j * 16;
+                ok[j * 4 + e] = FULL || ((rr < rows_a) & (cc < rows_b) & !(p.lower && r0 + rr + p.diag_off < c0 + cc));
+            }
+        TC cv[16];
+        if (!p.set)
+        {
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    cv[j * 4 + e] = ok[j * 4 + e] ? cw[static_cast<long>(i * 16 + e) * p.ldc + j * 16] : TC(0);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                if (ok[j * 4 + e])
+                    cw[static_cast<long>(i * 16 + e) * p.ldc + j * 16]
+                        = p.set ? static_cast<TC>(acc[i][j][e]) : cv[j * 4 + e] - static_cast<TC>(acc[i][j][e]);
+    }
+}
+
 template <class TC>
 __device__ __forceinline__ void gemm_tile32(const Params32<TC> &p, long r0, long c0, float (*As)[BK * KS], float (*Bs)[BK * KS])
 {
@@ -150,26 +191,13 @@ __device__ __forceinline__ void gemm_tile32(const Params32<TC> &p, long r0, long
 
     // Accumulator layout of the 16x16x4 f32 MFMA on gfx950: lane holds
     // column (lane % 16) and rows 4 (lane / 16) + e, e = 0..3.
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int e = 0; e < 4; e++)
-            {
-                const int rr = wr * 64 + i * 16 + lk * 4 + e;
-                const int cc = wc * 64 + j * 16 + lm;
-                if (rr >= rows_a || cc >= rows_b)
-                    continue;
-                const long gi = r0 + rr, gj = c0 + cc;
-                if (p.lower && gi + p.diag_off < gj)
-                    continue;
-                TC *dst = p.C + gi * p.ldc + gj;
-                if (p.set)
-                    *dst = static_cast<TC>(acc[i][j][e]);
-                else
-                    *dst -= static_cast<TC>(acc[i][j][e]);
-            }
+    const bool full = rows_a == BM && rows_b == BM && (!p.lower || r0 + p.diag_off >= c0 + BM - 1);
+    const int er = wr * 64 + lk * 4, ec = wc * 64 + lm;
+    TC *cw = p.C + (r0 + er) * p.ldc + c0 + ec;
+    if (full)
+        store_quarter32<true>(p, acc, cw, er, ec, rows_a, rows_b, r0, c0);
+    else
+        store_quarter32<false>(p, acc, cw, er, ec, rows_a, rows_b, r0, c0);
 }
 
 // ROLE only separates profiler names: 0 trailing update (FP64 C), 1 TRSM

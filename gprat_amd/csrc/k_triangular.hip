@@ -39,13 +39,12 @@ __global__ __launch_bounds__(THREADS) void k_potrf64_inv(double *__restrict__ A,
 {
     __shared__ double D[NB][NB + 1];
     __shared__ double Wl[NB][NB + 1];
-    __shared__ double S[3][16][17];
     __shared__ double R[NB];
     const int tid = threadIdx.x;
     double *blk = A + static_cast<long>(c0) * lda + c0;
     load_lower_block(A, lda, c0, D);
     __syncthreads();
-    factor_block64(D, Wl, S, R, info, c0);
+    factor_block64(D, Wl, R, info, c0);
     for (int e = tid; e < NB * NB; e += THREADS)
     {
         const int r = e / NB, c = e % NB;
@@ -143,6 +142,99 @@ __global__ __launch_bounds__(THREADS) void k_solve_upper_step(const double *__re
     __syncthreads();
     if (tid < NB)
         rhs[I * NB + tid] -= (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
+}
+
+// out[c] = (or -=) sum_{r < K} M[r * ld + c] v[r] for c < N. Lane = column
+// (each row read is a coalesced 512-byte segment), the four wavefronts take
+// contiguous quarters of the rows with 16 loads in flight per lane, v comes
+// in through scalar loads, and the quarters are combined in a fixed order.
+__global__ __launch_bounds__(THREADS) void k_coldot(const double *__restrict__ M,
+                                                     long ld,
+                                                     int K,
+                                                     int N,
+                                                     const double *__restrict__ v,
+                                                     double *__restrict__ out,
+                                                     bool sub)
+{
+    __shared__ double part[4][64];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c = blockIdx.x * 64 + lane;
+    const int q = (K + 3) / 4;
+    const int r0 = min(K, w * q), r1 = min(K, r0 + q);
+    double s = 0.0;
+    if (c < N)
+    {
+        int r = r0;
+        for (; r + 16 <= r1; r += 16)
+        {
+            double m[16];
+#pragma unroll
+            for (int t = 0; t < 16; t++)
+                m[t] = M[static_cast<long>(r + t) * ld + c];
+#pragma unroll
+            for (int t = 0; t < 16; t++)
+                s = fma(m[t], v[r + t], s);
+        }
+        for (; r < r1; r++)
+            s = fma(M[static_cast<long>(r) * ld + c], v[r], s);
+    }
+    part[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && c < N)
+    {
+        const double t = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+        out[c] = sub ? out[c] - t : t;
+    }
+}
+
+// out[i] = (or -=) sum_{c < K} M[i * ld + c] v[c] for i < N, K a multiple
+// of 64. Each wavefront owns four consecutive rows: lane l reads columns
+// l, l + 64, ... of each (coalesced 512-byte segments), then every row sum
+// is reduced across the wavefront by a fixed butterfly.
+__global__ __launch_bounds__(THREADS) void k_rowdot(const double *__restrict__ M,
+                                                     long ld,
+                                                     int K,
+                                                     int N,
+                                                     const double *__restrict__ v,
+                                                     double *__restrict__ out,
+                                                     bool sub)
+{
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i0 = (blockIdx.x * 4 + w) * 4;
+    if (i0 >= N)
+        return;
+    const int rows = min(4, N - i0);
+    double s[4] = { 0.0, 0.0, 0.0, 0.0 };
+    for (int c0 = 0; c0 < K; c0 += 256)
+    {
+        double m[4][4], x[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+        {
+            const int c = c0 + u * 64 + lane;
+            x[u] = c < K ? v[c] : 0.0;
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++)
+                m[rr][u] = (c < K && rr < rows) ? M[static_cast<long>(i0 + rr) * ld + c] : 0.0;
+        }
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                s[rr] = fma(m[rr][u], x[u], s[rr]);
+    }
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            s[rr] += __shfl_xor(s[rr], off);
+    if (lane < rows)
+    {
+        const double t = lane == 0 ? s[0] : lane == 1 ? s[1] : lane == 2 ? s[2] : s[3];
+        out[i0 + lane] = sub ? out[i0 + lane] - t : t;
+    }
 }
 
 // colsq[j] over rows [0, n) of V (row-major, ldv): grid (ceil(m/64), splits),
@@ -432,6 +524,20 @@ void solve_upper_step(hipStream_t s, const double *L, long lda, const double *Wi
     const int nb = n_pad / NB;
     require(n_pad % NB == 0 && j >= 0 && j < nb, "solve_upper_step: bad shape");
     hipLaunchKernelGGL(k_solve_upper_step, dim3(j + 1), dim3(THREADS), 0, s, L, lda, Winv, rhs, out, j);
+    GP_CHECK_LAUNCH();
+}
+
+void coldot(hipStream_t s, const double *M, long ld, int K, int N, const double *v, double *out, bool sub)
+{
+    require(K >= 1 && N >= 1 && ld >= N, "coldot: bad shape");
+    hipLaunchKernelGGL(k_coldot, dim3(ceil_div(N, 64)), dim3(THREADS), 0, s, M, ld, K, N, v, out, sub);
+    GP_CHECK_LAUNCH();
+}
+
+void rowdot(hipStream_t s, const double *M, long ld, int K, int N, const double *v, double *out, bool sub)
+{
+    require(K >= 64 && K % 64 == 0 && N >= 1 && ld >= K, "rowdot: bad shape");
+    hipLaunchKernelGGL(k_rowdot, dim3(ceil_div(N, 16)), dim3(THREADS), 0, s, M, ld, K, N, v, out, sub);
     GP_CHECK_LAUNCH();
 }
 

@@ -176,6 +176,13 @@ void potrf64_inv(hipStream_t s, double *A, long lda, int c0, double *w, int *inf
 void solve_lower_step(hipStream_t s, const double *L, long lda, const double *Winv, double *rhs, double *out, int n_pad, int j);
 void solve_upper_step(hipStream_t s, const double *L, long lda, const double *Winv, double *rhs, double *out, int n_pad, int j);
 
+// Dense FP64 matrix-vector products for the wide-block solves (row-major M):
+// coldot: out[c] = (or -=) sum_{r<K} M[r][c] v[r], c < N;
+// rowdot: out[i] = (or -=) sum_{c<K} M[i][c] v[c], i < N, K % 64 == 0.
+// v must not overlap out.
+void coldot(hipStream_t s, const double *M, long ld, int K, int N, const double *v, double *out, bool sub);
+void rowdot(hipStream_t s, const double *M, long ld, int K, int N, const double *v, double *out, bool sub);
+
 // Inverses of the 256-wide diagonal blocks of L covering rows
 // [row0, row0 + rows) (the last block may be narrower), built from the
 // 64-wide inverses Winv by W_ij = -W_ii sum_{t=j}^{i-1} L_it W_tj over
