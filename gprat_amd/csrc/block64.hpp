@@ -38,19 +38,24 @@ __device__ __forceinline__ double rsqrt_refined(double d)
 
 // Cholesky factor and inverse of one 64x64 block held in LDS, by one
 // workgroup of 256 threads, in 16-wide steps:
-//  * factor, per step: wavefront 0 factors the 16x16 diagonal block in
-//    registers (lane i owns row i; each pivot's column reaches the other
-//    lanes through v_readlane, and the pivot's reciprocal square root is a
-//    refined hardware estimate), one thread per row below solves against it
-//    column by column, and the rank-16 update of the trailing block runs on
-//    the matrix cores (one 16x16 output tile per wavefront at a time);
-//  * inverse: the four 16x16 diagonal inverses are formed in parallel (one
-//    wavefront each, one column per lane), then the off-diagonal blocks by
-//    distance, W_ij = -W_ii * sum_{t=j}^{i-1} L_it W_tj, as two chained
-//    matrix-core products whose intermediate never leaves registers.
+//  * factor, per step: wavefront 0 holds the step's rows o.. 63 one per lane
+//    (16 columns each) and eliminates the 16 pivots in registers: each
+//    pivot's column reaches the other lanes through v_readlane and its
+//    reciprocal square root is a refined hardware estimate, so the lanes
+//    below the diagonal block finish the panel solve in the same pass.
+//    Wavefront 1 runs the same elimination on the diagonal rows with the
+//    identity rows e_s in lanes 16..31: what those lanes end with is
+//    e_s L^-T, column s of the diagonal block's inverse. The rank-16 update
+//    of the trailing block then runs on the matrix cores (one 16x16 output
+//    tile per wavefront at a time);
+//  * inverse: the off-diagonal blocks by distance,
+//    W_ij = -W_ii * sum_{t=j}^{i-1} L_it W_tj, as two chained matrix-core
+//    products whose intermediate never leaves registers.
 // On exit D holds L in its lower triangle (the strict upper triangle of the
-// 16x16 diagonal blocks holds scratch values), W = L^-1 (lower, zero above)
-// and R the reciprocals of L's diagonal. A non-positive pivot stores the
+// 16x16 diagonal blocks holds scratch values), W holds L^-1 on and below
+// the diagonal (exact zeros above it inside the 16x16 diagonal blocks, the
+// 16x16 blocks above those untouched) and R the reciprocals of L's
+// diagonal. A non-positive pivot stores the
 // smallest failing global column in *info.
 template <int NB = 64, int THREADS = 256>
 __device__ __forceinline__ void factor_block64(double (*D)[NB + 1], double (*W)[NB + 1], double *R, int *info, int col0)
@@ -62,12 +67,22 @@ __device__ __forceinline__ void factor_block64(double (*D)[NB + 1], double (*W)[
     for (int kb = 0; kb < 4; kb++)
     {
         const int o = kb * 16;
-        if (wave == 0)
+        const int rows = NB - o;  // rows o .. 63 of the step's columns
+        double r[16];
+        if (wave < 2)
         {
-            double r[16];
 #pragma unroll
             for (int t = 0; t < 16; t++)
-                r[t] = (lane < 16) ? D[o + lane][o + t] : 0.0;
+            {
+                if (wave == 0)
+                    r[t] = (lane < rows) ? D[o + lane][o + t] : 0.0;
+                else
+                    r[t] = (lane < 16) ? D[o + lane][o + t] : ((lane - 16 == t) ? 1.0 : 0.0);
+            }
+        }
+        __syncthreads();  // wavefront 1 has read the diagonal rows before wavefront 0 overwrites them
+        if (wave < 2)
+        {
             double piv = 1.0, rinv = 0.0;
 #pragma unroll
             for (int j = 0; j < 16; j++)
@@ -85,41 +100,36 @@ __device__ __forceinline__ void factor_block64(double (*D)[NB + 1], double (*W)[
                 for (int c = j + 1; c < 16; c++)
                     r[c] = fma(-lij, b64::lane_value(lij, c), r[c]);
             }
-            if (lane < 16)
+            if (wave == 0)
             {
-                if (!(piv > 0.0))
-                    atomicMin(info, col0 + o + lane);
-                R[o + lane] = rinv;
+                if (lane < 16)
+                {
+                    if (!(piv > 0.0))
+                        atomicMin(info, col0 + o + lane);
+                    R[o + lane] = rinv;
+#pragma unroll
+                    for (int t = 0; t < 16; t++)
+                        if (t <= lane)
+                            D[o + lane][o + t] = r[t];
+                }
+                else if (lane < rows)
+                {
+#pragma unroll
+                    for (int t = 0; t < 16; t++)
+                        D[o + lane][o + t] = r[t];
+                }
+            }
+            else if (lane >= 16 && lane < 32)
+            {
+                // zeros above the diagonal come out exact: e_s has none
+                // before column s
 #pragma unroll
                 for (int t = 0; t < 16; t++)
-                    if (t <= lane)
-                        D[o + lane][o + t] = r[t];
+                    W[o + t][o + lane - 16] = r[t];
             }
         }
         __syncthreads();
         const int below = NB - 16 - o;  // rows o+16 .. 63
-        if (tid < below)
-        {
-            // x L_oo^T = a, column by column: x_j = x_j / l_jj, then
-            // x_c -= x_j l_cj for the later columns
-            const int rr = o + 16 + tid;
-            double x[16];
-#pragma unroll
-            for (int t = 0; t < 16; t++)
-                x[t] = D[rr][o + t];
-#pragma unroll
-            for (int j = 0; j < 16; j++)
-            {
-                x[j] *= R[o + j];
-#pragma unroll
-                for (int c = j + 1; c < 16; c++)
-                    x[c] = fma(-x[j], D[o + c][o + j], x[c]);
-            }
-#pragma unroll
-            for (int t = 0; t < 16; t++)
-                D[rr][o + t] = x[t];
-        }
-        __syncthreads();
         if (below > 0)
         {
             // trailing block -= A A^T, A = D[o+16.., o..o+15], lower tiles only
@@ -146,41 +156,13 @@ __device__ __forceinline__ void factor_block64(double (*D)[NB + 1], double (*W)[
                         D[base + rw][base + cl] -= acc[e];
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
 
-    // diagonal 16x16 inverses, one wavefront per block, one column per lane:
-    // L w = e_c column by column (w_j /= l_jj, then w_i -= l_ij w_j)
-    {
-        const int o = wave * 16;
-        if (lane < 16)
-        {
-            double w[16];
-#pragma unroll
-            for (int i = 0; i < 16; i++)
-                w[i] = (i == lane) ? 1.0 : 0.0;
-#pragma unroll
-            for (int j = 0; j < 16; j++)
-            {
-                w[j] *= R[o + j];
-#pragma unroll
-                for (int i = j + 1; i < 16; i++)
-                    w[i] = fma(-w[j], D[o + i][o + j], w[i]);
-            }
-#pragma unroll
-            for (int i = 0; i < 16; i++)
-                W[o + i][o + lane] = w[i];
-        }
-        // zero the strict upper blocks of W
-        for (int e = tid; e < NB * NB; e += THREADS)
-        {
-            const int r = e / NB, c = e % NB;
-            if ((c >> 4) > (r >> 4))
-                W[r][c] = 0.0;
-        }
-    }
-    __syncthreads();
+    // the d = 1 step needs the diagonal inverses written by wavefront 1 in
+    // the last step (the barrier after the kb = 3 pivots covers it); the
+    // blocks above the diagonal of W are never read or written here
 #pragma unroll
     for (int d = 1; d < 4; d++)
     {

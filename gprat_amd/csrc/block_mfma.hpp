@@ -130,4 +130,37 @@ __device__ __forceinline__ void store_acc(double *C, long ld, const v4d (&acc)[2
              });
 }
 
+// C (global 64x64 block, ld) -= acc, optionally only on or below the
+// block's diagonal. C is read and written through one pointer, so every
+// load is issued before the first store (left to itself the compiler keeps
+// each element's load, wait and store in order).
+__device__ __forceinline__ void sub_acc(double *C, long ld, const v4d (&acc)[2][2], bool lower)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lm = lane & 15, lk = lane >> 4;
+    const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
+    double v[16];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+            {
+                const int r = r0 + i * 16 + lk + 4 * e, c = c0 + j * 16 + lm;
+                v[(i * 2 + j) * 4 + e] = (lower && c > r) ? 0.0 : C[static_cast<long>(r) * ld + c];
+            }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+            {
+                const int r = r0 + i * 16 + lk + 4 * e, c = c0 + j * 16 + lm;
+                if (!(lower && c > r))
+                    C[static_cast<long>(r) * ld + c] = v[(i * 2 + j) * 4 + e] - acc[i][j][e];
+            }
+}
+
 }  // namespace gpamd::k::blk

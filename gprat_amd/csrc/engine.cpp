@@ -323,8 +323,11 @@ static void trailing_kmajor(hipStream_t s, const double *P, long ldp, long off, 
 // Factor the diagonal block A[p0:pe, p0:pe] in place in 64-wide steps: the
 // fused factor + inverse of each 64x64 block (saved in Winv), the in-place
 // product with that inverse for the block's rows below it, and the update
-// of the rest of the block. Only rows inside the block are touched.
-static void factor_diagonal_block(Context &c, hipStream_t s, double *A, long lda, int p0, int pe, double *Winv)
+// of the rest of the block. Only rows inside the block are touched. The
+// update runs on at most update_wgs workgroups (beside a capped trailing
+// update: the slots that update leaves free, so it never waits for a
+// trailing tile to finish).
+static void factor_diagonal_block(Context &c, hipStream_t s, double *A, long lda, int p0, int pe, double *Winv, int update_wgs)
 {
     for (int c0 = p0; c0 < pe; c0 += 64)
     {
@@ -333,35 +336,11 @@ static void factor_diagonal_block(Context &c, hipStream_t s, double *A, long lda
         const int rest = pe - c0 - 64;
         if (rest <= 0)
             break;
-        const long o = static_cast<long>(c0 + 64) * lda;
-        k::GemmArgs g{};
-        g.A = A + o + c0;  // rows below: X = A W^T, in place (one column tile)
-        g.lda = lda;
-        g.B = w;
-        g.ldb = 64;
-        g.C = A + o + c0;
-        g.ldc = lda;
-        g.M = rest;
-        g.N = 64;
-        g.K = 64;
-        g.b_trans = true;
-        g.set = true;
-        g.role = k::GemmRole::PanelSolve;
-        k::gemm_f64(s, g);
-        k::GemmArgs u{};
-        u.A = A + o + c0;
-        u.lda = lda;
-        u.B = A + o + c0;
-        u.ldb = lda;
-        u.C = A + o + c0 + 64;
-        u.ldc = lda;
-        u.M = rest;
-        u.N = rest;
-        u.K = 64;
-        u.b_trans = true;
-        u.lower = true;
-        u.role = k::GemmRole::Panel;
-        k::gemm_f64(s, u);
+        // rows below: X = A W^T in place, then the rank-64 update of the
+        // rest of the block, both on 64x64 blocks so the short products
+        // spread over as many CUs as there are blocks
+        k::panel_solve64(s, A, lda, c0 + 64, c0, rest, w);
+        k::panel_update64(s, A, lda, c0 + 64, c0, rest, update_wgs);
     }
 }
 
@@ -499,7 +478,7 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
     auto diagonal = [&](int p0, int pe)
     {
         Timed t(c, P, "panel");
-        factor_diagonal_block(c, P, A, lda, p0, pe, Winv);
+        factor_diagonal_block(c, P, A, lda, p0, pe, Winv, ahead ? std::max(1, trailing_reserve()) : INT_MAX);
         if (pe < n_pad || Wpanel)
             invert_diagonal_block(P, A, lda, Winv, p0, pe - p0, Wblock(p0), Bo, tmp.get());
     };

@@ -54,6 +54,51 @@ __global__ __launch_bounds__(THREADS) void k_potrf64_inv(double *__restrict__ A,
     }
 }
 
+// Panel solve inside a diagonal block, 64 rows per workgroup: X = A W^T in
+// place for rows r0 + 64 b .. + 63 of the columns [c0, c0 + 64), W the
+// 64x64 row-major inverse of the step's diagonal block. Each workgroup
+// reads and writes only its own rows.
+__global__ __launch_bounds__(THREADS) void k_panel_solve64(double *__restrict__ A, long lda, long r0, long c0, const double *__restrict__ W)
+{
+    __shared__ double a[NB * blk::XS];
+    __shared__ double w[NB * blk::XS];
+    double *p = A + (r0 + static_cast<long>(NB) * blockIdx.x) * lda + c0;
+    blk::stage_block(a, p, lda);
+    blk::stage_block(w, W, NB);
+    __syncthreads();
+    blk::v4d acc[2][2];
+    blk::mfma_nt(a, w, acc);
+    blk::store_acc(p, lda, acc, true, false);
+}
+
+// Rank-64 update inside a diagonal block: C_ij -= X_i X_j^T over the lower
+// 64-block pairs i >= j of the nb x nb block region at (rs, rs), with
+// X_i = A[rs + 64 i .., c0 .. c0 + 63] (columns left of the region, never
+// written here); diagonal blocks on and below their diagonal only. G
+// workgroups loop over the pairs t = g, g + G, ...
+__global__ __launch_bounds__(THREADS) void k_panel_update64(double *__restrict__ A, long lda, long rs, long c0, int nb)
+{
+    __shared__ double xi[NB * blk::XS];
+    __shared__ double xj[NB * blk::XS];
+    const int pairs = nb * (nb + 1) / 2;
+    for (int t = blockIdx.x; t < pairs; t += gridDim.x)
+    {
+        int i = 0, j = t;
+        while (j > i)
+        {
+            j -= i + 1;
+            i++;
+        }
+        __syncthreads();  // the previous pair's LDS reads are done
+        blk::stage_block(xi, A + (rs + static_cast<long>(NB) * i) * lda + c0, lda);
+        blk::stage_block(xj, A + (rs + static_cast<long>(NB) * j) * lda + c0, lda);
+        __syncthreads();
+        blk::v4d acc[2][2];
+        blk::mfma_nt(xi, xj, acc);
+        blk::sub_acc(A + (rs + static_cast<long>(NB) * i) * lda + rs + static_cast<long>(NB) * j, lda, acc, i == j);
+    }
+}
+
 // x_J = W_JJ rhs_J (or W_JJ^T rhs_J), four column quarters per row combined
 // in a fixed order; every workgroup of a solve step computes it.
 template <bool TRANSPOSED>
@@ -145,22 +190,25 @@ __global__ __launch_bounds__(THREADS) void k_solve_upper_step(const double *__re
 }
 
 // out[c] = (or -=) sum_{r < K} M[r * ld + c] v[r] for c < N. Lane = column
-// (each row read is a coalesced 512-byte segment), the four wavefronts take
-// contiguous quarters of the rows with 16 loads in flight per lane, v comes
-// in through scalar loads, and the quarters are combined in a fixed order.
-__global__ __launch_bounds__(THREADS) void k_coldot(const double *__restrict__ M,
-                                                     long ld,
-                                                     int K,
-                                                     int N,
-                                                     const double *__restrict__ v,
-                                                     double *__restrict__ out,
-                                                     bool sub)
+// (each row read is a coalesced 512-byte segment), the sixteen wavefronts
+// of a 1024-thread workgroup take contiguous sixteenths of the rows with 16
+// loads in flight per lane (a 512-row block is then two batches deep), v
+// comes in through scalar loads, and the partial sums are combined in a
+// fixed order.
+constexpr int COLDOT_WAVES = 16;
+__global__ __launch_bounds__(64 * COLDOT_WAVES) void k_coldot(const double *__restrict__ M,
+                                                               long ld,
+                                                               int K,
+                                                               int N,
+                                                               const double *__restrict__ v,
+                                                               double *__restrict__ out,
+                                                               bool sub)
 {
-    __shared__ double part[4][64];
+    __shared__ double part[COLDOT_WAVES][64];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int c = blockIdx.x * 64 + lane;
-    const int q = (K + 3) / 4;
+    const int q = (K + COLDOT_WAVES - 1) / COLDOT_WAVES;
     const int r0 = min(K, w * q), r1 = min(K, r0 + q);
     double s = 0.0;
     if (c < N)
@@ -183,7 +231,10 @@ __global__ __launch_bounds__(THREADS) void k_coldot(const double *__restrict__ M
     __syncthreads();
     if (w == 0 && c < N)
     {
-        const double t = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+        double t = 0.0;
+#pragma unroll
+        for (int q2 = 0; q2 < COLDOT_WAVES; q2++)
+            t += part[q2][lane];
         out[c] = sub ? out[c] - t : t;
     }
 }
@@ -511,6 +562,21 @@ void potrf64_inv(hipStream_t s, double *A, long lda, int c0, double *w, int *inf
     GP_CHECK_LAUNCH();
 }
 
+void panel_solve64(hipStream_t s, double *A, long lda, long r0, long c0, int rows, const double *W)
+{
+    require(rows > 0 && rows % NB == 0 && c0 % NB == 0 && lda % 2 == 0, "panel_solve64: bad shape");
+    hipLaunchKernelGGL(k_panel_solve64, dim3(rows / NB), dim3(THREADS), 0, s, A, lda, r0, c0, W);
+    GP_CHECK_LAUNCH();
+}
+
+void panel_update64(hipStream_t s, double *A, long lda, long rs, long c0, int rows, int max_wgs)
+{
+    require(rows > 0 && rows % NB == 0 && c0 % NB == 0 && rs % NB == 0 && lda % 2 == 0 && max_wgs >= 1, "panel_update64: bad shape");
+    const int nb = rows / NB, pairs = nb * (nb + 1) / 2;
+    hipLaunchKernelGGL(k_panel_update64, dim3(std::min(pairs, max_wgs)), dim3(THREADS), 0, s, A, lda, rs, c0, nb);
+    GP_CHECK_LAUNCH();
+}
+
 void solve_lower_step(hipStream_t s, const double *L, long lda, const double *Winv, double *rhs, double *out, int n_pad, int j)
 {
     const int nb = n_pad / NB;
@@ -530,7 +596,7 @@ void solve_upper_step(hipStream_t s, const double *L, long lda, const double *Wi
 void coldot(hipStream_t s, const double *M, long ld, int K, int N, const double *v, double *out, bool sub)
 {
     require(K >= 1 && N >= 1 && ld >= N, "coldot: bad shape");
-    hipLaunchKernelGGL(k_coldot, dim3(ceil_div(N, 64)), dim3(THREADS), 0, s, M, ld, K, N, v, out, sub);
+    hipLaunchKernelGGL(k_coldot, dim3(ceil_div(N, 64)), dim3(64 * COLDOT_WAVES), 0, s, M, ld, K, N, v, out, sub);
     GP_CHECK_LAUNCH();
 }
 

@@ -167,6 +167,15 @@ void batch_fit(hipStream_t s, const BatchGP *gps, const int *order, int count, i
 // write its inverse W = L_cc^-1 (64x64 row-major, zero upper) to w. info
 // (device int) takes the smallest global column whose pivot is not positive.
 void potrf64_inv(hipStream_t s, double *A, long lda, int c0, double *w, int *info);
+// Inside a diagonal block, after potrf64_inv of the step at column c0:
+// panel_solve64: X = A W^T in place for A[r0 : r0 + rows, c0 : c0 + 64], W
+// the step's 64x64 row-major inverse (one workgroup per 64 rows);
+// panel_update64:
+// A[rs + i, rs + j] -= sum_k X[i][k] X[j][k] for the lower (i >= j) part of
+// the rows x rows region at (rs, rs), X = A[rs :, c0 : c0 + 64], on at most
+// max_wgs workgroups. rows, c0 and rs are multiples of 64.
+void panel_solve64(hipStream_t s, double *A, long lda, long r0, long c0, int rows, const double *W);
+void panel_update64(hipStream_t s, double *A, long lda, long rs, long c0, int rows, int max_wgs);
 
 // Block steps of the vector solves with the saved diagonal-block inverses
 // Winv (block J at Winv + J*64*64). Forward: out[J] = W_JJ rhs[J],
