@@ -37,6 +37,27 @@ __device__ __forceinline__ void stage_block(double *dst, const double *__restric
     }
 }
 
+// Copy a 64x64 row-major block into LDS transposed: dst[c][r] = src[r][c]
+// (stride XS); loads as in stage_block, then scattered LDS stores.
+__device__ __forceinline__ void stage_block_t(double *dst, const double *__restrict__ src, long ld)
+{
+    d2 v[8];
+#pragma unroll
+    for (int s = 0; s < 8; s++)
+    {
+        const int e = threadIdx.x + THREADS * s;
+        v[s] = *reinterpret_cast<const d2 *>(src + static_cast<long>(e >> 5) * ld + (e & 31) * 2);
+    }
+#pragma unroll
+    for (int s = 0; s < 8; s++)
+    {
+        const int e = threadIdx.x + THREADS * s;
+        const int r = e >> 5, c = (e & 31) * 2;
+        dst[c * XS + r] = v[s][0];
+        dst[(c + 1) * XS + r] = v[s][1];
+    }
+}
+
 __device__ __forceinline__ void zero(v4d (&acc)[2][2])
 {
 #pragma unroll

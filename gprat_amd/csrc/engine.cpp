@@ -345,50 +345,15 @@ static void factor_diagonal_block(Context &c, hipStream_t s, double *A, long lda
 }
 
 // Wb (k-major, leading dimension ldw: W(r, c) at Wb[c * ldw + r]) = inverse
-// of the factored diagonal block L[p0:p0+w, p0:p0+w]: the 256-wide diagonal
-// inverses first, then adjacent halves joined bottom-up by
-//   W_bt = -W_b L_bt W_t
-// (two MFMA products through `tmp`, at least w/2 x w/2 doubles). Every
-// entry of the w x w inverse is written, its upper part as zeros.
+// of the factored diagonal block L[p0:p0+w, p0:p0+w], by doubling from the
+// 64-wide inverses: W_ba = -W_b L_ba W_a for adjacent halves of 64, 128,
+// 256 columns, each level two launches of one workgroup per 64x64 output
+// block (`tmp` holds at least w/2 x w/2 doubles). Every entry of the w x w
+// inverse is written, its upper part as zeros.
 static void invert_diagonal_block(hipStream_t s, const double *A, long lda, const double *Winv, int p0, int w, double *Wb, long ldw,
-                                  double *tmp)
+                                  double *tmp, int max_wgs)
 {
-    k::invert_blocks256(s, A, lda, Winv, p0, w, Wb, ldw, 256 * ldw + 256, true);
-    for (int size = 256; size < w; size *= 2)
-        for (int t0 = 0; t0 + size < w; t0 += 2 * size)
-        {
-            const int sb = std::min(size, w - t0 - size);
-            // tmp = -L_bt W_t  (sb x size, leading dimension size)
-            k::fill_narrow(s, tmp, static_cast<long>(sb) * size, 0.0);
-            k::GemmArgs g{};
-            g.A = A + static_cast<long>(p0 + t0 + size) * lda + p0 + t0;
-            g.lda = lda;
-            g.B = Wb + static_cast<long>(t0) * ldw + t0;  // B(k, j) = W_t(k, j)
-            g.ldb = ldw;
-            g.b_trans = true;
-            g.C = tmp;
-            g.ldc = size;
-            g.M = sb;
-            g.N = size;
-            g.K = size;
-            g.role = k::GemmRole::Panel;
-            k::gemm_f64(s, g);
-            // W_bt^T = tmp^T W_b^T, stored straight into the k-major layout
-            k::GemmArgs h{};
-            h.A = tmp;  // A(j, k) = tmp(k, j)
-            h.lda = size;
-            h.a_trans = true;
-            h.B = Wb + static_cast<long>(t0 + size) * ldw + t0 + size;  // B(k, i) = W_b(i, k)
-            h.ldb = ldw;
-            h.C = Wb + static_cast<long>(t0) * ldw + t0 + size;
-            h.ldc = ldw;
-            h.M = size;
-            h.N = sb;
-            h.K = sb;
-            h.set = true;
-            h.role = k::GemmRole::Inverse;
-            k::gemm_f64(s, h);
-        }
+    k::invert_block_doubling(s, A, lda, Winv, p0, w, Wb, ldw, tmp, max_wgs);
 }
 
 // Rows r0.. of the panel columns [p0, p0 + w): X = A W^T with W the inverse
@@ -480,7 +445,7 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
         Timed t(c, P, "panel");
         factor_diagonal_block(c, P, A, lda, p0, pe, Winv, ahead ? std::max(1, trailing_reserve()) : INT_MAX);
         if (pe < n_pad || Wpanel)
-            invert_diagonal_block(P, A, lda, Winv, p0, pe - p0, Wblock(p0), Bo, tmp.get());
+            invert_diagonal_block(P, A, lda, Winv, p0, pe - p0, Wblock(p0), Bo, tmp.get(), ahead ? std::max(1, trailing_reserve()) : 64);
     };
     auto rows_below = [&](int p0, int pe)
     {

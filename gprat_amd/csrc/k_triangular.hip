@@ -99,6 +99,97 @@ __global__ __launch_bounds__(THREADS) void k_panel_update64(double *__restrict__
     }
 }
 
+// Inverse of a factored w x w diagonal block, k-major (W(r, c) at
+// Wb[c * ldw + r]), by doubling from its 64-wide inverses.
+// k_inv_place: the 64x64 diagonal blocks (row-major inverses, transposed
+// through LDS so both sides are coalesced) and zeros in every block above
+// the diagonal; G workgroups loop over the blocks rb <= cb.
+__global__ __launch_bounds__(THREADS) void k_inv_place(const double *__restrict__ Winv, int nb, double *__restrict__ Wb, long ldw)
+{
+    __shared__ double a[NB * blk::XS];
+    const int tid = threadIdx.x;
+    for (int t = blockIdx.x; t < nb * (nb + 1) / 2; t += gridDim.x)
+    {
+        int cb = 0, rb = t;  // t = cb (cb + 1) / 2 + rb, rb <= cb
+        while (rb > cb)
+        {
+            rb -= cb + 1;
+            cb++;
+        }
+        double *out = Wb + static_cast<long>(NB) * cb * ldw + NB * rb;
+        if (rb == cb)
+        {
+            __syncthreads();  // the previous block's LDS reads are done
+            blk::stage_block(a, Winv + static_cast<long>(rb) * NB * NB, NB);
+            __syncthreads();
+            for (int e = tid; e < NB * NB; e += THREADS)
+            {
+                const int c = e >> 6, r = e & 63;
+                out[static_cast<long>(c) * ldw + r] = a[r * blk::XS + c];
+            }
+        }
+        else
+        {
+            for (int e = tid; e < NB * NB; e += THREADS)
+                out[static_cast<long>(e >> 6) * ldw + (e & 63)] = 0.0;
+        }
+    }
+}
+
+// One doubling level: for every pair of adjacent size-wide halves a = [t0,
+// t0 + size), b = [t0 + size, t0 + size + sb) of the block (t0 = 2 size p),
+//   mode 0: T = L_ba W_a, kept transposed in tmp (pair p at p size^2,
+//           T(r, c) at [c * size + r]); W_a lower, so k runs from c's block;
+//   mode 1: W_ba = -W_b T into Wb; W_b lower, so k stops at r's block.
+// One workgroup per 64x64 output block: blockIdx.x = (p, rb, cb).
+__global__ __launch_bounds__(THREADS) void k_inv_join(const double *__restrict__ L,
+                                                       long lda,
+                                                       long p0,
+                                                       int w,
+                                                       int size,
+                                                       double *__restrict__ Wb,
+                                                       long ldw,
+                                                       double *__restrict__ tmp,
+                                                       int mode)
+{
+    __shared__ double a[NB * blk::XS];
+    __shared__ double b[NB * blk::XS];
+    const int nbs = size / NB;
+    const int pair = blockIdx.x / (nbs * nbs), rem = blockIdx.x % (nbs * nbs);
+    const int rb = rem / nbs, cb = rem % nbs;
+    const long t0 = 2L * size * pair;
+    const int sb = static_cast<int>(min(static_cast<long>(size), w - t0 - size));
+    if (NB * rb >= sb)
+        return;
+    double *tp = tmp + static_cast<long>(pair) * size * size;
+    blk::v4d acc[2][2];
+    blk::zero(acc);
+    if (mode == 0)
+    {
+        for (int kb = cb; kb < nbs; kb++)
+        {
+            __syncthreads();
+            blk::stage_block(a, L + (p0 + t0 + size + NB * rb) * lda + p0 + t0 + NB * kb, lda);  // L_ba(r, k)
+            blk::stage_block(b, Wb + (t0 + NB * cb) * ldw + t0 + NB * kb, ldw);                // [c][k] = W_a(k, c)
+            __syncthreads();
+            blk::mfma_nt_acc(a, b, acc);
+        }
+        blk::for_each(acc, [&](int r, int c, double v) { tp[static_cast<long>(NB * cb + c) * size + NB * rb + r] = v; });
+    }
+    else
+    {
+        for (int kb = 0; kb <= rb; kb++)
+        {
+            __syncthreads();
+            blk::stage_block_t(a, Wb + (t0 + size + NB * kb) * ldw + t0 + size + NB * rb, ldw);  // [r][k] = W_b(r, k)
+            blk::stage_block(b, tp + static_cast<long>(NB * cb) * size + NB * kb, size);       // [c][k] = T(k, c)
+            __syncthreads();
+            blk::mfma_nt_acc(a, b, acc);
+        }
+        blk::for_each(acc, [&](int r, int c, double v) { Wb[(t0 + NB * cb + c) * ldw + t0 + size + NB * rb + r] = -v; });
+    }
+}
+
 // x_J = W_JJ rhs_J (or W_JJ^T rhs_J), four column quarters per row combined
 // in a fixed order; every workgroup of a solve step computes it.
 template <bool TRANSPOSED>
@@ -560,6 +651,28 @@ void potrf64_inv(hipStream_t s, double *A, long lda, int c0, double *w, int *inf
     require(c0 >= 0 && c0 % NB == 0 && lda % 2 == 0, "potrf64_inv: bad shape");
     hipLaunchKernelGGL(k_potrf64_inv, dim3(1), dim3(THREADS), 0, s, A, lda, c0, w, info);
     GP_CHECK_LAUNCH();
+}
+
+void invert_block_doubling(hipStream_t s, const double *L, long lda, const double *Winv, int p0, int w, double *Wb, long ldw, double *tmp,
+                           int max_wgs)
+{
+    require(w > 0 && w % NB == 0 && p0 % NB == 0 && ldw >= w && ldw % 2 == 0 && lda % 2 == 0 && max_wgs >= 1,
+            "invert_block_doubling: bad shape");
+    const int nb = w / NB;
+    hipLaunchKernelGGL(k_inv_place, dim3(std::min(nb * (nb + 1) / 2, max_wgs)), dim3(THREADS), 0, s, Winv + static_cast<long>(p0 / NB) * NB * NB,
+                       nb, Wb, ldw);
+    GP_CHECK_LAUNCH();
+    for (int size = NB; size < w; size *= 2)
+    {
+        const int pairs = static_cast<int>(ceil_div(static_cast<long>(w - size), 2L * size));  // t0 = 2 size p with t0 + size < w
+        const int nbs = size / NB;
+        for (int mode = 0; mode < 2; mode++)
+        {
+            hipLaunchKernelGGL(k_inv_join, dim3(pairs * nbs * nbs), dim3(THREADS), 0, s, L, lda, static_cast<long>(p0), w, size, Wb, ldw, tmp,
+                               mode);
+            GP_CHECK_LAUNCH();
+        }
+    }
 }
 
 void panel_solve64(hipStream_t s, double *A, long lda, long r0, long c0, int rows, const double *W)

@@ -174,6 +174,15 @@ void potrf64_inv(hipStream_t s, double *A, long lda, int c0, double *w, int *inf
 // A[rs + i, rs + j] -= sum_k X[i][k] X[j][k] for the lower (i >= j) part of
 // the rows x rows region at (rs, rs), X = A[rs :, c0 : c0 + 64], on at most
 // max_wgs workgroups. rows, c0 and rs are multiples of 64.
+// Wb (k-major, W(r, c) at Wb[c * ldw + r], upper part zero) = inverse of
+// the factored w x w diagonal block L[p0 :, p0 :], built by doubling from
+// the 64-wide row-major inverses in Winv: W_ba = -W_b L_ba W_a for adjacent
+// halves of 64, 128, 256, ... columns. tmp holds at least (w/2)^2 doubles.
+// The diagonal placement and zero fill run on at most max_wgs workgroups;
+// the joins use one workgroup per 64x64 output block, at most (w/128)^2 per
+// launch.
+void invert_block_doubling(hipStream_t s, const double *L, long lda, const double *Winv, int p0, int w, double *Wb, long ldw, double *tmp,
+                           int max_wgs);
 void panel_solve64(hipStream_t s, double *A, long lda, long r0, long c0, int rows, const double *W);
 void panel_update64(hipStream_t s, double *A, long lda, long rs, long c0, int rows, int max_wgs);
 
