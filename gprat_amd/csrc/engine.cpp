@@ -743,9 +743,9 @@ void factor(Context &c, Model &m, const double *x, std::size_t x_len, int n, int
 
 // FP64 iterative refinement of alpha against a factor of reduced accuracy:
 // r = y - K alpha with K regenerated from the series in FP64, delta from the
-// factor, alpha += delta, until the correction is at most 1e-12 max|alpha|
-// (the error left after it is smaller still, since every step contracts it)
-// or stops shrinking.
+// factor, alpha += delta, until the correction, or the next one as the
+// contraction so far predicts it, is at most 1e-12 max|alpha|, or the
+// corrections stop shrinking.
 void refine_alpha(Model &m, double *work)
 {
     Context &c = *m.ctx;
@@ -778,7 +778,10 @@ void refine_alpha(Model &m, double *work)
         }
         m.refine_iterations = it + 1;
         m.refine_last = amax > 0.0 ? dmax / amax : 0.0;
-        if (!(dmax <= 1e-12 * amax) && dmax < 0.5 * prev)
+        // the next correction is about (dmax / prev) * dmax: when that is
+        // already below the target, this step has reached it
+        const bool converged = dmax <= 1e-12 * amax || (it > 0 && dmax < prev && dmax * (dmax / prev) <= 1e-12 * amax);
+        if (!converged && dmax < 0.5 * prev)
         {
             prev = dmax;
             continue;
