@@ -170,8 +170,11 @@ __device__ __forceinline__ void store_quarter(const Params &p, const v4d (&acc)[
     }
 }
 
+// part 0: the tile's whole K range; parts 1 and 2: its first and second
+// half (in slabs), for C -= updates only.
 template <bool AT, bool BT>
-__device__ __forceinline__ void gemm_tile(const Params &p, long r0, long c0, double (*As)[BK * LDS_STRIDE], double (*Bs)[BK * LDS_STRIDE])
+__device__ __forceinline__ void gemm_tile(const Params &p, long r0, long c0, double (*As)[BK * LDS_STRIDE], double (*Bs)[BK * LDS_STRIDE],
+                                          int part = 0)
 {
     long kbeg = 0;
     if (p.k_from_tile)
@@ -179,6 +182,15 @@ __device__ __forceinline__ void gemm_tile(const Params &p, long r0, long c0, dou
         kbeg = (r0 > c0 ? r0 : c0) + p.kstart_off;
         kbeg = kbeg < 0 ? 0 : kbeg;
         kbeg = (kbeg / BK) * BK;
+    }
+    long kend = p.K;
+    if (part != 0)
+    {
+        const long mid = kbeg < kend ? kbeg + ((kend - kbeg) / BK / 2) * BK : kend;
+        if (part == 1)
+            kend = mid;
+        else
+            kbeg = mid;
     }
     const int rows_a = static_cast<int>(min(static_cast<long>(BM), p.M - r0));
     const int rows_b = static_cast<int>(min(static_cast<long>(BN), p.N - c0));
@@ -195,7 +207,7 @@ __device__ __forceinline__ void gemm_tile(const Params &p, long r0, long c0, dou
         for (int j = 0; j < 4; j++)
             acc[i][j] = v4d{ 0.0, 0.0, 0.0, 0.0 };
 
-    if (kbeg < p.K)
+    if (kbeg < kend)
     {
         Slab<AT> sa;
         Slab<!BT> sb;
@@ -217,7 +229,7 @@ __device__ __forceinline__ void gemm_tile(const Params &p, long r0, long c0, dou
         __syncthreads();
 
         int buf = 0;
-        const long steps = (p.K - kbeg) / BK;
+        const long steps = (kend - kbeg) / BK;
         for (long t = 1; t < steps; t++)
         {
             ak += a_step;
@@ -269,6 +281,11 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
 
 // Capped launch: G workgroups loop over the row-major tiles (w, w + G, ...),
 // so the launch leaves workgroup slots free for kernels on other streams.
+// Two workgroups share a CU and their tiles take equal time, so left alone
+// they would reach their epilogues (memory-bound, no matrix-core work)
+// together. For C -= updates the upper half of the grid therefore starts
+// with the first half of its first tile's K range and finishes that tile's
+// second half last: its epilogues fall half a tile after the lower half's.
 template <bool AT, bool BT, int ROLE>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gemm_f64_capped(Params p)
 {
@@ -276,6 +293,8 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
     __shared__ double Bs[2][BK * LDS_STRIDE];
     const long gx = (p.N + BN - 1) / BN;
     const long tiles = gx * ((p.M + BM - 1) / BM);
+    const bool stagger = !p.set && blockIdx.x >= gridDim.x / 2;
+    long first = -1;
     for (long t = blockIdx.x; t < tiles; t += gridDim.x)
     {
         const long r0 = (t / gx) * BM;
@@ -283,7 +302,18 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
         if (p.lower && r0 + (BM - 1) + p.diag_off < c0)
             continue;  // tile entirely above the diagonal
         __syncthreads();  // the previous tile's last slab has been read by every wavefront
-        gemm_tile<AT, BT>(p, r0, c0, As, Bs);
+        if (stagger && first < 0)
+        {
+            first = t;
+            gemm_tile<AT, BT>(p, r0, c0, As, Bs, 1);
+        }
+        else
+            gemm_tile<AT, BT>(p, r0, c0, As, Bs, 0);
+    }
+    if (first >= 0)
+    {
+        __syncthreads();
+        gemm_tile<AT, BT>(p, (first / gx) * BM, (first % gx) * BN, As, Bs, 2);
     }
 }
 
