@@ -34,6 +34,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "fit+predict req/s & p50/p99 latency at ≤1e-8 rel err; n=32k Cholesky TFLOP/s"
 FP64_MFMA_PEAK_TFLOPS = 78.6
+FP32_MFMA_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
 N_TILES, TILE, N_REG, N_TEST = 32, 512, 8, 4096
@@ -316,6 +317,9 @@ def serving_leg(capi, count=192, budget=200 << 30, slots=4):
                             "p50_ms": percentile(po_lat, 50) * 1e3, "p99_ms": percentile(po_lat, 99) * 1e3})
         stats = srv.stats()
     n_fit = sum(r[0] == "fit" for r in reqs)
+    # dense work of the trace: n^3/3 per fit, n^2 m per predict (variance
+    # solve); the request rate if all of it ran at the matrix-core peaks
+    work = sum(r[3] ** 3 / 3.0 for r in reqs if r[0] == "fit") + sum(reqs[r[1]][3] ** 2 * r[3] for r in reqs if r[0] == "predict")
     return {
         "requests": len(reqs),
         "fits": n_fit,
@@ -333,6 +337,9 @@ def serving_leg(capi, count=192, budget=200 << 30, slots=4):
         "sequential_p50_ms": percentile(times, 50) * 1e3,
         "sequential_p99_ms": percentile(times, 99) * 1e3,
         "failed": sat_bad + po_bad,
+        "trace_flops": work,
+        "req_s_if_all_work_at_fp64_peak": len(reqs) / (work / (FP64_MFMA_PEAK_TFLOPS * 1e12)),
+        "req_s_if_all_work_at_fp32_peak": len(reqs) / (work / (FP32_MFMA_PEAK_TFLOPS * 1e12)),
         "server_stats": {k: stats[k] for k in ("batched_fits", "batch_launches", "evictions", "refits")},
         "workload": "70% fits n log-uniform [256, 32768] (n_reg 8), 30% predicts m uniform [1, 4096] on earlier models",
     }

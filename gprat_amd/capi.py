@@ -61,10 +61,13 @@ SIGNATURES = {
                                        C.c_int, _D, _D],
     "gprat_predict_with_full_cov": [_VP, _D, _SZ, _D, _SZ, _D, _SZ, _D, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_int, _D, _D],
+    "gprat_predict_full_posterior": [_VP, _D, _SZ, _D, _SZ, _D, _SZ, _D, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_int, _D, _D],
     "gprat_compute_loss": [_VP, _D, _SZ, _D, _SZ, _D, C.c_int, C.c_int, C.c_int, _D],
     "gprat_loss_gradient": [_VP, _D, _SZ, _D, _SZ, _D, C.c_int, C.c_int, C.c_int, _I, _D, _D],
     "gprat_model_fit": [_VP, _D, _SZ, _D, _SZ, C.c_int, C.c_int, _D, C.c_int, C.POINTER(_VP)],
     "gprat_model_predict": [_VP, _D, _SZ, C.c_int, _D, _D],
+    "gprat_model_predict_full_cov": [_VP, _D, _SZ, C.c_int, _D, _D],
     "gprat_model_loss": [_VP, _D],
     "gprat_model_gradient": [_VP, _I, _D, _D],
     "gprat_model_bytes": [_VP, C.POINTER(_SZ)],
@@ -187,6 +190,16 @@ class Context:
             return mean
         return mean, var
 
+    def predict_full_posterior(self, train_in, train_out, test_in, theta, n_tiles, n_tile_size, m_tiles, m_tile_size, n_reg):
+        """(mean, the whole m x m latent posterior covariance)."""
+        x, y, z, t = _arr(train_in), _arr(train_out), _arr(test_in), _arr(theta)
+        m = m_tiles * m_tile_size
+        mean = np.empty(m)
+        cov = np.empty((m, m))
+        _check(lib().gprat_predict_full_posterior(self.handle, _dptr(x), x.size, _dptr(y), y.size, _dptr(z), z.size, _dptr(t),
+                                                  n_tiles, n_tile_size, m_tiles, m_tile_size, n_reg, _dptr(mean), _dptr(cov)))
+        return mean, cov
+
     def compute_loss(self, train_in, train_out, theta, n_tiles, n_tile_size, n_reg):
         x, y, t = _arr(train_in), _arr(train_out), _arr(theta)
         out = C.c_double(0.0)
@@ -303,6 +316,14 @@ class Model:
         var = np.empty(m) if variance else None
         _check(lib().gprat_model_predict(self.handle, zp, zl, m, _dptr(mean), _dptr(var) if variance else None))
         return (mean, var) if variance else mean
+
+    def predict_full_cov(self, test_in, m):
+        """(mean, the whole m x m latent posterior covariance)."""
+        zp, zl, _zk = _ptr_len(test_in)
+        mean = np.empty(m)
+        cov = np.empty((m, m))
+        _check(lib().gprat_model_predict_full_cov(self.handle, zp, zl, m, _dptr(mean), _dptr(cov)))
+        return mean, cov
 
     def loss(self):
         out = C.c_double(0.0)

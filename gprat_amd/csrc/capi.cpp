@@ -370,6 +370,36 @@ int gprat_predict_with_full_cov(gprat_context *ctx,
                           n_tile_size, m_tiles, m_tile_size, n_reg, mean_out, var_out, true);
 }
 
+int gprat_predict_full_posterior(gprat_context *ctx,
+                                 const double *train_in,
+                                 size_t train_in_len,
+                                 const double *train_out,
+                                 size_t train_out_len,
+                                 const double *test_in,
+                                 size_t test_in_len,
+                                 const double theta[3],
+                                 int n_tiles,
+                                 int n_tile_size,
+                                 int m_tiles,
+                                 int m_tile_size,
+                                 int n_reg,
+                                 double *mean_out,
+                                 double *cov_out)
+{
+    return guarded(
+        [&]
+        {
+            auto &c = ctx_ref(ctx);
+            const Tiling t = check_tiling(n_tiles, n_tile_size, m_tiles, m_tile_size, n_reg);
+            if (t.m > 0)
+                gpamd::require(mean_out != nullptr && cov_out != nullptr, "mean_out or cov_out is null");
+            std::lock_guard<std::recursive_mutex> g(c.mu);
+            gpamd::Model model;
+            fit_transient(c, model, train_in, train_in_len, train_out, train_out_len, theta, t.n, n_reg);
+            gpamd::predict_full_cov(model, test_in, test_in_len, t.m, mean_out, cov_out);
+        });
+}
+
 int gprat_compute_loss(gprat_context *ctx,
                        const double *train_in,
                        size_t train_in_len,
@@ -452,6 +482,17 @@ int gprat_model_predict(gprat_model *model, const double *test_in, size_t test_i
             auto &md = model_ref(model);
             std::lock_guard<std::recursive_mutex> g(md.ctx->mu);
             gpamd::predict(md, test_in, test_in_len, m, mean_out, var_out);
+        });
+}
+
+int gprat_model_predict_full_cov(gprat_model *model, const double *test_in, size_t test_in_len, int m, double *mean_out, double *cov_out)
+{
+    return guarded(
+        [&]
+        {
+            auto &md = model_ref(model);
+            std::lock_guard<std::recursive_mutex> g(md.ctx->mu);
+            gpamd::predict_full_cov(md, test_in, test_in_len, m, mean_out, cov_out);
         });
 }
 

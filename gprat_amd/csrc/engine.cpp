@@ -356,35 +356,6 @@ static void invert_diagonal_block(hipStream_t s, const double *A, long lda, cons
     k::invert_block_doubling(s, A, lda, Winv, p0, w, Wb, ldw, tmp, max_wgs);
 }
 
-// Rows r0.. of the panel columns [p0, p0 + w): X = A W^T with W the inverse
-// of the panel's diagonal block (k-major Wb). In place, 128 columns per
-// launch from right to left: X[:, c:c+128] needs A[:, 0:c+128] only (W is
-// lower triangular) and every launch has a single column tile, so no
-// workgroup reads what another writes.
-static void solve_panel_rows(hipStream_t s, double *A, long lda, int p0, int w, int r0, int n_pad, const double *Wb, long ldw,
-                             int max_wgs)
-{
-    if (r0 >= n_pad)
-        return;
-    for (int cc = w - 128; cc >= 0; cc -= 128)
-    {
-        k::GemmArgs g{};
-        g.A = A + static_cast<long>(r0) * lda + p0;
-        g.lda = lda;
-        g.B = Wb + cc;  // B(k, j) = W(cc + j, k)
-        g.ldb = ldw;
-        g.C = A + static_cast<long>(r0) * lda + p0 + cc;
-        g.ldc = lda;
-        g.M = n_pad - r0;
-        g.N = 128;
-        g.K = cc + 128;
-        g.set = true;
-        g.role = k::GemmRole::PanelX;
-        g.max_wgs = max_wgs;
-        k::gemm_f64(s, g);
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Cholesky: blocked right-looking with one panel of look-ahead.
 //
@@ -421,6 +392,7 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
     // inverses of the panels' diagonal blocks: kept (Wpanel, block kb at
     // kb * Bo * Bo) or one scratch block reused by every panel
     DevBuf<double> Wscratch, tmp(c, static_cast<std::size_t>(Bo / 2) * (Bo / 2) + 256 * 256);
+    DevBuf<double> Xs(c, static_cast<std::size_t>(Bo) * n_pad);  // the panel's rows below, before the copy back
     if (!Wpanel)
         Wscratch = DevBuf<double>(c, static_cast<std::size_t>(Bo) * Bo);
     auto Wblock = [&](int p0) { return Wpanel ? Wpanel + static_cast<long>(p0 / Bo) * Bo * Bo : Wscratch.get(); };
@@ -447,14 +419,36 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
         if (pe < n_pad || Wpanel)
             invert_diagonal_block(P, A, lda, Winv, p0, pe - p0, Wblock(p0), Bo, tmp.get(), ahead ? std::max(1, trailing_reserve()) : 64);
     };
+    // The panel's rows below its diagonal block: X = A W^T with W the
+    // inverse of the diagonal block (k-major), one launch into scratch (each
+    // column tile stops K at its last column: W is lower triangular), then
+    // one pass that writes the k-major copy for the trailing update and X
+    // back into the panel.
     auto rows_below = [&](int p0, int pe)
     {
         Timed t(c, T, "panel_rows");
-        solve_panel_rows(T, A, lda, p0, pe - p0, pe, n_pad, Wblock(p0), Bo, 0);
+        const int w = pe - p0, R = n_pad - pe;
+        if (R <= 0)
+            return;
+        double *panel = A + static_cast<long>(pe) * lda + p0;
+        k::GemmArgs g{};
+        g.A = panel;
+        g.lda = lda;
+        g.B = Wblock(p0);  // B(k, j) = W(j, k)
+        g.ldb = Bo;
+        g.C = Xs.get();
+        g.ldc = w;
+        g.M = R;
+        g.N = w;
+        g.K = w;
+        g.set = true;
+        g.k_to_col = true;
+        g.role = k::GemmRole::PanelX;
+        k::gemm_f64(T, g);
         if (mixed)
-            k::panel_to_f32t(T, A, lda, pe, p0, n_pad - pe, pe - p0, p32.get(), n_pad);
+            k::panel_to_f32t(T, Xs.get(), w, 0, 0, R, w, p32.get(), n_pad, panel, lda);
         else
-            k::panel_to_f64t(T, A, lda, pe, p0, n_pad - pe, pe - p0, p64.get(), n_pad);
+            k::panel_to_f64t(T, Xs.get(), w, 0, 0, R, w, p64.get(), n_pad, panel, lda);
     };
     auto trailing = [&](long off, double *C, int M, int N, int K)
     {
@@ -659,6 +653,110 @@ void trsm_blocked(Context &c, const double *L, long lda, const double *Winv, con
     }
 }
 
+// Precision-specific steps of the variance solve: the diagonal step
+// V_J = W_J B_J (W_J k-major), the update B -= P V_J from the k-major copy
+// P of L's column panel, the squares of V_J, and the copy itself.
+struct VarianceF64
+{
+    using T = double;
+    static void leaf(hipStream_t s, const T *W, int ldw, const T *B, long ldb, T *V, int nb, int cols)
+    {
+        k::GemmArgs g{};
+        g.A = W;
+        g.lda = ldw;
+        g.a_trans = true;
+        g.B = B;
+        g.ldb = ldb;
+        g.C = V;
+        g.ldc = ldb;
+        g.M = nb;
+        g.N = cols;
+        g.K = nb;
+        g.set = true;
+        g.role = k::GemmRole::LeafT;
+        k::gemm_f64(s, g);
+    }
+    static void update(hipStream_t s, const T *P, long ldp, const T *V, long ldv, T *B, long ldb, int M, int cols, int K)
+    {
+        k::GemmArgs u{};
+        u.A = P;
+        u.lda = ldp;
+        u.a_trans = true;
+        u.B = V;
+        u.ldb = ldv;
+        u.C = B;
+        u.ldc = ldb;
+        u.M = M;
+        u.N = cols;
+        u.K = K;
+        u.role = k::GemmRole::SolveT;
+        k::gemm_f64(s, u);
+    }
+    static void squares(hipStream_t s, const T *V, long ldv, int rows, int mt, double *colsq) { k::colsq_accumulate(s, V, ldv, rows, mt, colsq); }
+    static void panel(hipStream_t s, const double *L, long lda, long j1, long j0, int R, int W, T *P, long ldp)
+    {
+        k::panel_to_f64t(s, L, lda, j1, j0, R, W, P, ldp);
+    }
+};
+
+struct VarianceF32
+{
+    using T = float;
+    static void leaf(hipStream_t s, const T *W, int ldw, const T *B, long ldb, T *V, int nb, int cols)
+    {
+        k::gemm_f32_tn(s, W, ldw, B, ldb, V, ldb, nb, cols, nb, true);
+    }
+    static void update(hipStream_t s, const T *P, long ldp, const T *V, long ldv, T *B, long ldb, int M, int cols, int K)
+    {
+        k::gemm_f32_tn(s, P, ldp, V, ldv, B, ldb, M, cols, K, false);
+    }
+    static void squares(hipStream_t s, const T *V, long ldv, int rows, int mt, double *colsq)
+    {
+        k::colsq_accumulate_f32(s, V, ldv, rows, mt, colsq);
+    }
+    static void panel(hipStream_t s, const double *L, long lda, long j1, long j0, int R, int W, T *P, long ldp)
+    {
+        k::panel_to_f32t(s, L, lda, j1, j0, R, W, P, ldp);
+    }
+};
+
+// V = L^-1 B (B n_pad x m_pad, overwritten), blocked right-looking in
+// NBK-row steps with the k-major block inverses W: V_J = W_J B_J goes to
+// consume(stream, V_J, rows) (V_J is NBK x m_pad, leading dimension m_pad,
+// valid until the next step), then B[below] -= P_J V_J with P_J the k-major
+// copy of L's column panel J. V itself is never stored whole.
+template <class Ops, class Consume>
+static void blocked_solve(Context &c, const double *L, long lda, const typename Ops::T *W, int NBK, int n_pad, typename Ops::T *B, int m_pad,
+                          Consume consume)
+{
+    using T = typename Ops::T;
+    hipStream_t s = c.main;
+    const int nblk = static_cast<int>(ceil_div(n_pad, NBK));
+    DevBuf<T> V(c, static_cast<std::size_t>(NBK) * m_pad), P(c, static_cast<std::size_t>(NBK) * n_pad);
+    for (int J = 0; J < nblk; J++)
+    {
+        const int j0 = J * NBK, nb = std::min(NBK, n_pad - j0), j1 = j0 + nb;
+        Ops::leaf(s, W + static_cast<long>(J) * NBK * NBK, NBK, B + static_cast<long>(j0) * m_pad, m_pad, V.get(), nb, m_pad);
+        consume(s, static_cast<const T *>(V.get()), nb);
+        if (j1 >= n_pad)
+            break;
+        Ops::panel(s, L, lda, j1, j0, n_pad - j1, nb, P.get(), n_pad);
+        Ops::update(s, P.get(), n_pad, V.get(), m_pad, B + static_cast<long>(j1) * m_pad, m_pad, n_pad - j1, m_pad, nb);
+    }
+    c.sync();
+}
+
+// colsq[j] = sum_i V[i][j]^2 over the solve above, for j < mt; squares are
+// summed in step order.
+template <class Ops>
+static void variance_solve(Context &c, const double *L, long lda, const typename Ops::T *W, int NBK, int n_pad, typename Ops::T *B,
+                           int m_pad, int mt, double *colsq)
+{
+    k::fill(c.main, colsq, m_pad, 0.0);
+    blocked_solve<Ops>(c, L, lda, W, NBK, n_pad, B, m_pad,
+                       [&](hipStream_t s, const typename Ops::T *V, int rows) { Ops::squares(s, V, m_pad, rows, mt, colsq); });
+}
+
 // Column sums of squares of V = L^-1 K*^T for a mixed-precision model: the
 // same 256-step blocked solve as trsm_blocked, with FP32 copies of the block
 // inverses, of L's column panels and of K*^T on FP32 matrix cores; the
@@ -680,27 +778,13 @@ static void variance_colsq_f32(Context &c, const FactorView &m, const double *z,
         k::to_f32(s, Wt.get(), Wt32.get(), static_cast<long>(wcount));
         c.sync();  // Wt goes back to the pool
     }
-    DevBuf<float> B(c, static_cast<std::size_t>(n_pad) * m_pad), V(c, static_cast<std::size_t>(NBK) * m_pad),
-        Pt(c, static_cast<std::size_t>(NBK) * n_pad);
+    DevBuf<float> B(c, static_cast<std::size_t>(n_pad) * m_pad);
     {
         Timed t(c, s, "assemble");
         k::assemble_cross_t_f32(s, m.x, z, m.r, m.sek, B.get(), m_pad, m.n, n_pad, mt, m_pad);
     }
     Timed t(c, s, "trsm");
-    k::fill(s, colsq, m_pad, 0.0);
-    for (int J = 0; J < nblk; J++)
-    {
-        const int j0 = J * NBK, nb = std::min(NBK, n_pad - j0), j1 = j0 + nb;
-        k::gemm_f32_tn(s, Wt32.get() + static_cast<long>(J) * NBK * NBK, NBK, B.get() + static_cast<long>(j0) * m_pad, m_pad,
-                       V.get(), m_pad, nb, m_pad, nb, true);
-        k::colsq_accumulate_f32(s, V.get(), m_pad, nb, mt, colsq);
-        if (j1 >= n_pad)
-            break;
-        k::panel_to_f32t(s, m.L, m.lda, j1, j0, n_pad - j1, nb, Pt.get(), n_pad);
-        k::gemm_f32_tn(s, Pt.get(), n_pad, V.get(), m_pad, B.get() + static_cast<long>(j1) * m_pad, m_pad, n_pad - j1, m_pad, nb,
-                       false);
-    }
-    c.sync();
+    variance_solve<VarianceF32>(c, m.L, m.lda, Wt32.get(), NBK, n_pad, B.get(), m_pad, mt, colsq);
 }
 
 // ---------------------------------------------------------------------------
@@ -865,16 +949,88 @@ void predict_view(const FactorView &m, const double *z, std::size_t z_len, int m
         }
         {
             Timed t(c, c.main, "trsm");
-            DevBuf<double> V(c, static_cast<std::size_t>(m.Wpanel ? m.wblock : 256) * ldb);
-            k::fill(c.main, colsq.get(), m_pad, 0.0);
-            trsm_blocked(c, m.L, m.lda, m.Winv, m.Wpanel, m.wblock, m.n_pad, B.get(), ldb, m_pad, false, V.get(), ldb, true, colsq.get(),
-                         mt);
+            // the factorisation's block inverses, or 256-wide ones built here
+            const int NBK = m.Wpanel ? m.wblock : 256;
+            DevBuf<double> Wt;
+            if (!m.Wpanel)
+            {
+                Wt = DevBuf<double>(c, static_cast<std::size_t>(ceil_div(m.n_pad, NBK)) * NBK * NBK);
+                k::invert_blocks256(c.main, m.L, m.lda, m.Winv, 0, m.n_pad, Wt.get(), NBK, static_cast<long>(NBK) * NBK);
+            }
+            variance_solve<VarianceF64>(c, m.L, m.lda, m.Wpanel ? m.Wpanel : Wt.get(), NBK, m.n_pad, B.get(), m_pad, mt, colsq.get());
         }
         k::variance_finish(c.main, colsq.get(), m.sek.vertical, mt, dvar.get());
         GP_HIP(hipMemcpyAsync(var, dvar.get(), sizeof(double) * mt, hipMemcpyDefault, c.main));
     }
     GP_HIP(hipMemcpyAsync(mean, dmean.get(), sizeof(double) * mt, hipMemcpyDefault, c.main));
     c.sync();
+}
+
+// Mean and the whole latent posterior covariance Sigma = K** - V^T V,
+// V = L^-1 K*^T, into cov (mt x mt, row-major). The reference forms Sigma
+// tile by tile (gen_tile_full_prior_covariance, gp_algorithms.cpp:64-88;
+// symmetric_matrix_matrix_tiled, tiled_algorithms.cpp:230-254) and returns
+// only its diagonal (gp_functions.cpp:663-695); here Sigma starts as K**
+// and every block V_J of the blocked solve subtracts V_J^T V_J on the
+// matrix cores, in step order. FP64 throughout (a mixed-precision model's
+// factor carries its own accuracy into Sigma).
+void predict_full_cov_view(const FactorView &m, const double *z, std::size_t z_len, int mt, double *mean, double *cov)
+{
+    Context &c = *m.ctx;
+    require(mt >= 0, "predict_full_cov: negative test count");
+    if (mt == 0)
+        return;
+    require(cov != nullptr, "predict_full_cov: covariance output is null");
+    predict_view(m, z, z_len, mt, mean, nullptr);
+    c.make_current();
+    const std::size_t zl = static_cast<std::size_t>(mt) + m.r - 1;
+    DevBuf<double> dz(c, zl);
+    upload(c, dz.get(), z, zl);
+    const int m_pad = static_cast<int>(round_up(mt, 64));
+    DevBuf<double> B(c, static_cast<std::size_t>(m.n_pad) * m_pad), S(c, static_cast<std::size_t>(m_pad) * m_pad);
+    {
+        Timed t(c, c.main, "assemble");
+        k::assemble_cross_t(c.main, m.x, dz.get(), m.r, m.sek, B.get(), m_pad, m.n, m.n_pad, mt, m_pad);
+        // K** (no noise term: the latent covariance), zero in the padding
+        k::assemble_cross_t(c.main, dz.get(), dz.get(), m.r, m.sek, S.get(), m_pad, mt, m_pad, mt, m_pad);
+    }
+    {
+        Timed t(c, c.main, "trsm");
+        const int NBK = m.Wpanel ? m.wblock : 256;
+        DevBuf<double> Wt;
+        if (!m.Wpanel)
+        {
+            Wt = DevBuf<double>(c, static_cast<std::size_t>(ceil_div(m.n_pad, NBK)) * NBK * NBK);
+            k::invert_blocks256(c.main, m.L, m.lda, m.Winv, 0, m.n_pad, Wt.get(), NBK, static_cast<long>(NBK) * NBK);
+        }
+        blocked_solve<VarianceF64>(c, m.L, m.lda, m.Wpanel ? m.Wpanel : Wt.get(), NBK, m.n_pad, B.get(), m_pad,
+                                   [&](hipStream_t s, const double *V, int rows)
+                                   {
+                                       // Sigma -= V_J^T V_J
+                                       k::GemmArgs g{};
+                                       g.A = V;  // A(i, k) = V_J[k][i]
+                                       g.lda = m_pad;
+                                       g.a_trans = true;
+                                       g.B = V;  // B(k, j) = V_J[k][j]
+                                       g.ldb = m_pad;
+                                       g.C = S.get();
+                                       g.ldc = m_pad;
+                                       g.M = m_pad;
+                                       g.N = m_pad;
+                                       g.K = rows;
+                                       g.role = k::GemmRole::SolveT;
+                                       k::gemm_f64(s, g);
+                                   });
+    }
+    GP_HIP(hipMemcpy2DAsync(cov, sizeof(double) * mt, S.get(), sizeof(double) * m_pad, sizeof(double) * mt, mt, hipMemcpyDefault, c.main));
+    c.sync();
+}
+
+void predict_full_cov(Model &m, const double *z, std::size_t z_len, int mt, double *mean, double *cov)
+{
+    predict_full_cov_view(FactorView{ m.ctx, m.x.get(), m.n, m.n_pad, m.r, m.lda, m.sek, m.L.get(), m.Winv.get(), m.alpha.get(),
+                                      m.precision == GPRAT_PRECISION_MIXED, m.Wpanel.get(), m.wblock },
+                          z, z_len, mt, mean, cov);
 }
 
 void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean, double *var)

@@ -189,6 +189,9 @@ void fit(Context &c,
          int precision);
 void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean, double *var);
 void predict_view(const FactorView &m, const double *z, std::size_t z_len, int mt, double *mean, double *var);
+// Mean and the whole mt x mt latent posterior covariance (row-major).
+void predict_full_cov(Model &m, const double *z, std::size_t z_len, int mt, double *mean, double *cov);
+void predict_full_cov_view(const FactorView &m, const double *z, std::size_t z_len, int mt, double *mean, double *cov);
 std::vector<int> batch_run(Context &c, const std::vector<k::BatchGP> &gps, double *losses_host);
 void batch_fit(Context &c, Batch &b, int count, const gprat_gp_desc *gps, int precision);
 void batch_losses(Batch &b, double *losses);

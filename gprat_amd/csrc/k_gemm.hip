@@ -45,6 +45,7 @@ struct Params
     int k_from_tile;
     long kstart_off;
     int set;
+    int k_to_col;
 };
 
 // Per-thread geometry of one BK-deep slab of a 128-wide operand, fixed for
@@ -184,6 +185,8 @@ __device__ __forceinline__ void gemm_tile(const Params &p, long r0, long c0, dou
         kbeg = (kbeg / BK) * BK;
     }
     long kend = p.K;
+    if (p.k_to_col)
+        kend = min(kend, c0 + BN);
     if (part != 0)
     {
         const long mid = kbeg < kend ? kbeg + ((kend - kbeg) / BK / 2) * BK : kend;
@@ -279,10 +282,13 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
     gemm_tile<AT, BT>(p, r0, c0, As, Bs);
 }
 
-// Capped launch: G workgroups loop over the row-major tiles (w, w + G, ...),
-// so the launch leaves workgroup slots free for kernels on other streams.
-// Two workgroups share a CU and their tiles take equal time, so left alone
-// they would reach their epilogues (memory-bound, no matrix-core work)
+// Capped launch: G workgroups share the tiles that hold part of the update
+// (for a lower update, only tiles touching the lower band), numbered row by
+// row: workgroup w takes tiles w, w + G, ..., so every workgroup gets the
+// same number of tiles within one and the tiles in flight together are
+// neighbours. The launch leaves workgroup slots free for kernels on other
+// streams. Two workgroups share a CU and their tiles take equal time, so left
+// alone they would reach their epilogues (memory-bound, no matrix-core work)
 // together. For C -= updates the upper half of the grid therefore starts
 // with the first half of its first tile's K range and finishes that tile's
 // second half last: its epilogues fall half a tile after the lower half's.
@@ -291,29 +297,40 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
 {
     __shared__ double As[2][BK * LDS_STRIDE];
     __shared__ double Bs[2][BK * LDS_STRIDE];
-    const long gx = (p.N + BN - 1) / BN;
-    const long tiles = gx * ((p.M + BM - 1) / BM);
-    const bool stagger = !p.set && blockIdx.x >= gridDim.x / 2;
-    long first = -1;
-    for (long t = blockIdx.x; t < tiles; t += gridDim.x)
+    const int gx = (p.N + BN - 1) / BN;
+    const int gy = (p.M + BM - 1) / BM;
+    const int G = gridDim.x;
+    const bool stagger = !p.set && static_cast<int>(blockIdx.x) >= G / 2;
+    long first_r = -1, first_c = 0;
+    int rem = blockIdx.x;  // this workgroup's next tile, counted from the current row's first
+    for (int tr = 0; tr < gy; tr++)
     {
-        const long r0 = (t / gx) * BM;
-        const long c0 = (t % gx) * BN;
-        if (p.lower && r0 + (BM - 1) + p.diag_off < c0)
-            continue;  // tile entirely above the diagonal
-        __syncthreads();  // the previous tile's last slab has been read by every wavefront
-        if (stagger && first < 0)
+        const long r0 = static_cast<long>(tr) * BM;
+        int nc = gx;
+        if (p.lower)
         {
-            first = t;
-            gemm_tile<AT, BT>(p, r0, c0, As, Bs, 1);
+            const long last = r0 + (BM - 1) + p.diag_off;  // last column the band reaches in this tile row
+            nc = last < 0 ? 0 : static_cast<int>(min(static_cast<long>(gx), last / BN + 1));
         }
-        else
-            gemm_tile<AT, BT>(p, r0, c0, As, Bs, 0);
+        for (; rem < nc; rem += G)
+        {
+            const long c0 = static_cast<long>(rem) * BN;
+            __syncthreads();  // the previous tile's last slab has been read by every wavefront
+            if (stagger && first_r < 0)
+            {
+                first_r = r0;
+                first_c = c0;
+                gemm_tile<AT, BT>(p, r0, c0, As, Bs, 1);
+            }
+            else
+                gemm_tile<AT, BT>(p, r0, c0, As, Bs, 0);
+        }
+        rem -= nc;
     }
-    if (first >= 0)
+    if (first_r >= 0)
     {
         __syncthreads();
-        gemm_tile<AT, BT>(p, (first / gx) * BM, (first % gx) * BN, As, Bs, 2);
+        gemm_tile<AT, BT>(p, first_r, first_c, As, Bs, 2);
     }
 }
 
@@ -331,10 +348,16 @@ void gemm_f64(hipStream_t s, const GemmArgs &g)
     require(reinterpret_cast<uintptr_t>(g.A) % 16 == 0 && reinterpret_cast<uintptr_t>(g.B) % 16 == 0,
             "gemm_f64: operands must be 16-byte aligned");
     Params p{ g.A, g.lda, g.B, g.ldb, g.C, g.ldc, g.M, g.N, g.K, g.lower ? 1 : 0, g.diag_off, g.k_from_tile ? 1 : 0,
-              g.kstart_off, g.set ? 1 : 0 };
+              g.kstart_off, g.set ? 1 : 0, g.k_to_col ? 1 : 0 };
     dim3 grid(static_cast<unsigned>(ceil_div(g.N, BN)), static_cast<unsigned>(ceil_div(g.M, BM)));
     require(grid.y <= 65535, "gemm_f64: M too large for one launch");
-    const long tiles = static_cast<long>(grid.x) * grid.y;
+    // tiles holding part of the update, counted as the capped kernel numbers them
+    long tiles = 0;
+    for (long tr = 0; tr < grid.y; tr++)
+    {
+        const long last = tr * BM + (BM - 1) + g.diag_off;
+        tiles += g.lower ? (last < 0 ? 0 : std::min<long>(grid.x, last / BN + 1)) : grid.x;
+    }
     if (g.max_wgs > 0 && g.max_wgs < tiles)
     {
         // only the trailing update runs capped

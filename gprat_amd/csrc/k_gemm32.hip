@@ -121,8 +121,10 @@ __device__ __forceinline__ void store_quarter32(const Params32<TC> &p, const v4f
     }
 }
 
+// part 0: the tile's whole K range; parts 1 and 2: its first and second
+// half (in slabs), for C -= updates only.
 template <class TC>
-__device__ __forceinline__ void gemm_tile32(const Params32<TC> &p, long r0, long c0, float (*As)[BK * KS], float (*Bs)[BK * KS])
+__device__ __forceinline__ void gemm_tile32(const Params32<TC> &p, long r0, long c0, float (*As)[BK * KS], float (*Bs)[BK * KS], int part = 0)
 {
     const int rows_a = static_cast<int>(min(static_cast<long>(BM), p.M - r0));
     const int rows_b = static_cast<int>(min(static_cast<long>(BM), p.N - c0));
@@ -143,10 +145,14 @@ __device__ __forceinline__ void gemm_tile32(const Params32<TC> &p, long r0, long
     int la[4], lb[4];
     slab_offsets(p.lda, rows_a, tid, oa, la);
     slab_offsets(p.ldb, rows_b, tid, ob, lb);
-    const float *ak = p.A + r0;
-    const float *bk = p.B + c0;
+    const int half = (p.K / BK / 2) * BK;
+    const int k0 = part == 2 ? half : 0, k1 = part == 1 ? half : p.K;
+    const float *ak = p.A + r0 + static_cast<long>(k0) * p.lda;
+    const float *bk = p.B + c0 + static_cast<long>(k0) * p.ldb;
     const long astep = BK * p.lda, bstep = BK * p.ldb;
 
+    if (k1 > k0)
+    {
     f4 ra[4], rb[4];
 #pragma unroll
     for (int s = 0; s < 4; s++)
@@ -163,7 +169,7 @@ __device__ __forceinline__ void gemm_tile32(const Params32<TC> &p, long r0, long
     __syncthreads();
 
     int buf = 0;
-    const int steps = p.K / BK;
+    const int steps = (k1 - k0) / BK;
     for (int t = 1; t < steps; t++)
     {
         ak += astep;
@@ -188,6 +194,7 @@ __device__ __forceinline__ void gemm_tile32(const Params32<TC> &p, long r0, long
         buf ^= 1;
     }
     mfma_slab32(As[buf], Bs[buf], wr, wc, lm, lk, acc);
+    }
 
     // Accumulator layout of the 16x16x4 f32 MFMA on gfx950: lane holds
     // column (lane % 16) and rows 4 (lane / 16) + e, e = 0..3.
@@ -208,24 +215,64 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2))) vo
 {
     __shared__ float As[2][BK * KS];
     __shared__ float Bs[2][BK * KS];
-    const long gx = (p.N + BM - 1) / BM;
-    const long tiles = gx * ((p.M + BM - 1) / BM);
-    for (long t = blockIdx.x; t < tiles; t += gridDim.x)
+    const int gx = (p.N + BM - 1) / BM;
+    const int gy = (p.M + BM - 1) / BM;
+    if (p.max_wgs <= 0)
     {
-        const long r0 = (t / gx) * BM;
-        const long c0 = (t % gx) * BM;
+        // one workgroup per tile, row-major; tiles above the band do nothing
+        const long r0 = static_cast<long>(blockIdx.x / gx) * BM, c0 = static_cast<long>(blockIdx.x % gx) * BM;
         if (p.lower && r0 + (BM - 1) + p.diag_off < c0)
-            continue;  // tile entirely above the diagonal
-        __syncthreads();  // the previous tile's last slab has been read by every wavefront
-        gemm_tile32<TC>(p, r0, c0, As, Bs);
+            return;
+        gemm_tile32<TC>(p, r0, c0, As, Bs, 0);
+        return;
+    }
+    // Capped: the tiles holding part of the update (for a lower update, those
+    // touching the band) are numbered row by row and workgroup w takes w,
+    // w + G, ...: equal shares within one tile, neighbours in flight
+    // together. The upper half of the grid starts with the first K half of
+    // its first tile and ends with the second, so the two workgroups of a CU
+    // do not reach their epilogues together.
+    const int G = gridDim.x;
+    const bool stagger = !p.set && static_cast<int>(blockIdx.x) >= G / 2;
+    long first_r = -1, first_c = 0;
+    int rem = blockIdx.x;  // this workgroup's next tile, counted from the current row's first
+    for (int tr = 0; tr < gy; tr++)
+    {
+        const long r0 = static_cast<long>(tr) * BM;
+        int nc = gx;
+        if (p.lower)
+        {
+            const long last = r0 + (BM - 1) + p.diag_off;
+            nc = last < 0 ? 0 : static_cast<int>(min(static_cast<long>(gx), last / BM + 1));
+        }
+        for (; rem < nc; rem += G)
+        {
+            const long c0 = static_cast<long>(rem) * BM;
+            __syncthreads();  // the previous tile's last slab has been read by every wavefront
+            if (stagger && first_r < 0)
+            {
+                first_r = r0;
+                first_c = c0;
+                gemm_tile32<TC>(p, r0, c0, As, Bs, 1);
+            }
+            else
+                gemm_tile32<TC>(p, r0, c0, As, Bs, 0);
+        }
+        rem -= nc;
+    }
+    if (first_r >= 0)
+    {
+        __syncthreads();
+        gemm_tile32<TC>(p, first_r, first_c, As, Bs, 2);
     }
 }
 
 // P[k][r] = (T) A[(r0 + r) * lda + p0 + k] for r < R, k < W, through a
-// 64x64 LDS tile so both the FP64 reads and the k-major writes are coalesced.
+// 64x64 LDS tile so both the FP64 reads and the k-major writes are coalesced;
+// with `back`, the same FP64 values are also copied to back[r * ldback + k].
 template <class T>
 __global__ __launch_bounds__(THREADS) void k_panel_to_t(const double *__restrict__ A, long lda, long r0, long p0, int R, int W,
-                                                         T *__restrict__ P, long ldp)
+                                                         T *__restrict__ P, long ldp, double *__restrict__ back, long ldback)
 {
     __shared__ T t[64][65];
     const long rb = static_cast<long>(blockIdx.x) * 64, kb = static_cast<long>(blockIdx.y) * 64;
@@ -233,7 +280,10 @@ __global__ __launch_bounds__(THREADS) void k_panel_to_t(const double *__restrict
     for (int rr = q; rr < 64; rr += 4)
     {
         const long r = rb + rr, k = kb + c;
-        t[rr][c] = (r < R && k < W) ? static_cast<T>(A[(r0 + r) * lda + p0 + k]) : T(0);
+        const double v = (r < R && k < W) ? A[(r0 + r) * lda + p0 + k] : 0.0;
+        t[rr][c] = static_cast<T>(v);
+        if (back && r < R && k < W)
+            back[r * ldback + k] = v;
     }
     __syncthreads();
     for (int kk = q; kk < 64; kk += 4)
@@ -247,25 +297,26 @@ __global__ __launch_bounds__(THREADS) void k_panel_to_t(const double *__restrict
 }  // namespace
 
 template <class T>
-static void panel_to_t(hipStream_t s, const double *A, long lda, long r0, long p0, int R, int W, T *P, long ldp)
+static void panel_to_t(hipStream_t s, const double *A, long lda, long r0, long p0, int R, int W, T *P, long ldp, double *back, long ldback)
 {
     require(R >= 0 && W >= 0 && ldp >= R, "panel copy: bad shape");
     if (R == 0 || W == 0)
         return;
     dim3 grid(static_cast<unsigned>(ceil_div(R, 64)), static_cast<unsigned>(ceil_div(W, 64)));
     require(grid.y <= 65535, "panel copy: panel too wide");
-    hipLaunchKernelGGL(k_panel_to_t<T>, grid, dim3(THREADS), 0, s, A, lda, r0, p0, R, W, P, ldp);
+    require(!back || ldback >= W, "panel copy: bad copy-back layout");
+    hipLaunchKernelGGL(k_panel_to_t<T>, grid, dim3(THREADS), 0, s, A, lda, r0, p0, R, W, P, ldp, back, ldback);
     GP_CHECK_LAUNCH();
 }
 
-void panel_to_f32t(hipStream_t s, const double *A, long lda, long r0, long p0, int R, int W, float *P, long ldp)
+void panel_to_f32t(hipStream_t s, const double *A, long lda, long r0, long p0, int R, int W, float *P, long ldp, double *back, long ldback)
 {
-    panel_to_t<float>(s, A, lda, r0, p0, R, W, P, ldp);
+    panel_to_t<float>(s, A, lda, r0, p0, R, W, P, ldp, back, ldback);
 }
 
-void panel_to_f64t(hipStream_t s, const double *A, long lda, long r0, long p0, int R, int W, double *P, long ldp)
+void panel_to_f64t(hipStream_t s, const double *A, long lda, long r0, long p0, int R, int W, double *P, long ldp, double *back, long ldback)
 {
-    panel_to_t<double>(s, A, lda, r0, p0, R, W, P, ldp);
+    panel_to_t<double>(s, A, lda, r0, p0, R, W, P, ldp, back, ldback);
 }
 
 template <class TC, int ROLE>
@@ -278,10 +329,22 @@ static void launch_f32_tn(hipStream_t s, const Params32<TC> &p)
                 && reinterpret_cast<uintptr_t>(p.A) % 16 == 0 && reinterpret_cast<uintptr_t>(p.B) % 16 == 0,
             "FP32 product: operand rows must come in aligned groups of four");
     require(static_cast<long>(BK) * std::max(p.lda, p.ldb) + BM < (1L << 31), "FP32 product: operand too tall");
-    const long tiles = ceil_div(static_cast<long>(p.N), static_cast<long>(BM)) * ceil_div(static_cast<long>(p.M), static_cast<long>(BM));
-    require(tiles < (1L << 31), "FP32 product: too many tiles for one launch");
-    dim3 grid(static_cast<unsigned>(p.max_wgs > 0 ? std::min<long>(tiles, p.max_wgs) : tiles));
-    hipLaunchKernelGGL((k_gemm_f32_tn<TC, ROLE>), grid, dim3(THREADS), 0, s, p);
+    const long gx = ceil_div(static_cast<long>(p.N), static_cast<long>(BM)), gy = ceil_div(static_cast<long>(p.M), static_cast<long>(BM));
+    require(gx * gy < (1L << 31), "FP32 product: too many tiles for one launch");
+    // tiles holding part of the update, counted as the capped kernel numbers them
+    long active = 0;
+    for (long tr = 0; tr < gy; tr++)
+    {
+        const long last = tr * BM + (BM - 1) + p.diag_off;
+        active += p.lower ? (last < 0 ? 0 : std::min(gx, last / BM + 1)) : gx;
+    }
+    if (active == 0)
+        return;
+    Params32<TC> q = p;
+    if (q.max_wgs >= active)
+        q.max_wgs = 0;  // the cap does not bind: one workgroup per tile
+    dim3 grid(static_cast<unsigned>(q.max_wgs > 0 ? q.max_wgs : gx * gy));
+    hipLaunchKernelGGL((k_gemm_f32_tn<TC, ROLE>), grid, dim3(THREADS), 0, s, q);
     GP_CHECK_LAUNCH();
 }
 
@@ -292,9 +355,10 @@ void gemm_f32_trailing(hipStream_t s, const float *P, long ldp, long a_off, long
     launch_f32_tn<double, 0>(s, p);
 }
 
-void gemm_f32_tn(hipStream_t s, const float *A, long lda, const float *B, long ldb, float *C, long ldc, int M, int N, int K, bool set)
+void gemm_f32_tn(hipStream_t s, const float *A, long lda, const float *B, long ldb, float *C, long ldc, int M, int N, int K, bool set,
+                 int max_wgs)
 {
-    Params32<float> p{ A, lda, B, ldb, C, ldc, M, N, K, 0, 0, set ? 1 : 0, 0 };
+    Params32<float> p{ A, lda, B, ldb, C, ldc, M, N, K, 0, 0, set ? 1 : 0, max_wgs };
     if (set)
         launch_f32_tn<float, 1>(s, p);
     else

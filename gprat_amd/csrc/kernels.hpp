@@ -119,6 +119,9 @@ struct GemmArgs
     // are known zero and skipped (triangular operands, e.g. W^T W).
     bool k_from_tile;
     long kstart_off;
+    // B(k, j) is known zero for k > j (a lower-triangular matrix seen as
+    // B(k, j) = W(j, k)): each column tile stops K at its last column.
+    bool k_to_col;
     // true: C = acc; false: C -= acc.
     bool set;
     GemmRole role;
@@ -131,9 +134,11 @@ void gemm_f64(hipStream_t s, const GemmArgs &g);
 // ---- FP32 matrix-core trailing update, k-major panel copies (k_gemm32.hip) --
 
 // P[k][r] = (float) A[(r0 + r) * lda + p0 + k], r < R, k < W (k-major copy).
-void panel_to_f32t(hipStream_t s, const double *A, long lda, long r0, long p0, int R, int W, float *P, long ldp);
+void panel_to_f32t(hipStream_t s, const double *A, long lda, long r0, long p0, int R, int W, float *P, long ldp, double *back = nullptr,
+                   long ldback = 0);
 // The same copy in FP64: the A^T operand layout of gemm_f64 (a_trans).
-void panel_to_f64t(hipStream_t s, const double *A, long lda, long r0, long p0, int R, int W, double *P, long ldp);
+void panel_to_f64t(hipStream_t s, const double *A, long lda, long r0, long p0, int R, int W, double *P, long ldp, double *back = nullptr,
+                   long ldback = 0);
 // C[i][j] -= sum_k P[k][a_off + i] P[k][b_off + j] (FP32 products and sums),
 // for i < M, j < N, i + diag_off >= j; K % 32 == 0.
 // max_wgs > 0 caps the launch (workgroups loop over tiles).
@@ -141,7 +146,8 @@ void gemm_f32_trailing(hipStream_t s, const float *P, long ldp, long a_off, long
                        int max_wgs = 0);
 // C (= or -=) A B over FP32 operands, A(i,k) = A[k*lda + i], B(k,j) = B[k*ldb + j],
 // FP32 C; K % 32 == 0, M, N, lda, ldb multiples of 4, A and B 16-byte aligned.
-void gemm_f32_tn(hipStream_t s, const float *A, long lda, const float *B, long ldb, float *C, long ldc, int M, int N, int K, bool set);
+void gemm_f32_tn(hipStream_t s, const float *A, long lda, const float *B, long ldb, float *C, long ldc, int M, int N, int K, bool set,
+                 int max_wgs = 0);
 
 // ---- batched fit of many GPs (k_batch.hip) --------------------------------
 

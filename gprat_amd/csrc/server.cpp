@@ -193,9 +193,10 @@ std::size_t Server::fit_need(const Entry &e) const
     {
         np = padded(e.n);
         ld = leading_dim(static_cast<long>(np));
-        // Cholesky panel copies (two of at most 512 x n_pad doubles), solve
-        // and refinement vectors, the refinement's partial sums
-        temp = 8 * (2 * 512 * np + 8 * np + static_cast<std::size_t>(k::mean_splits(e.n, e.n)) * n);
+        // Cholesky panel copy and panel scratch (at most 512 x n_pad doubles
+        // each), the diagonal-block scratch, solve and refinement vectors,
+        // the refinement's partial sums
+        temp = 8 * (3 * 512 * np + 8 * np + static_cast<std::size_t>(k::mean_splits(e.n, e.n)) * n);
     }
     // factor, 64-block inverses, panel inverses (at most 512 wide), vectors
     const std::size_t resident = 8 * (np * ld + np * 64 + (e.batched_path ? 0 : np * 512) + 2 * np + n + r);
@@ -208,7 +209,9 @@ std::size_t Server::predict_need(const Entry &e, int m) const
     const std::size_t mp = round_up(m, 64);
     const std::size_t blocks = (np + 255) / 256;
     const std::size_t splits = static_cast<std::size_t>(k::mean_splits(e.n, m));
-    return 8 * (np * mp + 256 * mp + 4 * mp + blocks * 65536 + 256 * np + splits * m + m + e.r) + (std::size_t(1) << 20);
+    // K*^T, two staged V blocks and two panel copies (at most 512 wide), block
+    // inverses built for a factor without kept ones, the mean's partial sums
+    return 8 * (np * mp + 2 * 512 * mp + 4 * mp + blocks * 65536 + 2 * 512 * np + splits * m + m + e.r) + (std::size_t(1) << 20);
 }
 
 // Estimated flops of a request: the factorisation n^3/3 for a fit (or a

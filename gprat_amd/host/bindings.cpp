@@ -173,6 +173,8 @@ PYBIND11_MODULE(gprat, m)
              py::arg("m_tile_size"), py::call_guard<py::gil_scoped_release>())
         .def("predict_with_full_cov", &gprat::GP::predict_with_full_cov, py::arg("test_data"), py::arg("m_tiles"),
              py::arg("m_tile_size"), py::call_guard<py::gil_scoped_release>())
+        .def("predict_full_posterior", &gprat::GP::predict_full_posterior, py::arg("test_data"), py::arg("m_tiles"),
+             py::arg("m_tile_size"), py::call_guard<py::gil_scoped_release>())
         .def("optimize", &gprat::GP::optimize, py::arg("AdamParams"), py::call_guard<py::gil_scoped_release>())
         .def("optimize_step", &gprat::GP::optimize_step, py::arg("AdamParams"), py::arg("iter"),
              py::call_guard<py::gil_scoped_release>())

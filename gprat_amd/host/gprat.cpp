@@ -327,6 +327,18 @@ std::vector<std::vector<double>> GP::predict_with_full_cov(const std::vector<dou
     return { std::move(mean), std::move(var) };
 }
 
+std::vector<std::vector<double>> GP::predict_full_posterior(const std::vector<double> &test, int m_tiles, int m_tile_size)
+{
+    const std::size_t m = test_count(m_tiles, m_tile_size);
+    std::vector<double> mean(m), cov(m * m);
+    const double theta[3] = { kernel_params.lengthscale, kernel_params.vertical_lengthscale, kernel_params.noise_variance };
+    check(gprat_predict_full_posterior(gpu_context(), training_input_.data(), training_input_.size(), training_output_.data(),
+                                       training_output_.size(), test.data(), test.size(), theta, n_tiles_, n_tile_size_, m_tiles,
+                                       m_tile_size, n_reg, mean.data(), cov.data()),
+          "predict_full_posterior");
+    return { std::move(mean), std::move(cov) };
+}
+
 double GP::loss_and_gradient(double grad[3])
 {
     const double theta[3] = { kernel_params.lengthscale, kernel_params.vertical_lengthscale, kernel_params.noise_variance };

@@ -161,6 +161,11 @@ class GP
     predict_with_uncertainty(const std::vector<double> &test_data, int m_tiles, int m_tile_size);
     std::vector<std::vector<double>>
     predict_with_full_cov(const std::vector<double> &test_data, int m_tiles, int m_tile_size);
+    // {mean, the whole m x m latent posterior covariance, row-major}: the
+    // matrix the reference's predict_with_full_cov forms and reduces to its
+    // diagonal (core/src/cpu/gp_functions.cpp:452-696).
+    std::vector<std::vector<double>>
+    predict_full_posterior(const std::vector<double> &test_data, int m_tiles, int m_tile_size);
     std::vector<double> optimize(const gprat_hyper::AdamParams &adam_params);
     double optimize_step(gprat_hyper::AdamParams &adam_params, int iter);
     double calculate_loss();

@@ -159,6 +159,28 @@ int gprat_predict_with_full_cov(gprat_context *ctx,
                                 double *mean_out,
                                 double *var_out);
 
+/* Mean and the whole m x m latent posterior covariance
+ * Sigma = K** - K* K^-1 K*^T (row-major; cov_out holds m * m doubles).
+ * The reference forms this matrix in predict_with_full_cov
+ * (core/src/cpu/gp_functions.cpp:452-696, tiled_algorithms.cpp:230-254) but
+ * returns only its diagonal; same arguments as gprat_predict_with_full_cov
+ * otherwise. */
+int gprat_predict_full_posterior(gprat_context *ctx,
+                                 const double *train_in,
+                                 size_t train_in_len,
+                                 const double *train_out,
+                                 size_t train_out_len,
+                                 const double *test_in,
+                                 size_t test_in_len,
+                                 const double theta[3],
+                                 int n_tiles,
+                                 int n_tile_size,
+                                 int m_tiles,
+                                 int m_tile_size,
+                                 int n_reg,
+                                 double *mean_out,
+                                 double *cov_out);
+
 /* Replaces gpu::compute_loss (gp_functions.cuh:108-114): the normalised
  * negative log marginal likelihood
  * 0.5 * (y^T K^-1 y + sum_i log(L_ii^2) + n log(2 pi)) / n. */
@@ -216,6 +238,15 @@ int gprat_model_predict(gprat_model *model,
                         int m,
                         double *mean_out,
                         double *var_out);
+
+/* Mean and the whole m x m latent posterior covariance (row-major, m * m
+ * doubles) at m test points; see gprat_predict_full_posterior. */
+int gprat_model_predict_full_cov(gprat_model *model,
+                                 const double *test_in,
+                                 size_t test_in_len,
+                                 int m,
+                                 double *mean_out,
+                                 double *cov_out);
 
 /* Loss at the fitted hyperparameters. */
 int gprat_model_loss(gprat_model *model, double *loss_out);

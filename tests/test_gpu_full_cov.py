@@ -1,0 +1,102 @@
+"""Whole latent posterior covariance Sigma = K** - K* K^-1 K*^T. The
+reference forms this matrix in predict_with_full_cov
+(core/src/cpu/gp_functions.cpp:452-696) but returns only its diagonal, so the
+oracle has no full matrix to compare with: Sigma is pinned by the closed form
+evaluated densely in numpy (LAPACK Cholesky and solves, independent of this
+library's tiling and kernels), and its diagonal by the variance of
+predict_with_uncertainty, which the oracle pins. Bar: 1e-10 normwise."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import norm_rel
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _dense_sigma(x, z, n, m, r, theta):
+    l, v, s2 = theta
+    X = np.array([x[i:i + r] for i in range(n)])
+    Z = np.array([z[j:j + r] for j in range(m)])
+
+    def se(P, Q):
+        return v * np.exp(-0.5 / (l * l) * ((P[:, None, :] - Q[None, :, :]) ** 2).sum(-1))
+
+    K = se(X, X) + s2 * np.eye(n)
+    Ks = se(Z, X)
+    L = np.linalg.cholesky(K)
+    V = np.linalg.solve(L, Ks.T)
+    return se(Z, Z) - V.T @ V
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from gprat_amd import capi
+    c = capi.Context(0, 1)
+    yield c
+    c.close()
+
+
+# (n, m, n_reg, theta): ragged n and m, a single test point, n_reg 16 with
+# a small noise (condition number ~1e5)
+CASES = [
+    (300, 70, 8, (1.3, 0.7, 0.05)),
+    (1000, 257, 4, (1.0, 1.0, 0.1)),
+    (640, 1, 16, (2.0, 1.5, 0.01)),
+]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_model_full_cov_matches_closed_form(ctx, case):
+    from gprat_amd import capi, datagen
+    n, m, r, theta = case
+    x, y, z = datagen.gp_problem(n, m, r)
+    mdl = capi.Model(ctx, x, y, n, r, theta)
+    try:
+        mean, cov = mdl.predict_full_cov(z, m)
+        mean2, var = mdl.predict(z, m)
+    finally:
+        mdl.release()
+    sigma = _dense_sigma(x, z, n, m, r, theta)
+    assert cov.shape == (m, m)
+    assert norm_rel(cov, sigma) < TOL
+    assert np.array_equal(mean, mean2)
+    assert norm_rel(np.diag(cov), var) < 1e-12
+    assert np.max(np.abs(cov - cov.T)) <= 1e-13 * np.max(np.abs(cov))
+
+
+def test_stateless_full_posterior_diagonal_matches_full_cov(ctx):
+    """The tiled entry point's matrix has the diagonal that
+    predict_with_full_cov returns (the reference's contract)."""
+    from gprat_amd import datagen
+    T, b, m_tiles, m_b, r = 4, 256, 4, 64, 8
+    x, y, z = datagen.gp_problem(T * b, m_tiles * m_b, r)
+    theta = (1.0, 1.0, 0.1)
+    mean, cov = ctx.predict_full_posterior(x, y, z, theta, T, b, m_tiles, m_b, r)
+    mean2, var = ctx.predict(x, y, z, theta, T, b, m_tiles, m_b, r, full_cov=True)
+    assert np.array_equal(mean, mean2)
+    assert norm_rel(np.diag(cov), var) < 1e-12
+    assert norm_rel(cov, _dense_sigma(x, z, T * b, m_tiles * m_b, r, theta)) < TOL
+
+
+def test_facade_full_posterior(golden_dir):
+    """Through the drop-in Python API on the reference's golden problem."""
+    import gprat_amd as gprat
+    n_train = n_test = 128
+    n_tiles, n_reg = 4, 8
+    ts = gprat.compute_train_tile_size(n_train, n_tiles)
+    mt = gprat.compute_test_tiles(n_test, n_tiles, ts)
+    xi = gprat.GP_data(os.path.join(golden_dir, "training_input.txt"), n_train, n_reg)
+    yo = gprat.GP_data(os.path.join(golden_dir, "training_output.txt"), n_train, n_reg)
+    zi = gprat.GP_data(os.path.join(golden_dir, "test_input.txt"), n_test, n_reg)
+    gp = gprat.GP(xi.data, yo.data, n_tiles, ts, n_reg, [1.0, 1.0, 0.1], [True, True, True], 0, 1)
+    mean, cov = gp.predict_full_posterior(zi.data, *mt)
+    fmean, fvar = gp.predict_with_full_cov(zi.data, *mt)
+    cov = np.asarray(cov).reshape(n_test, n_test)
+    assert np.array_equal(np.asarray(mean), np.asarray(fmean))
+    assert norm_rel(np.diag(cov), fvar) < 1e-12
+    sigma = _dense_sigma(np.asarray(xi.data), np.asarray(zi.data), n_train, n_test, n_reg, (1.0, 1.0, 0.1))
+    assert norm_rel(cov, sigma) < TOL
