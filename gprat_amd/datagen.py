@@ -62,3 +62,27 @@ def gp_problem(n_train: int, n_test: int, n_reg: int, seed_train: int = 0, seed_
     u, y = msd_series(n_train, seed_train)
     ut, _ = msd_series(max(n_test, 1), seed_test)
     return padded_input(u, n_reg), y, padded_input(ut[:n_test], n_reg)
+
+
+def write_dataset(directory: str, n_train: int, n_test: int, seed_train: int = 0, seed_test: int = 1) -> None:
+    """Write a seeded dataset in the layout and text format of the
+    reference's data directories (data/data_1024: training_input.txt,
+    training_output.txt, test_input.txt, test_output.txt, one value per line,
+    read by GP_data / utils::load_data with "%lf", core/src/utils_c.cpp:54-81).
+    The series are unpadded on disk; GP_data adds the n_reg - 1 leading
+    zeros when it loads them."""
+    import os
+    os.makedirs(directory, exist_ok=True)
+    u, y = msd_series(n_train, seed_train)
+    ut, yt = msd_series(n_test, seed_test)
+    for name, values in (("training_input", u), ("training_output", y), ("test_input", ut), ("test_output", yt)):
+        with open(os.path.join(directory, name + ".txt"), "w") as f:
+            f.writelines(f"{v!r}\n" for v in np.asarray(values, dtype=np.float64).tolist())
+
+
+if __name__ == "__main__":
+    import sys
+    if len(sys.argv) not in (4, 6):
+        sys.exit("usage: python -m gprat_amd.datagen DIRECTORY N_TRAIN N_TEST [SEED_TRAIN SEED_TEST]")
+    seeds = (int(sys.argv[4]), int(sys.argv[5])) if len(sys.argv) == 6 else (0, 1)
+    write_dataset(sys.argv[1], int(sys.argv[2]), int(sys.argv[3]), *seeds)

@@ -46,3 +46,27 @@ def test_properties():
     u2, _ = datagen.msd_series(50, 3)
     u3, _ = datagen.msd_series(10, 3)
     assert np.array_equal(u2[:10], u3)
+
+
+def test_written_dataset_reads_back_through_gp_data(tmp_path):
+    """The seeded dataset writer produces the reference's data-directory
+    files, and GP_data (utils_c.cpp:54-81 semantics) reads them back exactly,
+    with the n_reg - 1 leading zeros."""
+    import gprat_amd as gprat
+    from gprat_amd import datagen
+    n_train, n_test, n_reg = 300, 40, 8
+    datagen.write_dataset(str(tmp_path), n_train, n_test)
+    names = sorted(os.listdir(tmp_path))
+    assert names == ["test_input.txt", "test_output.txt", "training_input.txt", "training_output.txt"]
+    u, y = datagen.msd_series(n_train, 0)
+    ut, yt = datagen.msd_series(n_test, 1)
+    for name, ref, n in (("training_input", u, n_train), ("training_output", y, n_train),
+                         ("test_input", ut, n_test), ("test_output", yt, n_test)):
+        d = gprat.GP_data(os.path.join(str(tmp_path), name + ".txt"), n, n_reg)
+        got = np.asarray(d.data)
+        assert got.shape == (n + n_reg - 1,)
+        assert np.all(got[:n_reg - 1] == 0.0)
+        assert np.array_equal(got[n_reg - 1:], ref)
+    x, _, z = datagen.gp_problem(n_train, n_test, n_reg)
+    assert np.array_equal(np.asarray(gprat.GP_data(os.path.join(str(tmp_path), "training_input.txt"), n_train, n_reg).data), x)
+    assert np.array_equal(np.asarray(gprat.GP_data(os.path.join(str(tmp_path), "test_input.txt"), n_test, n_reg).data), z)
