@@ -542,50 +542,6 @@ void solve_spd(Context &c, const double *L, long lda, const double *Winv, int n_
     GP_HIP(hipMemcpyAsync(out, work, sizeof(double) * n_pad, hipMemcpyDeviceToDevice, s));
 }
 
-// B[a:b, :ncols] <- L[a:b, a:b]^-1 B[a:b, :ncols] by recursive halving; the
-// off-diagonal coupling is one MFMA product per level.
-// B[a:b, :ncols] <- L[a:b, a:b]^-1 B[a:b, :ncols] by recursive halving: the
-// 64-row leaves multiply by the saved diagonal-block inverse (in place), the
-// coupling between halves is one MFMA product per level. With tri_rhs the
-// right-hand side is lower triangular (B = I for L^-1), so rows [a, b) only
-// have non-zero columns below b and every step's column range is cut to it.
-void trsm_lower_left(Context &c, const double *L, long lda, const double *Winv, double *B, long ldb, int ncols, int a, int b, bool tri_rhs)
-{
-    if (b - a == 64)
-    {
-        k::GemmArgs g{};
-        g.A = Winv + static_cast<long>(a / 64) * 64 * 64;
-        g.lda = 64;
-        g.B = B + static_cast<long>(a) * ldb;
-        g.ldb = ldb;
-        g.C = B + static_cast<long>(a) * ldb;
-        g.ldc = ldb;
-        g.M = 64;
-        g.N = tri_rhs ? std::min(ncols, b) : ncols;
-        g.K = 64;
-        g.set = true;
-        g.role = k::GemmRole::Solve;
-        k::gemm_f64(c.main, g);
-        return;
-    }
-    const int blocks = (b - a) / 64;
-    const int mid = a + 64 * ((blocks + 1) / 2);
-    trsm_lower_left(c, L, lda, Winv, B, ldb, ncols, a, mid, tri_rhs);
-    k::GemmArgs g{};
-    g.A = L + static_cast<long>(mid) * lda + a;
-    g.lda = lda;
-    g.B = B + static_cast<long>(a) * ldb;
-    g.ldb = ldb;
-    g.C = B + static_cast<long>(mid) * ldb;
-    g.ldc = ldb;
-    g.M = b - mid;
-    g.N = tri_rhs ? std::min(ncols, mid) : ncols;
-    g.K = mid - a;
-    g.role = k::GemmRole::Solve;
-    k::gemm_f64(c.main, g);
-    trsm_lower_left(c, L, lda, Winv, B, ldb, ncols, mid, b, tri_rhs);
-}
-
 // out = L^-1 B, blocked right-looking in NBK-wide steps: the diagonal step
 // out_J = W_J B_J uses the k-major inverse of the diagonal block (the
 // factorisation's panel inverses when given, NBK = their width, else
@@ -593,11 +549,9 @@ void trsm_lower_left(Context &c, const double *L, long lda, const double *Winv, 
 // reads a k-major copy of L's column panel, so both products take whole
 // 1 KiB rows of their A operand. B is overwritten. With tri_rhs, B is lower
 // triangular (B = I for L^-1) and each step's columns stop at the end of
-// its block. With `staged`, out is a single NBK-row block reused by every
-// step (only the squares are wanted); colsq, when given, accumulates the
-// column sums of squares of out over rows, for columns < mt.
+// its block.
 void trsm_blocked(Context &c, const double *L, long lda, const double *Winv, const double *Wpanel, int wblock, int n_pad, double *B,
-                  long ldb, int ncols, bool tri_rhs, double *out, long ldo, bool staged, double *colsq, int mt)
+                  long ldb, int ncols, bool tri_rhs, double *out, long ldo)
 {
     hipStream_t s = c.main;
     const int NBK = Wpanel ? wblock : 256;
@@ -615,7 +569,7 @@ void trsm_blocked(Context &c, const double *L, long lda, const double *Winv, con
     {
         const int j0 = J * NBK, nb = std::min(NBK, n_pad - j0), j1 = j0 + nb;
         const int cols = tri_rhs ? std::min(ncols, j1) : ncols;
-        double *oJ = staged ? out : out + static_cast<long>(j0) * ldo;
+        double *oJ = out + static_cast<long>(j0) * ldo;
         {
             k::GemmArgs g{};
             g.A = W + static_cast<long>(J) * NBK * NBK;
@@ -632,8 +586,6 @@ void trsm_blocked(Context &c, const double *L, long lda, const double *Winv, con
             g.role = k::GemmRole::LeafT;
             k::gemm_f64(s, g);
         }
-        if (colsq)
-            k::colsq_accumulate(s, oJ, ldo, nb, mt, colsq);
         if (j1 >= n_pad)
             break;
         k::panel_to_f64t(s, L, lda, j1, j0, n_pad - j1, nb, Pt.get(), n_pad);
@@ -1236,7 +1188,7 @@ void model_gradient(Model &m, const int trainable[3], double *loss_out, double g
     {
         Timed tm(c, c.main, "inverse");
         trsm_blocked(c, m.L.get(), m.lda, m.Winv.get(), m.Wpanel.get(), m.wblock, m.n_pad, Kinv.get(), m.lda, m.n_pad, true, W.get(),
-                     m.lda, false, nullptr, 0);
+                     m.lda);
         k::GemmArgs g{};
         g.A = W.get();
         g.lda = m.lda;

@@ -171,9 +171,8 @@ int panel_width(int n_pad);
 void refine_alpha(Model &m, double *work);
 void solve_spd(Context &c, const double *L, long lda, const double *Winv, int n_pad, const double *y, double *work, double *out,
                const double *Wpanel = nullptr, int wblock = 0);
-void trsm_lower_left(Context &c, const double *L, long lda, const double *Winv, double *B, long ldb, int ncols, int a, int b, bool tri_rhs);
 void trsm_blocked(Context &c, const double *L, long lda, const double *Winv, const double *Wpanel, int wblock, int n_pad, double *B,
-                  long ldb, int ncols, bool tri_rhs, double *out, long ldo, bool staged, double *colsq, int mt);
+                  long ldb, int ncols, bool tri_rhs, double *out, long ldo);
 
 void factor(Context &c, Model &m, const double *x, std::size_t x_len, int n, int r, const double theta[3], int precision);
 void attach_targets(Model &m, const double *y, std::size_t y_len);

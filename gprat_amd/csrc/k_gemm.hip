@@ -10,9 +10,10 @@
 // steady-state loop has no branch and no predicated load, so the
 // accumulators stay in place and two workgroups share a CU.
 //
-// The same kernel serves the Cholesky trailing update (A = B^T = panel,
-// lower triangle only), the panel-internal updates, the right-looking TRSM
-// updates (B not transposed) and W^T W (A transposed, triangular K range).
+// The same kernel serves the Cholesky trailing update (both operands the
+// k-major panel copy, lower triangle only), the panel rows times the
+// diagonal block's inverse (both operands row-major), the blocked TRSM's
+// diagonal steps and updates, and W^T W (triangular K range).
 #include "kernels.hpp"
 
 namespace gpamd::k
@@ -366,28 +367,20 @@ void gemm_f64(hipStream_t s, const GemmArgs &g)
         GP_CHECK_LAUNCH();
         return;
     }
-    // Instantiated combinations: (N,T) trailing/panel, (N,N) solve, (T,N) inverse
-    // and the updates fed by k-major panel copies.
-    if (!g.a_trans && g.b_trans && g.role == GemmRole::Trailing)
-        hipLaunchKernelGGL((k_gemm_f64<false, true, 0>), grid, dim3(THREADS), 0, s, p);
-    else if (!g.a_trans && g.b_trans && g.role == GemmRole::PanelSolve)
-        hipLaunchKernelGGL((k_gemm_f64<false, true, 4>), grid, dim3(THREADS), 0, s, p);
-    else if (!g.a_trans && g.b_trans)
-        hipLaunchKernelGGL((k_gemm_f64<false, true, 1>), grid, dim3(THREADS), 0, s, p);
-    else if (!g.a_trans && !g.b_trans && g.role == GemmRole::PanelX)
+    // Instantiated combinations: (N,N) panel rows; (T,N) with both operands
+    // k-major for the updates, the TRSM steps and W^T W.
+    if (!g.a_trans && !g.b_trans && g.role == GemmRole::PanelX)
         hipLaunchKernelGGL((k_gemm_f64<false, false, 8>), grid, dim3(THREADS), 0, s, p);
-    else if (!g.a_trans && !g.b_trans)
-        hipLaunchKernelGGL((k_gemm_f64<false, false, 2>), grid, dim3(THREADS), 0, s, p);
     else if (g.a_trans && !g.b_trans && g.role == GemmRole::TrailingT)
         hipLaunchKernelGGL((k_gemm_f64<true, false, 5>), grid, dim3(THREADS), 0, s, p);
     else if (g.a_trans && !g.b_trans && g.role == GemmRole::SolveT)
         hipLaunchKernelGGL((k_gemm_f64<true, false, 6>), grid, dim3(THREADS), 0, s, p);
     else if (g.a_trans && !g.b_trans && g.role == GemmRole::LeafT)
         hipLaunchKernelGGL((k_gemm_f64<true, false, 7>), grid, dim3(THREADS), 0, s, p);
-    else if (g.a_trans && !g.b_trans)
+    else if (g.a_trans && !g.b_trans && g.role == GemmRole::Inverse)
         hipLaunchKernelGGL((k_gemm_f64<true, false, 3>), grid, dim3(THREADS), 0, s, p);
     else
-        throw Error(GPRAT_ERR_INTERNAL, "gemm_f64: unsupported operand layout (T,T)");
+        throw Error(GPRAT_ERR_INTERNAL, "gemm_f64: operand layout and role not instantiated");
     GP_CHECK_LAUNCH();
 }
 

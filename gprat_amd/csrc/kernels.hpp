@@ -84,17 +84,15 @@ void gradient_sums(hipStream_t s,
 
 // Which algorithm a product belongs to; each role is its own kernel symbol
 // so profilers can attribute time (same code otherwise).
+// The role only separates kernel names in profiles (it is the last template
+// argument of k_gemm_f64); the values are kept stable for that reason.
 enum class GemmRole
 {
-    Trailing,  // Cholesky trailing update
-    Panel,     // Cholesky panel-internal update
-    Solve,     // TRSM coupling update
-    Inverse,   // W^T W
-    PanelSolve,  // panel rows times the diagonal-block inverse
-    TrailingT,   // Cholesky trailing update from a k-major panel copy
-    SolveT,      // TRSM update from a k-major panel copy
-    LeafT,       // TRSM diagonal step: k-major block inverse times the right-hand side
-    PanelX,      // Cholesky panel rows times the diagonal block's inverse
+    Inverse = 3,    // W^T W (A transposed, triangular K range)
+    TrailingT = 5,  // Cholesky trailing update from a k-major panel copy
+    SolveT = 6,     // TRSM update from a k-major panel copy
+    LeafT = 7,      // TRSM diagonal step: k-major block inverse times the right-hand side
+    PanelX = 8,     // Cholesky panel rows times the diagonal block's inverse
 };
 
 struct GemmArgs
