@@ -193,24 +193,43 @@ def batched_leg(capi, ctx, count=2048, seed=4):
         u, y = datagen.msd_series(n, 100000 + g)
         probs.append((datagen.padded_input(u, r), y, n, r, theta))
         flops += n ** 3 / 3.0
-    capi.Batch(ctx, probs[:16]).release()  # warm-up
-    ctx.set_profiling(True)
-    t0 = time.perf_counter()
-    b = capi.Batch(ctx, probs)
-    wall = time.perf_counter() - t0
-    launches, kernel_ms = ctx.kernel_stats("batch_fit")
-    ctx.set_profiling(False)
-    losses = b.losses()
-    b.release()
-    return {
-        "gps": count,
-        "kernel_ms": kernel_ms,
-        "wall_ms_incl_upload": wall * 1e3,
-        "gps_per_s": count / (kernel_ms * 1e-3),
-        "cholesky_tflops": flops / (kernel_ms * 1e-3) / 1e12,
-        "finite_losses": bool(np.all(np.isfinite(losses))),
-        "workload": "2048 GPs, n log-uniform [64, 2048], n_reg in {4, 8, 16}, FP64",
-    }
+    out = {"gps": count, "workload": "2048 GPs, n log-uniform [64, 2048], n_reg in {4, 8, 16}"}
+    batches = {}
+    for name, precision in (("fp64", capi.PRECISION_FP64), ("mixed", capi.PRECISION_MIXED)):
+        capi.Batch(ctx, probs[:16], precision=precision).release()  # warm-up
+        ctx.set_profiling(True)
+        t0 = time.perf_counter()
+        b = capi.Batch(ctx, probs, precision=precision)
+        wall = time.perf_counter() - t0
+        launches, kernel_ms = ctx.kernel_stats("batch_fit")
+        ctx.set_profiling(False)
+        batches[name] = b
+        out[name] = {
+            "kernel_ms": kernel_ms,
+            "wall_ms_incl_upload": wall * 1e3,
+            "gps_per_s": count / (kernel_ms * 1e-3),
+            "cholesky_tflops": flops / (kernel_ms * 1e-3) / 1e12,
+            "finite_losses": bool(np.all(np.isfinite(b.losses()))),
+        }
+    # mixed against FP64 on every GP: loss, and mean/variance at 64 points
+    l64, l32 = batches["fp64"].losses(), batches["mixed"].losses()
+    zt = datagen.padded_input(datagen.msd_series(64, 99)[0], 16)
+    em = ev = 0.0
+    for g in range(0, count, 16):
+        r = probs[g][3]
+        z = zt[15 - (r - 1):]
+        m64, v64 = batches["fp64"].predict(g, z, 64)
+        m32, v32 = batches["mixed"].predict(g, z, 64)
+        em = max(em, float(np.max(np.abs(m32 - m64)) / np.max(np.abs(m64))))
+        ev = max(ev, float(np.max(np.abs(v32 - v64)) / np.max(np.abs(v64))))
+    steps = batches["mixed"].refinement()
+    out["mixed"]["speedup_vs_fp64"] = out["fp64"]["kernel_ms"] / out["mixed"]["kernel_ms"]
+    out["mixed"]["rel_err_vs_fp64"] = {"mean_every_16th_gp": em, "var_every_16th_gp": ev,
+                                       "loss_max": float(np.max(np.abs(l32 - l64) / np.abs(l64)))}
+    out["mixed"]["refinement_steps"] = [int(steps.min()), int(steps.max())]
+    for b in batches.values():
+        b.release()
+    return out
 
 
 def serving_trace(count=192, seed=7):

@@ -75,6 +75,7 @@ SIGNATURES = {
     "gprat_model_release": [_VP],
     "gprat_batch_fit": [_VP, C.c_int, C.c_void_p, C.c_int, C.POINTER(_VP)],
     "gprat_batch_losses": [_VP, _D],
+    "gprat_batch_refinement": [_VP, _I],
     "gprat_batch_predict": [_VP, C.c_int, _D, _SZ, C.c_int, _D, _D],
     "gprat_batch_release": [_VP],
     "gprat_server_create": [C.POINTER(ServerConfig), C.POINTER(_VP)],
@@ -390,6 +391,12 @@ class Batch:
     def losses(self):
         out = np.empty(self.count)
         _check(lib().gprat_batch_losses(self.handle, _dptr(out)))
+        return out
+
+    def refinement(self):
+        """Refinement steps per GP (zeros for an FP64 batch)."""
+        out = np.zeros(self.count, dtype=np.int32)
+        _check(lib().gprat_batch_refinement(self.handle, out.ctypes.data_as(_I)))
         return out
 
     def predict(self, index, test_in, m, variance=True):

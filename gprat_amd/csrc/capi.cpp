@@ -589,6 +589,18 @@ int gprat_batch_losses(gprat_batch *batch, double *losses)
         });
 }
 
+int gprat_batch_refinement(const gprat_batch *batch, int *steps)
+{
+    return guarded(
+        [&]
+        {
+            gpamd::require(batch != nullptr && steps != nullptr, "null argument");
+            auto *h = reinterpret_cast<const BatchHandle *>(batch);
+            std::lock_guard<std::recursive_mutex> g(h->owner->mu);
+            std::copy(h->batch->refine_steps.begin(), h->batch->refine_steps.end(), steps);
+        });
+}
+
 int gprat_batch_predict(gprat_batch *batch, int index, const double *test_in, size_t test_in_len, int m, double *mean_out,
                         double *var_out)
 {

@@ -999,7 +999,7 @@ void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean,
 // (series and zero-padded targets); losses[g] receives GP g's loss and the
 // result's entry g the first non-positive pivot column of GP g (INT_MAX if
 // its matrix was positive definite).
-std::vector<int> batch_run(Context &c, const std::vector<k::BatchGP> &gps, double *losses_host)
+std::vector<int> batch_run(Context &c, const std::vector<k::BatchGP> &gps, double *losses_host, bool mixed, int *refine_host)
 {
     const int count = static_cast<int>(gps.size());
     require(count >= 1, "batch: need at least one GP");
@@ -1012,6 +1012,10 @@ std::vector<int> batch_run(Context &c, const std::vector<k::BatchGP> &gps, doubl
     DevBuf<double> desc(c, (sizeof(k::BatchGP) * count + 7) / 8);
     DevBuf<int> dorder(c, count), info(c, count), counter(c, 1);
     DevBuf<double> losses(c, count);
+    DevBuf<int> steps(c, count);
+    if (mixed)
+        for (const auto &q : gps)
+            require(q.work != nullptr, "batch: mixed precision needs a work vector per GP");
     GP_HIP(hipMemcpyAsync(desc.get(), gps.data(), sizeof(k::BatchGP) * count, hipMemcpyDefault, c.main));
     GP_HIP(hipMemcpyAsync(dorder.get(), order.data(), sizeof(int) * count, hipMemcpyDefault, c.main));
     hipDeviceProp_t prop;
@@ -1020,8 +1024,10 @@ std::vector<int> batch_run(Context &c, const std::vector<k::BatchGP> &gps, doubl
     {
         Timed t(c, c.main, "batch_fit");
         k::batch_fit(c.main, reinterpret_cast<const k::BatchGP *>(desc.get()), dorder.get(), count, counter.get(), losses.get(),
-                     info.get(), workgroups);
+                     info.get(), workgroups, mixed, mixed ? steps.get() : nullptr);
     }
+    if (mixed && refine_host)
+        GP_HIP(hipMemcpyAsync(refine_host, steps.get(), sizeof(int) * count, hipMemcpyDefault, c.main));
     std::vector<int> hinfo(count);
     GP_HIP(hipMemcpyAsync(hinfo.data(), info.get(), sizeof(int) * count, hipMemcpyDefault, c.main));
     GP_HIP(hipMemcpyAsync(losses_host, losses.get(), sizeof(double) * count, hipMemcpyDefault, c.main));
@@ -1041,13 +1047,16 @@ static void check_desc(const gprat_gp_desc &d, int g)
 void batch_fit(Context &c, Batch &b, int count, const gprat_gp_desc *gps, int precision)
 {
     require(count >= 1 && gps != nullptr, "batch: need at least one GP");
-    require(precision == GPRAT_PRECISION_FP64, "batch: only GPRAT_PRECISION_FP64 is available for batches");
+    require(precision == GPRAT_PRECISION_FP64 || precision == GPRAT_PRECISION_MIXED, "batch: unknown precision mode");
+    const bool mixed = precision == GPRAT_PRECISION_MIXED;
     c.make_current();
     b.ctx = &c;
     b.count = count;
+    b.precision = precision;
     b.gps.resize(count);
-    // arena layout: [inputs of all GPs][factors, inverses, alpha of all GPs]
-    std::vector<long> x_off(count), y_off(count), l_off(count), w_off(count), a_off(count);
+    // arena layout: [inputs of all GPs][factors, inverses, alpha (and the
+    // refinement work vector) of all GPs]
+    std::vector<long> x_off(count), y_off(count), l_off(count), w_off(count), a_off(count), r_off(count, -1);
     long in_total = 0, out_total = 0;
     for (int g = 0; g < count; g++)
     {
@@ -1067,6 +1076,11 @@ void batch_fit(Context &c, Batch &b, int count, const gprat_gp_desc *gps, int pr
         out_total += np * 64;
         a_off[g] = in_total + out_total;
         out_total += np;
+        if (mixed)
+        {
+            r_off[g] = in_total + out_total;
+            out_total += np;
+        }
     }
     std::vector<double> staging(static_cast<std::size_t>(in_total), 0.0);
     for (int g = 0; g < count; g++)
@@ -1086,6 +1100,7 @@ void batch_fit(Context &c, Batch &b, int count, const gprat_gp_desc *gps, int pr
         q.L = base + l_off[g];
         q.W = base + w_off[g];
         q.alpha = base + a_off[g];
+        q.work = mixed ? base + r_off[g] : nullptr;
         q.n = d.n;
         q.n_pad = static_cast<int>(round_up(d.n, 64));
         q.n_reg = d.n_reg;
@@ -1095,7 +1110,8 @@ void batch_fit(Context &c, Batch &b, int count, const gprat_gp_desc *gps, int pr
     }
     GP_HIP(hipMemcpyAsync(base, staging.data(), staging.size() * sizeof(double), hipMemcpyDefault, c.main));
     b.losses.assign(count, 0.0);
-    const std::vector<int> info = batch_run(c, b.gps, b.losses.data());
+    b.refine_steps.assign(count, 0);
+    const std::vector<int> info = batch_run(c, b.gps, b.losses.data(), mixed, b.refine_steps.data());
     for (int g = 0; g < count; g++)
         if (info[g] != INT_MAX)
             throw Error(GPRAT_ERR_NOT_SPD, "batch: GP " + std::to_string(g)
@@ -1109,7 +1125,8 @@ void batch_predict(Batch &b, int index, const double *z, std::size_t z_len, int 
 {
     require(index >= 0 && index < b.count, "batch: GP index out of range");
     const k::BatchGP &q = b.gps[index];
-    predict_view(FactorView{ b.ctx, q.x, q.n, q.n_pad, q.n_reg, q.n_pad, SEK{ q.lengthscale, q.vertical, q.noise }, q.L, q.W, q.alpha },
+    predict_view(FactorView{ b.ctx, q.x, q.n, q.n_pad, q.n_reg, q.n_pad, SEK{ q.lengthscale, q.vertical, q.noise }, q.L, q.W, q.alpha,
+                             b.precision == GPRAT_PRECISION_MIXED },
                  z, z_len, mt, mean, var);
 }
 

@@ -156,6 +156,8 @@ struct Batch
     std::vector<k::BatchGP> gps;  // device addresses inside the arena
     DevBuf<double> arena;
     std::vector<double> losses;
+    int precision = GPRAT_PRECISION_FP64;
+    std::vector<int> refine_steps;  // mixed precision: refinement steps per GP
     std::size_t bytes() const { return arena.bytes(); }
 };
 
@@ -191,7 +193,10 @@ void predict_view(const FactorView &m, const double *z, std::size_t z_len, int m
 // Mean and the whole mt x mt latent posterior covariance (row-major).
 void predict_full_cov(Model &m, const double *z, std::size_t z_len, int mt, double *mean, double *cov);
 void predict_full_cov_view(const FactorView &m, const double *z, std::size_t z_len, int mt, double *mean, double *cov);
-std::vector<int> batch_run(Context &c, const std::vector<k::BatchGP> &gps, double *losses_host);
+// mixed: FP32 block products and FP64 refinement of alpha (each GP needs
+// its `work` vector); refine_host (count ints) then gets the steps per GP.
+std::vector<int> batch_run(Context &c, const std::vector<k::BatchGP> &gps, double *losses_host, bool mixed = false,
+                           int *refine_host = nullptr);
 void batch_fit(Context &c, Batch &b, int count, const gprat_gp_desc *gps, int precision);
 void batch_losses(Batch &b, double *losses);
 void batch_predict(Batch &b, int index, const double *z, std::size_t z_len, int mt, double *mean, double *var);

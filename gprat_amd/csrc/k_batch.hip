@@ -42,11 +42,24 @@ struct Factor
     double W[NB][NB + 1];
     double R[NB];
 };
+// FP32 copies of two 64x64 blocks for the mixed-precision products; row
+// stride XF floats (the 16 rows x 4 k-slots of an MFMA operand read fall in
+// distinct banks)
+constexpr int XF = NB + 4;
+struct Stage32
+{
+    float a[NB * XF];
+    float b[NB * XF];
+};
 union Shared
 {
     Stage st;
+    Stage32 st32;
     Factor fa;
 };
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float f2 __attribute__((ext_vector_type(2)));
 
 // One thread's share of a 64x64 row-major block: 8 pairs of doubles, row
 // e >> 5 and columns (e & 31) * 2 for e = tid + 256 s.
@@ -73,7 +86,56 @@ struct BlockRegs
             *reinterpret_cast<d2 *>(dst + (e >> 5) * XS + (e & 31) * 2) = v[s];
         }
     }
+
+    // rounded to FP32, row stride XF
+    __device__ __forceinline__ void store32(float *dst) const
+    {
+#pragma unroll
+        for (int s = 0; s < 8; s++)
+        {
+            const int e = threadIdx.x + THREADS * s;
+            *reinterpret_cast<f2 *>(dst + (e >> 5) * XF + (e & 31) * 2) = f2{ static_cast<float>(v[s].x), static_cast<float>(v[s].y) };
+        }
+    }
 };
+
+// Position (row, column within the 64x64 block) of this lane's accumulator
+// entry q = (i * 2 + j) * 4 + e. f64 16x16x4: column lane % 16, rows
+// lane / 16 + 4 e; f32 16x16x4: column lane % 16, rows 4 (lane / 16) + e.
+template <bool F32>
+__device__ __forceinline__ void acc_position(int q, int &r, int &c)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lm = lane & 15, lk = lane >> 4;
+    const int i = q >> 3, j = (q >> 2) & 1, e = q & 3;
+    r = (w >> 1) * 32 + i * 16 + (F32 ? 4 * lk + e : lk + 4 * e);
+    c = (w & 1) * 32 + j * 16 + lm;
+}
+
+// acc += sum_k A[r][k] B[c][k] over a 64-deep FP32 block pair in LDS, FP32
+// accumulation; wavefront w owns rows 32 (w >> 1).., columns 32 (w & 1)..
+__device__ __forceinline__ void mfma_nt_acc32(const float *A, const float *B, v4f (&acc)[2][2])
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lm = lane & 15, lk = lane >> 4;
+    const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
+#pragma unroll 4
+    for (int k0 = 0; k0 < NB; k0 += 4)
+    {
+        float a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+            a[i] = A[(r0 + i * 16 + lm) * XF + k0 + lk];
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+            b[j] = B[(c0 + j * 16 + lm) * XF + k0 + lk];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+}
 
 // acc = sum_{k < kn} L[i, k] L[j, k]^T over 64-blocks of block rows i and j
 // (block (r, k) at L + r*64*np + k*64), streamed through the LDS stage with
@@ -106,12 +168,194 @@ __device__ __forceinline__ void accumulate_row_products(const double *L, long np
     __syncthreads();
 }
 
+// The same sum with FP32 operands and FP32 accumulation.
+__device__ __forceinline__ void accumulate_row_products32(const double *L, long np, int ib, int jb, int kn, Stage32 &st, v4f (&acc)[2][2])
+{
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+            acc[i][j] = v4f{ 0.0f, 0.0f, 0.0f, 0.0f };
+    if (kn == 0)
+        return;
+    const double *ai = L + static_cast<long>(ib) * NB * np;
+    const double *bj = L + static_cast<long>(jb) * NB * np;
+    BlockRegs ra, rb;
+    ra.load(ai, np);
+    rb.load(bj, np);
+    for (int kb = 0; kb < kn; kb++)
+    {
+        __syncthreads();  // every wavefront is done with the previous pair
+        ra.store32(st.a);
+        rb.store32(st.b);
+        __syncthreads();
+        if (kb + 1 < kn)
+        {
+            ra.load(ai + (kb + 1) * NB, np);
+            rb.load(bj + (kb + 1) * NB, np);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        mfma_nt_acc32(st.a, st.b, acc);
+    }
+    __syncthreads();
+}
+
+// vec <- K^-1 vec for one GP from its factor: forward then backward block
+// substitution with the saved 64x64 inverses (W). xs and part are two
+// 256-double LDS buffers.
+__device__ __forceinline__ void block_solves(const double *L, const double *Wg, long np, int nblk, double *vec, double *xs, double *part)
+{
+    const int tid = threadIdx.x;
+    const int row = tid & 63, quarter = tid >> 6;
+    for (int J = 0; J < nblk; J++)
+    {
+        const double *W = Wg + static_cast<long>(J) * NB * NB;
+        const long base = static_cast<long>(J) * NB;
+        {
+            // x_J = W_JJ vec_J: four partial sums per row, fixed order
+            double w[16], a[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++)
+            {
+                w[k] = W[row * NB + quarter * 16 + k];
+                a[k] = vec[base + quarter * 16 + k];
+            }
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < 16; k++)
+                s += w[k] * a[k];
+            part[quarter * 64 + row] = s;
+        }
+        __syncthreads();
+        if (tid < NB)
+        {
+            xs[tid] = (part[tid] + part[64 + tid]) + (part[128 + tid] + part[192 + tid]);
+            vec[base + tid] = xs[tid];
+        }
+        __syncthreads();
+        for (long i = base + NB + tid; i < np; i += THREADS)
+        {
+            const double *lr = L + i * np + base;
+            double s = 0.0;
+            // 16 values of the row in flight at a time
+            for (int k0 = 0; k0 < NB; k0 += 16)
+            {
+                d2 v[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++)
+                    v[q] = *reinterpret_cast<const d2 *>(lr + k0 + 2 * q);
+#pragma unroll
+                for (int q = 0; q < 8; q++)
+                    s += v[q].x * xs[k0 + 2 * q] + v[q].y * xs[k0 + 2 * q + 1];
+            }
+            vec[i] -= s;
+        }
+        __syncthreads();
+    }
+    for (int J = nblk - 1; J >= 0; J--)
+    {
+        const double *W = Wg + static_cast<long>(J) * NB * NB;
+        const long base = static_cast<long>(J) * NB;
+        {
+            // x_J = W_JJ^T vec_J
+            double w[16], a[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++)
+            {
+                w[k] = W[(quarter * 16 + k) * NB + row];
+                a[k] = vec[base + quarter * 16 + k];
+            }
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < 16; k++)
+                s += w[k] * a[k];
+            part[quarter * 64 + row] = s;
+        }
+        __syncthreads();
+        if (tid < NB)
+        {
+            xs[tid] = (part[tid] + part[64 + tid]) + (part[128 + tid] + part[192 + tid]);
+            vec[base + tid] = xs[tid];
+        }
+        __syncthreads();
+        for (long jcol = tid; jcol < base; jcol += THREADS)
+        {
+            double s = 0.0;
+            for (int k0 = 0; k0 < NB; k0 += 16)
+            {
+                double v[16];
+#pragma unroll
+                for (int q = 0; q < 16; q++)
+                    v[q] = L[(base + k0 + q) * np + jcol];
+#pragma unroll
+                for (int q = 0; q < 16; q++)
+                    s += v[q] * xs[k0 + q];
+            }
+            vec[jcol] -= s;
+        }
+        __syncthreads();
+    }
+}
+
+// max over the workgroup of v (all threads get it); red holds THREADS doubles
+__device__ __forceinline__ double block_max(double v, double *red)
+{
+    const int tid = threadIdx.x;
+    red[tid] = v;
+    __syncthreads();
+    for (int w = THREADS / 2; w > 0; w >>= 1)
+    {
+        if (tid < w)
+            red[tid] = fmax(red[tid], red[tid + w]);
+        __syncthreads();
+    }
+    const double m = red[0];
+    __syncthreads();
+    return m;
+}
+
+// r = y - K alpha for one GP (r[i] = 0 for padding rows), K regenerated
+// from the series in FP64: thread = row (its window in registers), the
+// column index is uniform across the workgroup, so the column's window and
+// alpha_j arrive through scalar loads. R: the window length (compile time).
+template <int R>
+__device__ __forceinline__ void residual_rows(const BatchGP &g, double scale, const double *alpha, double *r)
+{
+    const double *x = g.x;
+    for (long i0 = 0; i0 < g.n_pad; i0 += THREADS)
+    {
+        const long i = i0 + threadIdx.x;
+        const bool act = i < g.n;
+        double xi[R];
+#pragma unroll
+        for (int k = 0; k < R; k++)
+            xi[k] = act ? x[i + k] : 0.0;
+        double s = 0.0;
+        if (i0 < g.n)
+            for (long j = 0; j < g.n; j++)
+            {
+                double d = 0.0;
+#pragma unroll
+                for (int k = 0; k < R; k++)
+                {
+                    const double t = xi[k] - x[j + k];
+                    d = fma(t, t, d);
+                }
+                s = fma(g.vertical * exp(scale * d), alpha[j], s);
+            }
+        if (i < g.n_pad)
+            r[i] = act ? g.y[i] - (s + g.noise * alpha[i]) : 0.0;
+    }
+}
+
+template <bool MIXED>
 __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict__ gps,
                                                         const int *__restrict__ order,
                                                         int count,
                                                         int *__restrict__ counter,
                                                         double *__restrict__ losses,
-                                                        int *__restrict__ info)
+                                                        int *__restrict__ info,
+                                                        int *__restrict__ refine_steps)
 {
     __shared__ Shared sh;
     __shared__ double wjj[NB * XS];  // inverse of the current diagonal block, rows = B^T rows
@@ -123,7 +367,7 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
         if (tid == 0)
             next = atomicAdd(counter, 1);
         __syncthreads();
-        const int slot = next;
+        const int slot = __builtin_amdgcn_readfirstlane(next);  // uniform: the GP descriptor then comes through scalar loads
         __syncthreads();
         if (slot >= count)
             break;
@@ -165,21 +409,42 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
             for (int ib = jb; ib < nblk; ib++)
             {
                 double *Cij = L + static_cast<long>(ib) * NB * np + jb * NB;
+                // this lane's 16 entries of K_ij - sum_k L_ik L_jk^T, at the
+                // accumulator positions of the product's precision
                 double kij[16];
-                blk::load_at_acc(Cij, np, kij);  // in flight while the products run
-                v4d acc[2][2];
-                accumulate_row_products(L, np, ib, jb, jb, sh.st, acc);
 #pragma unroll
-                for (int i = 0; i < 2; i++)
+                for (int q = 0; q < 16; q++)
+                {
+                    int r, c;
+                    acc_position<MIXED>(q, r, c);
+                    kij[q] = Cij[static_cast<long>(r) * np + c];  // in flight while the products run
+                }
+                if constexpr (MIXED)
+                {
+                    v4f acc[2][2];
+                    accumulate_row_products32(L, np, ib, jb, jb, sh.st32, acc);
 #pragma unroll
-                    for (int j = 0; j < 2; j++)
+                    for (int q = 0; q < 16; q++)
+                        kij[q] -= static_cast<double>(acc[q >> 3][(q >> 2) & 1][q & 3]);
+                }
+                else
+                {
+                    v4d acc[2][2];
+                    accumulate_row_products(L, np, ib, jb, jb, sh.st, acc);
 #pragma unroll
-                        for (int e = 0; e < 4; e++)
-                            acc[i][j][e] = kij[(i * 2 + j) * 4 + e] - acc[i][j][e];
+                    for (int q = 0; q < 16; q++)
+                        kij[q] -= acc[q >> 3][(q >> 2) & 1][q & 3];
+                }
                 if (ib == jb)
                 {
                     // D = K_jj - acc (lower), factor + inverse in LDS
-                    blk::for_each(acc, [&](int r, int c, double v) { sh.fa.D[r][c] = (c <= r) ? v : 0.0; });
+#pragma unroll
+                    for (int q = 0; q < 16; q++)
+                    {
+                        int r, c;
+                        acc_position<MIXED>(q, r, c);
+                        sh.fa.D[r][c] = (c <= r) ? kij[q] : 0.0;
+                    }
                     __syncthreads();
                     factor_block64(sh.fa.D, sh.fa.W, sh.fa.R, info + gi, jb * NB);
                     double *Wjj = Wg + static_cast<long>(jb) * NB * NB;
@@ -197,7 +462,13 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
                 else
                 {
                     // L_ij = (K_ij - acc) W_jj^T
-                    blk::for_each(acc, [&](int r, int c, double v) { sh.st.a[r * XS + c] = v; });
+#pragma unroll
+                    for (int q = 0; q < 16; q++)
+                    {
+                        int r, c;
+                        acc_position<MIXED>(q, r, c);
+                        sh.st.a[r * XS + c] = kij[q];
+                    }
                     __syncthreads();
                     v4d xb[2][2];
                     blk::mfma_nt(sh.st.a, wjj, xb);
@@ -213,94 +484,58 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
         for (long i = tid; i < np; i += THREADS)
             alpha[i] = (i < g.n) ? y[i] : 0.0;
         __syncthreads();
-        const int row = tid & 63, quarter = tid >> 6;
-        for (int J = 0; J < nblk; J++)
+        block_solves(L, Wg, np, nblk, alpha, xs, part);
+        if constexpr (MIXED)
         {
-            const double *W = Wg + static_cast<long>(J) * NB * NB;
-            const long base = static_cast<long>(J) * NB;
+            // FP64 refinement: r = y - K alpha with K regenerated from the
+            // series, delta = K^-1 r from the factor, alpha += delta, until
+            // the correction (or the next one, as the contraction so far
+            // predicts it) is at most 1e-12 max|alpha|, or stops shrinking
+            double *r = g.work;
+            double prev = 1e300;
+            int steps = 0;
+            for (int it = 0; it < 10; it++)
             {
-                // x_J = W_JJ alpha_J: four partial sums per row, fixed order
-                double w[16], a[16];
-#pragma unroll
-                for (int k = 0; k < 16; k++)
+                if (g.n_reg == 4)
+                    residual_rows<4>(g, scale, alpha, r);
+                else if (g.n_reg == 8)
+                    residual_rows<8>(g, scale, alpha, r);
+                else if (g.n_reg == 16)
+                    residual_rows<16>(g, scale, alpha, r);
+                else
+                    for (long i = tid; i < np; i += THREADS)
+                    {
+                        double s = 0.0;
+                        if (i < g.n)
+                        {
+                            for (long j = 0; j < g.n; j++)
+                                s += g.vertical * exp(scale * window_sqdist(x, i, x, j, g.n_reg)) * alpha[j];
+                            s += g.noise * alpha[i];
+                            s = y[i] - s;
+                        }
+                        r[i] = s;
+                    }
+                __syncthreads();
+                block_solves(L, Wg, np, nblk, r, xs, part);
+                double dmax = 0.0, amax = 0.0;
+                for (long i = tid; i < np; i += THREADS)
                 {
-                    w[k] = W[row * NB + quarter * 16 + k];
-                    a[k] = alpha[base + quarter * 16 + k];
+                    const double a = alpha[i] + r[i];
+                    alpha[i] = a;
+                    dmax = fmax(dmax, fabs(r[i]));
+                    amax = fmax(amax, fabs(a));
                 }
-                double s = 0.0;
-#pragma unroll
-                for (int k = 0; k < 16; k++)
-                    s += w[k] * a[k];
-                part[quarter * 64 + row] = s;
+                __syncthreads();
+                dmax = block_max(dmax, part);
+                amax = block_max(amax, part);
+                steps = it + 1;
+                const bool converged = dmax <= 1e-12 * amax || (it > 0 && dmax < prev && dmax * (dmax / prev) <= 1e-12 * amax);
+                if (converged || !(dmax < 0.5 * prev))
+                    break;
+                prev = dmax;
             }
-            __syncthreads();
-            if (tid < NB)
-            {
-                xs[tid] = (part[tid] + part[64 + tid]) + (part[128 + tid] + part[192 + tid]);
-                alpha[base + tid] = xs[tid];
-            }
-            __syncthreads();
-            for (long i = base + NB + tid; i < np; i += THREADS)
-            {
-                const double *lr = L + i * np + base;
-                double s = 0.0;
-                // 16 values of the row in flight at a time
-                for (int k0 = 0; k0 < NB; k0 += 16)
-                {
-                    d2 v[8];
-#pragma unroll
-                    for (int q = 0; q < 8; q++)
-                        v[q] = *reinterpret_cast<const d2 *>(lr + k0 + 2 * q);
-#pragma unroll
-                    for (int q = 0; q < 8; q++)
-                        s += v[q].x * xs[k0 + 2 * q] + v[q].y * xs[k0 + 2 * q + 1];
-                }
-                alpha[i] -= s;
-            }
-            __syncthreads();
-        }
-        for (int J = nblk - 1; J >= 0; J--)
-        {
-            const double *W = Wg + static_cast<long>(J) * NB * NB;
-            const long base = static_cast<long>(J) * NB;
-            {
-                // x_J = W_JJ^T alpha_J
-                double w[16], a[16];
-#pragma unroll
-                for (int k = 0; k < 16; k++)
-                {
-                    w[k] = W[(quarter * 16 + k) * NB + row];
-                    a[k] = alpha[base + quarter * 16 + k];
-                }
-                double s = 0.0;
-#pragma unroll
-                for (int k = 0; k < 16; k++)
-                    s += w[k] * a[k];
-                part[quarter * 64 + row] = s;
-            }
-            __syncthreads();
-            if (tid < NB)
-            {
-                xs[tid] = (part[tid] + part[64 + tid]) + (part[128 + tid] + part[192 + tid]);
-                alpha[base + tid] = xs[tid];
-            }
-            __syncthreads();
-            for (long jcol = tid; jcol < base; jcol += THREADS)
-            {
-                double s = 0.0;
-                for (int k0 = 0; k0 < NB; k0 += 16)
-                {
-                    double v[16];
-#pragma unroll
-                    for (int q = 0; q < 16; q++)
-                        v[q] = L[(base + k0 + q) * np + jcol];
-#pragma unroll
-                    for (int q = 0; q < 16; q++)
-                        s += v[q] * xs[k0 + q];
-                }
-                alpha[jcol] -= s;
-            }
-            __syncthreads();
+            if (refine_steps && tid == 0)
+                refine_steps[gi] = steps;
         }
 
         // 4. loss = 0.5 (y^T alpha + sum log L_ii^2 + n log 2 pi) / n
@@ -344,14 +579,18 @@ __global__ void k_set_ints(int *p, int count, int value)
 }  // namespace
 
 void batch_fit(hipStream_t s, const BatchGP *gps, const int *order, int count, int *counter, double *losses, int *info,
-               int workgroups)
+               int workgroups, bool mixed, int *refine_steps)
 {
     require(count >= 1 && workgroups >= 1, "batch_fit: bad shape");
     hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(1), 0, s, counter, 1, 0);
     GP_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_set_ints, dim3(ceil_div(count, 256)), dim3(256), 0, s, info, count, INT_MAX);
     GP_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_batch_fit, dim3(workgroups), dim3(THREADS), 0, s, gps, order, count, counter, losses, info);
+    if (mixed)
+        hipLaunchKernelGGL(k_batch_fit<true>, dim3(workgroups), dim3(THREADS), 0, s, gps, order, count, counter, losses, info, refine_steps);
+    else
+        hipLaunchKernelGGL(k_batch_fit<false>, dim3(workgroups), dim3(THREADS), 0, s, gps, order, count, counter, losses, info,
+                           refine_steps);
     GP_CHECK_LAUNCH();
 }
 

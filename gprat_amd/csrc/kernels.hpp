@@ -156,14 +156,18 @@ struct BatchGP
 {
     const double *x, *y;
     double *L, *W, *alpha;
+    double *work;  // n_pad doubles, mixed precision only (refinement residual and correction)
     int n, n_pad, n_reg, pad_;
     double lengthscale, vertical, noise;
 };
 // Persistent fit of `count` GPs (visited in `order`, largest first) by
 // `workgroups` workgroups; losses[g] gets GP g's loss, info[g] its first
-// non-positive pivot column (INT_MAX if none).
+// non-positive pivot column (INT_MAX if none). mixed: the block products
+// of the factorisation on FP32 matrix cores (FP64 diagonal blocks, FP64
+// storage) and alpha refined in FP64 against K regenerated from the
+// series; refine_steps[g] (when given) gets GP g's refinement steps.
 void batch_fit(hipStream_t s, const BatchGP *gps, const int *order, int count, int *counter, double *losses, int *info,
-               int workgroups);
+               int workgroups, bool mixed = false, int *refine_steps = nullptr);
 
 // ---- triangular family (k_triangular.hip) ---------------------------------
 

@@ -284,10 +284,16 @@ typedef struct gprat_batch gprat_batch;
 
 /* Fit `count` independent GPs of any sizes in one persistent launch
  * (assembly, Cholesky, alpha and loss per GP); factors stay resident.
- * No counterpart in the reference, which fits one GP per call. */
+ * GPRAT_PRECISION_MIXED: the block products of each factorisation on FP32
+ * matrix cores (diagonal blocks, storage and solves in FP64) and alpha
+ * refined in FP64 against K regenerated from the series, inside the same
+ * launch. No counterpart in the reference, which fits one GP per call. */
 int gprat_batch_fit(gprat_context *ctx, int count, const gprat_gp_desc *gps, int precision, gprat_batch **out);
 /* losses[g] = loss of GP g at its hyperparameters (count values). */
 int gprat_batch_losses(gprat_batch *batch, double *losses);
+/* steps[g] = refinement steps taken for GP g's alpha (count values; 0 for
+ * an FP64 batch). */
+int gprat_batch_refinement(const gprat_batch *batch, int *steps);
 /* Mean (and latent variance if var_out != NULL) of GP `index` at m points. */
 int gprat_batch_predict(gprat_batch *batch, int index, const double *test_in, size_t test_in_len, int m, double *mean_out,
                         double *var_out);

@@ -79,3 +79,40 @@ def test_batch_reports_not_spd(ctx):
     with pytest.raises(capi.GpratError) as e:
         capi.Batch(ctx, bad)
     assert e.value.status == capi.ERR_NOT_SPD and "GP 1" in str(e.value)
+
+
+def test_mixed_batch_vs_fp64_batch(ctx):
+    """Mixed precision batch (FP32 block products, FP64 refinement of alpha)
+    against the FP64 batch on the same 48 GPs: mean 1e-8 normwise (the
+    refined alpha; the bar of the mixed-precision configuration). The
+    variance (solved with the mixed factor, not refined) and the loss (its
+    log-determinant comes from the mixed factor) carry the factor's own
+    accuracy: every off-diagonal block product rounds its operands to FP32,
+    and these GPs reach condition numbers of ~4e5 (noise down to 0.01,
+    lengthscale up to 3), so their bars are 1e-3 and 1e-4 (measured
+    1.4e-4 and 2.0e-5). Every GP took at least one refinement step."""
+    from gprat_amd import capi
+    probs = _problems(48, seed=5, n_lo=64, n_hi=2048)
+    fits = [(x, y, n, r, t) for x, y, n, r, t, _ in probs]
+    b64 = capi.Batch(ctx, fits)
+    b32 = capi.Batch(ctx, fits, precision=capi.PRECISION_MIXED)
+    try:
+        l64, l32 = b64.losses(), b32.losses()
+        steps = b32.refinement()
+        assert np.all(steps >= 1) and np.all(steps <= 10)
+        assert np.all(b64.refinement() == 0)
+        worst_mean = worst_var = worst_loss = 0.0
+        for g, (x, y, n, r, t, z) in enumerate(probs):
+            m64, v64 = b64.predict(g, z, 50)
+            m32, v32 = b32.predict(g, z, 50)
+            worst_mean = max(worst_mean, norm_rel(m32, m64))
+            worst_var = max(worst_var, norm_rel(v32, v64))
+            worst_loss = max(worst_loss, abs(l32[g] - l64[g]) / abs(l64[g]))
+        print(f"mixed batch vs FP64 batch: mean {worst_mean:.2e} var {worst_var:.2e} loss {worst_loss:.2e}, "
+              f"refinement steps {steps.min()}..{steps.max()}")
+        assert worst_mean < 1e-8
+        assert worst_var < 1e-3
+        assert worst_loss < 1e-4
+    finally:
+        b64.release()
+        b32.release()
