@@ -47,6 +47,7 @@ struct Params
     long kstart_off;
     int set;
     int k_to_col;
+    int k_to_row;
 };
 
 // Per-thread geometry of one BK-deep slab of a 128-wide operand, fixed for
@@ -188,6 +189,8 @@ __device__ __forceinline__ void gemm_tile(const Params &p, long r0, long c0, dou
     long kend = p.K;
     if (p.k_to_col)
         kend = min(kend, c0 + BN);
+    if (p.k_to_row)
+        kend = min(kend, r0 + BM);
     if (part != 0)
     {
         const long mid = kbeg < kend ? kbeg + ((kend - kbeg) / BK / 2) * BK : kend;
@@ -349,7 +352,7 @@ void gemm_f64(hipStream_t s, const GemmArgs &g)
     require(reinterpret_cast<uintptr_t>(g.A) % 16 == 0 && reinterpret_cast<uintptr_t>(g.B) % 16 == 0,
             "gemm_f64: operands must be 16-byte aligned");
     Params p{ g.A, g.lda, g.B, g.ldb, g.C, g.ldc, g.M, g.N, g.K, g.lower ? 1 : 0, g.diag_off, g.k_from_tile ? 1 : 0,
-              g.kstart_off, g.set ? 1 : 0, g.k_to_col ? 1 : 0 };
+              g.kstart_off, g.set ? 1 : 0, g.k_to_col ? 1 : 0, g.k_to_row ? 1 : 0 };
     dim3 grid(static_cast<unsigned>(ceil_div(g.N, BN)), static_cast<unsigned>(ceil_div(g.M, BM)));
     require(grid.y <= 65535, "gemm_f64: M too large for one launch");
     // tiles holding part of the update, counted as the capped kernel numbers them

@@ -41,6 +41,7 @@ struct Params32
     long diag_off;
     int set;  // C = acc instead of C -= acc
     int max_wgs;  // > 0: at most this many workgroups, looping over tiles
+    int k_to_row;  // A(i, k) known zero for k > i: each row tile stops K at its last row
 };
 
 // Element offsets (within one slab) of this thread's four 16-byte loads,
@@ -145,8 +146,9 @@ __device__ __forceinline__ void gemm_tile32(const Params32<TC> &p, long r0, long
     int la[4], lb[4];
     slab_offsets(p.lda, rows_a, tid, oa, la);
     slab_offsets(p.ldb, rows_b, tid, ob, lb);
-    const int half = (p.K / BK / 2) * BK;
-    const int k0 = part == 2 ? half : 0, k1 = part == 1 ? half : p.K;
+    const int kmax = p.k_to_row ? static_cast<int>(min(static_cast<long>(p.K), r0 + BM)) : p.K;
+    const int half = (kmax / BK / 2) * BK;
+    const int k0 = part == 2 ? half : 0, k1 = part == 1 ? half : kmax;
     const float *ak = p.A + r0 + static_cast<long>(k0) * p.lda;
     const float *bk = p.B + c0 + static_cast<long>(k0) * p.ldb;
     const long astep = BK * p.lda, bstep = BK * p.ldb;
@@ -351,14 +353,14 @@ static void launch_f32_tn(hipStream_t s, const Params32<TC> &p)
 void gemm_f32_trailing(hipStream_t s, const float *P, long ldp, long a_off, long b_off, double *C, long ldc, int M, int N, int K, long diag_off,
                        int max_wgs)
 {
-    Params32<double> p{ P + a_off, ldp, P + b_off, ldp, C, ldc, M, N, K, 1, diag_off, 0, max_wgs };
+    Params32<double> p{ P + a_off, ldp, P + b_off, ldp, C, ldc, M, N, K, 1, diag_off, 0, max_wgs, 0 };
     launch_f32_tn<double, 0>(s, p);
 }
 
 void gemm_f32_tn(hipStream_t s, const float *A, long lda, const float *B, long ldb, float *C, long ldc, int M, int N, int K, bool set,
-                 int max_wgs)
+                 int max_wgs, bool a_lower)
 {
-    Params32<float> p{ A, lda, B, ldb, C, ldc, M, N, K, 0, 0, set ? 1 : 0, max_wgs };
+    Params32<float> p{ A, lda, B, ldb, C, ldc, M, N, K, 0, 0, set ? 1 : 0, max_wgs, a_lower ? 1 : 0 };
     if (set)
         launch_f32_tn<float, 1>(s, p);
     else

@@ -567,25 +567,48 @@ __global__ __launch_bounds__(THREADS) void k_invert_blocks256(const double *__re
         }
 }
 
-// colsq[j] += sum_{i<rows} V[i][j]^2: 64 columns per workgroup, four row
-// groups summed separately and combined in a fixed order.
+// colsq[j] += sum_{i<rows} V[i][j]^2: 64 columns per workgroup, the
+// sixteen wavefronts of a 1024-thread workgroup take rows g, g + 16, ...
+// with eight loads in flight per lane; the sixteen partial sums are
+// combined in a fixed order.
+constexpr int CQ_WAVES = 16;
 template <class T>
-__global__ __launch_bounds__(THREADS) void k_colsq_accumulate(const T *__restrict__ V, long ldv, int rows, int m, double *__restrict__ colsq)
+__global__ __launch_bounds__(64 * CQ_WAVES) void k_colsq_accumulate(const T *__restrict__ V, long ldv, int rows, int m,
+                                                                    double *__restrict__ colsq)
 {
-    __shared__ double part[4][64];
+    __shared__ double part[CQ_WAVES][64];
     const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
     const long j = static_cast<long>(blockIdx.x) * 64 + c;
     double s = 0.0;
     if (j < m)
-        for (int i = g; i < rows; i += 4)
+    {
+        int i = g;
+        for (; i + 7 * CQ_WAVES < rows; i += 8 * CQ_WAVES)
         {
-            const double v = static_cast<double>(V[i * ldv + j]);
-            s += v * v;
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                v[u] = static_cast<double>(V[static_cast<long>(i + u * CQ_WAVES) * ldv + j]);
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                s = fma(v[u], v[u], s);
         }
+        for (; i < rows; i += CQ_WAVES)
+        {
+            const double v = static_cast<double>(V[static_cast<long>(i) * ldv + j]);
+            s = fma(v, v, s);
+        }
+    }
     part[g][c] = s;
     __syncthreads();
     if (g == 0 && j < m)
-        colsq[j] += (part[0][c] + part[1][c]) + (part[2][c] + part[3][c]);
+    {
+        double t = 0.0;
+#pragma unroll
+        for (int q = 0; q < CQ_WAVES; q++)
+            t += part[q][c];
+        colsq[j] += t;
+    }
 }
 
 __global__ void k_fill_strided(double *__restrict__ p, long count, double value)
@@ -622,7 +645,7 @@ static void colsq_accumulate_impl(hipStream_t s, const T *V, long ldv, int rows,
     require(rows >= 0 && m >= 0 && ldv >= m, "colsq_accumulate: bad shape");
     if (rows == 0 || m == 0)
         return;
-    hipLaunchKernelGGL(k_colsq_accumulate<T>, dim3(static_cast<unsigned>(ceil_div(m, 64))), dim3(THREADS), 0, s, V, ldv, rows, m,
+    hipLaunchKernelGGL(k_colsq_accumulate<T>, dim3(static_cast<unsigned>(ceil_div(m, 64))), dim3(64 * CQ_WAVES), 0, s, V, ldv, rows, m,
                        colsq);
     GP_CHECK_LAUNCH();
 }

@@ -120,6 +120,9 @@ struct GemmArgs
     // B(k, j) is known zero for k > j (a lower-triangular matrix seen as
     // B(k, j) = W(j, k)): each column tile stops K at its last column.
     bool k_to_col;
+    // A(i, k) is known zero for k > i (A lower triangular): each row tile
+    // stops K at its last row.
+    bool k_to_row;
     // true: C = acc; false: C -= acc.
     bool set;
     GemmRole role;
@@ -144,8 +147,9 @@ void gemm_f32_trailing(hipStream_t s, const float *P, long ldp, long a_off, long
                        int max_wgs = 0);
 // C (= or -=) A B over FP32 operands, A(i,k) = A[k*lda + i], B(k,j) = B[k*ldb + j],
 // FP32 C; K % 32 == 0, M, N, lda, ldb multiples of 4, A and B 16-byte aligned.
+// a_lower: A(i, k) is known zero for k > i (each row tile stops K there)
 void gemm_f32_tn(hipStream_t s, const float *A, long lda, const float *B, long ldb, float *C, long ldc, int M, int N, int K, bool set,
-                 int max_wgs = 0);
+                 int max_wgs = 0, bool a_lower = false);
 
 // ---- batched fit of many GPs (k_batch.hip) --------------------------------
 

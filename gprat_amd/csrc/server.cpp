@@ -193,13 +193,15 @@ std::size_t Server::fit_need(const Entry &e) const
     {
         np = padded(e.n);
         ld = leading_dim(static_cast<long>(np));
-        // Cholesky panel copy and panel scratch (at most 512 x n_pad doubles
+        // Cholesky panel copy and panel scratch (panel width x n_pad doubles
         // each), the diagonal-block scratch, solve and refinement vectors,
         // the refinement's partial sums
-        temp = 8 * (3 * 512 * np + 8 * np + static_cast<std::size_t>(k::mean_splits(e.n, e.n)) * n);
+        const std::size_t bo = static_cast<std::size_t>(panel_width(static_cast<int>(np)));
+        temp = 8 * (3 * bo * np + 8 * np + static_cast<std::size_t>(k::mean_splits(e.n, e.n)) * n);
     }
-    // factor, 64-block inverses, panel inverses (at most 512 wide), vectors
-    const std::size_t resident = 8 * (np * ld + np * 64 + (e.batched_path ? 0 : np * 512) + 2 * np + n + r);
+    // factor, 64-block inverses, panel inverses (panel width wide), vectors
+    const std::size_t wide = e.batched_path ? 0 : np * static_cast<std::size_t>(panel_width(static_cast<int>(np)));
+    const std::size_t resident = 8 * (np * ld + np * 64 + wide + 2 * np + n + r);
     return resident + temp + (std::size_t(1) << 20);
 }
 
@@ -209,9 +211,10 @@ std::size_t Server::predict_need(const Entry &e, int m) const
     const std::size_t mp = round_up(m, 64);
     const std::size_t blocks = (np + 255) / 256;
     const std::size_t splits = static_cast<std::size_t>(k::mean_splits(e.n, m));
-    // K*^T, two staged V blocks and two panel copies (at most 512 wide), block
+    // K*^T, the staged V block and the panel copy (panel width wide), block
     // inverses built for a factor without kept ones, the mean's partial sums
-    return 8 * (np * mp + 2 * 512 * mp + 4 * mp + blocks * 65536 + 2 * 512 * np + splits * m + m + e.r) + (std::size_t(1) << 20);
+    const std::size_t bo = e.batched_path ? 256 : static_cast<std::size_t>(panel_width(static_cast<int>(np)));
+    return 8 * (np * mp + bo * mp + 4 * mp + blocks * 65536 + bo * np + splits * m + m + e.r) + (std::size_t(1) << 20);
 }
 
 // Estimated flops of a request: the factorisation n^3/3 for a fit (or a
