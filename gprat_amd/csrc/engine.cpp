@@ -156,6 +156,7 @@ Context::~Context()
     }
     for (auto e : spare_events)
         hipEventDestroy(e);
+    panel_plans.clear();
     info.reset();
     scalars.reset();
     if (trail && trail != main)
@@ -323,40 +324,27 @@ static void trailing_kmajor(hipStream_t s, const double *P, long ldp, long off, 
     k::gemm_f64(s, g);
 }
 
-// Factor the diagonal block A[p0:pe, p0:pe] in place in 64-wide steps: the
-// fused factor + inverse of each 64x64 block (saved in Winv), the in-place
-// product with that inverse for the block's rows below it, and the update
-// of the rest of the block. Only rows inside the block are touched. The
-// update runs on at most update_wgs workgroups (beside a capped trailing
-// update: the slots that update leaves free, so it never waits for a
-// trailing tile to finish).
-static void factor_diagonal_block(Context &c, hipStream_t s, double *A, long lda, int p0, int pe, double *Winv, int update_wgs)
+// Diagonal-block task schedules (panel_plan.hpp), built once per shape and
+// kept on the device for the context's lifetime.
+const Context::PanelPlanDev &Context::panel_plan(int nb, int chain_wgs, int tail_wgs)
 {
-    for (int c0 = p0; c0 < pe; c0 += 64)
-    {
-        double *w = Winv + static_cast<long>(c0 / 64) * 64 * 64;
-        k::potrf64_inv(s, A, lda, c0, w, c.info.get());
-        const int rest = pe - c0 - 64;
-        if (rest <= 0)
-            break;
-        // rows below: X = A W^T in place, then the rank-64 update of the
-        // rest of the block, both on 64x64 blocks so the short products
-        // spread over as many CUs as there are blocks
-        k::panel_solve64(s, A, lda, c0 + 64, c0, rest, w);
-        k::panel_update64(s, A, lda, c0 + 64, c0, rest, update_wgs);
-    }
-}
-
-// Wb (k-major, leading dimension ldw: W(r, c) at Wb[c * ldw + r]) = inverse
-// of the factored diagonal block L[p0:p0+w, p0:p0+w], by doubling from the
-// 64-wide inverses: W_ba = -W_b L_ba W_a for adjacent halves of 64, 128,
-// 256 columns, each level two launches of one workgroup per 64x64 output
-// block (`tmp` holds at least w/2 x w/2 doubles). Every entry of the w x w
-// inverse is written, its upper part as zeros.
-static void invert_diagonal_block(hipStream_t s, const double *A, long lda, const double *Winv, int p0, int w, double *Wb, long ldw,
-                                  double *tmp, int max_wgs)
-{
-    k::invert_block_doubling(s, A, lda, Winv, p0, w, Wb, ldw, tmp, max_wgs);
+    std::lock_guard<std::recursive_mutex> g(mu);
+    chain_wgs = std::clamp(chain_wgs, 1, 4096);
+    tail_wgs = std::clamp(tail_wgs, 1, 4096);
+    const long key = (static_cast<long>(nb) << 26) | (static_cast<long>(chain_wgs) << 13) | tail_wgs;
+    auto it = panel_plans.find(key);
+    if (it != panel_plans.end())
+        return it->second;
+    const PanelPlan p = build_panel_plan(nb, chain_wgs, tail_wgs);
+    PanelPlanDev d;
+    d.tasks = DevBuf<int>(*this, p.tasks.size() * 4);
+    d.wg_off = DevBuf<int>(*this, p.wg_off.size());
+    make_current();
+    GP_HIP(hipMemcpy(d.tasks.get(), p.tasks.data(), p.tasks.size() * sizeof(PanelTask), hipMemcpyHostToDevice));
+    GP_HIP(hipMemcpy(d.wg_off.get(), p.wg_off.data(), p.wg_off.size() * sizeof(int), hipMemcpyHostToDevice));
+    d.launches = p.launches;
+    d.tmp_doubles = p.tmp_doubles;
+    return panel_plans.emplace(key, std::move(d)).first->second;
 }
 
 // ---------------------------------------------------------------------------
@@ -394,7 +382,8 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
         p64 = DevBuf<double>(c, static_cast<std::size_t>(Bo) * n_pad);
     // inverses of the panels' diagonal blocks: kept (Wpanel, block kb at
     // kb * Bo * Bo) or one scratch block reused by every panel
-    DevBuf<double> Wscratch, tmp(c, static_cast<std::size_t>(Bo / 2) * (Bo / 2) + 256 * 256);
+    const int chain_wgs = ahead ? std::max(1, trailing_reserve()) : 4096, tail_wgs = ahead ? 64 : 4096;
+    DevBuf<double> Wscratch, tmp(c, static_cast<std::size_t>(std::max(1L, c.panel_plan(Bo / 64, chain_wgs, tail_wgs).tmp_doubles)));
     DevBuf<double> Xs(c, static_cast<std::size_t>(Bo) * n_pad);  // the panel's rows below, before the copy back
     if (!Wpanel)
         Wscratch = DevBuf<double>(c, static_cast<std::size_t>(Bo) * Bo);
@@ -415,12 +404,18 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
         GP_HIP(hipEventRecord(e, from));
         GP_HIP(hipStreamWaitEvent(to, e, 0));
     };
+    // The diagonal block: its factor (in place), the inverses of its 64x64
+    // blocks (Winv) and its whole inverse (Wblock), as the task schedule's
+    // launches; the chain launches stay within the slots the capped
+    // trailing update leaves free.
     auto diagonal = [&](int p0, int pe)
     {
         Timed t(c, P, "panel");
-        factor_diagonal_block(c, P, A, lda, p0, pe, Winv, ahead ? std::max(1, trailing_reserve()) : INT_MAX);
-        if (pe < n_pad || Wpanel)
-            invert_diagonal_block(P, A, lda, Winv, p0, pe - p0, Wblock(p0), Bo, tmp.get(), ahead ? std::max(1, trailing_reserve()) : 64);
+        const int nbk = (pe - p0) / 64;
+        const auto &plan = c.panel_plan(nbk, chain_wgs, tail_wgs);
+        for (const auto &l : plan.launches)
+            k::panel_tasks(P, A, lda, p0, nbk, Winv, Wblock(p0), Bo, tmp.get(), c.info.get(), plan.tasks.get(), plan.wg_off.get() + l.first,
+                           l.groups);
     };
     // The panel's rows below its diagonal block: X = A W^T with W the
     // inverse of the diagonal block (k-major), one launch into scratch (each

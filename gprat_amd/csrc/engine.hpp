@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "panel_plan.hpp"
 
 namespace gpamd
 {
@@ -107,6 +108,16 @@ struct Context
     void prof_begin(hipStream_t s, const char *family, Pending &p);
     void prof_end(hipStream_t s, Pending &p);
     void resolve_profiling();  // after a synchronize
+
+    // device copies of the diagonal-block task schedules, by shape
+    struct PanelPlanDev
+    {
+        DevBuf<int> tasks, wg_off;
+        std::vector<PanelLaunch> launches;
+        long tmp_doubles = 0;
+    };
+    std::map<long, PanelPlanDev> panel_plans;
+    const PanelPlanDev &panel_plan(int nb, int chain_wgs, int tail_wgs);
 
     void make_current() const;
     void sync();

@@ -5,6 +5,7 @@
 #include "block64.hpp"
 #include "block_mfma.hpp"
 #include "kernels.hpp"
+#include "panel_plan.hpp"
 
 namespace gpamd::k
 {
@@ -33,160 +34,195 @@ __device__ __forceinline__ void load_lower_block(const double *__restrict__ A, l
     }
 }
 
-// One workgroup per diagonal block: L overwrites the block's lower triangle,
-// W = L^-1 (64x64 row-major, zero upper) goes to `w`.
-__global__ __launch_bounds__(THREADS) void k_potrf64_inv(double *__restrict__ A, long lda, int c0, double *__restrict__ w, int *__restrict__ info)
+// Task interpreter of the diagonal-block factorisation (panel_plan.hpp):
+// workgroup g of a launch runs the tasks [wg_off[g], wg_off[g + 1]) of
+// the host-built schedule, each on 64x64 blocks of the diagonal block at
+// (p0, p0). Block (i, j) of A is at A + (p0 + 64 i) lda + p0 + 64 j;
+// Winv_k (row-major) at Winv + (p0 / 64 + k) 4096; the k-major inverse has
+// W(64 i + r, 64 j + c) at Wb[(64 j + c) ldw + 64 i + r]. Every task reads
+// only blocks that earlier launches finished and writes blocks no other
+// task of its launch touches, so the tasks of a launch need no ordering.
+struct PanelArgs
 {
-    __shared__ double D[NB][NB + 1];
-    __shared__ double Wl[NB][NB + 1];
-    __shared__ double R[NB];
-    const int tid = threadIdx.x;
-    double *blk = A + static_cast<long>(c0) * lda + c0;
-    load_lower_block(A, lda, c0, D);
-    __syncthreads();
-    factor_block64(D, Wl, R, info, c0);
-    for (int e = tid; e < NB * NB; e += THREADS)
+    double *A;
+    long lda;
+    long p0;
+    double *Winv;
+    double *Wb;
+    long ldw;
+    double *tmp;
+    int *info;
+    const int4 *tasks;
+    const int *wg_off;
+    int nb;
+};
+
+__device__ __forceinline__ double *blockA(const PanelArgs &a, int i, int j)
+{
+    return a.A + (a.p0 + static_cast<long>(NB) * i) * a.lda + a.p0 + NB * j;
+}
+__device__ __forceinline__ double *blockWinv(const PanelArgs &a, int k) { return a.Winv + (a.p0 / NB + k) * static_cast<long>(NB * NB); }
+// k-major W block (i, j): element (r, c) at [c * ldw + r]
+__device__ __forceinline__ double *blockW(const PanelArgs &a, int i, int j) { return a.Wb + static_cast<long>(NB) * j * a.ldw + NB * i; }
+
+// FACTOR / UFACTOR: block (k, k) (minus L_kq L_kq^T when q >= 0) factored
+// and inverted in LDS; L to A (lower triangle), the inverse to Winv_k
+// (row-major, zero upper).
+__device__ __noinline__ void task_factor(const PanelArgs &a, int k, int q, double *buf0, double *buf1, double *R)
+{
+    double(*D)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(buf0);
+    double(*Wl)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(buf1);
+    double *blk = blockA(a, k, k);
+    load_lower_block(blk, a.lda, 0, D);
+    if (q >= 0)
+    {
+        blk::stage_block(buf1, blockA(a, k, q), a.lda);
+        __syncthreads();
+        blk::v4d acc[2][2];
+        blk::mfma_nt(buf1, buf1, acc);
+        blk::for_each(acc,
+                      [&](int r, int c, double v)
+                      {
+                          if (c <= r)
+                              D[r][c] -= v;
+                      });
+    }
+    __syncthreads();  // D final; every read of buf1 done before W lands there
+    factor_block64(D, Wl, R, a.info, static_cast<int>(a.p0) + NB * k);
+    double *w = blockWinv(a, k);
+    for (int e = threadIdx.x; e < NB * NB; e += THREADS)
     {
         const int r = e / NB, c = e % NB;
         if (c <= r)
-            blk[static_cast<long>(r) * lda + c] = D[r][c];
-        w[r * NB + c] = (c <= r) ? Wl[r][c] : 0.0;
+            blk[static_cast<long>(r) * a.lda + c] = D[r][c];
+        w[e] = (c <= r) ? Wl[r][c] : 0.0;
     }
 }
 
-// Panel solve inside a diagonal block, 64 rows per workgroup: X = A W^T in
-// place for rows r0 + 64 b .. + 63 of the columns [c0, c0 + 64), W the
-// 64x64 row-major inverse of the step's diagonal block. Each workgroup
-// reads and writes only its own rows.
-__global__ __launch_bounds__(THREADS) void k_panel_solve64(double *__restrict__ A, long lda, long r0, long c0, const double *__restrict__ W)
+// L_ik = A_ik Winv_k^T in place.
+__device__ __noinline__ void task_solve(const PanelArgs &a, int i, int k, double *buf0, double *buf1)
 {
-    __shared__ double a[NB * blk::XS];
-    __shared__ double w[NB * blk::XS];
-    double *p = A + (r0 + static_cast<long>(NB) * blockIdx.x) * lda + c0;
-    blk::stage_block(a, p, lda);
-    blk::stage_block(w, W, NB);
+    double *p = blockA(a, i, k);
+    blk::stage_block(buf0, p, a.lda);
+    blk::stage_block(buf1, blockWinv(a, k), NB);
     __syncthreads();
     blk::v4d acc[2][2];
-    blk::mfma_nt(a, w, acc);
-    blk::store_acc(p, lda, acc, true, false);
+    blk::mfma_nt(buf0, buf1, acc);
+    blk::store_acc(p, a.lda, acc, true, false);
 }
 
-// Rank-64 update inside a diagonal block: C_ij -= X_i X_j^T over the lower
-// 64-block pairs i >= j of the nb x nb block region at (rs, rs), with
-// X_i = A[rs + 64 i .., c0 .. c0 + 63] (columns left of the region, never
-// written here); diagonal blocks on and below their diagonal only. G
-// workgroups loop over the pairs t = g, g + G, ...
-__global__ __launch_bounds__(THREADS) void k_panel_update64(double *__restrict__ A, long lda, long rs, long c0, int nb)
+// A_ij -= L_ik L_jk^T (lower half of a diagonal block).
+__device__ __noinline__ void task_update(const PanelArgs &a, int i, int j, int k, double *buf0, double *buf1)
 {
-    __shared__ double xi[NB * blk::XS];
-    __shared__ double xj[NB * blk::XS];
-    const int pairs = nb * (nb + 1) / 2;
-    for (int t = blockIdx.x; t < pairs; t += gridDim.x)
+    blk::stage_block(buf0, blockA(a, i, k), a.lda);
+    if (i != j)
+        blk::stage_block(buf1, blockA(a, j, k), a.lda);
+    __syncthreads();
+    blk::v4d acc[2][2];
+    blk::mfma_nt(buf0, i != j ? buf1 : buf0, acc);
+    blk::sub_acc(blockA(a, i, j), a.lda, acc, i == j);
+}
+
+// W_kk = Winv_k (transposed through LDS: both sides coalesced), W_kj = 0
+// right of it.
+__device__ __noinline__ void task_place(const PanelArgs &a, int k, double *buf0)
+{
+    blk::stage_block(buf0, blockWinv(a, k), NB);
+    __syncthreads();
+    double *out = blockW(a, k, k);
+    for (int e = threadIdx.x; e < NB * NB; e += THREADS)
     {
-        int i = 0, j = t;
-        while (j > i)
-        {
-            j -= i + 1;
-            i++;
-        }
-        __syncthreads();  // the previous pair's LDS reads are done
-        blk::stage_block(xi, A + (rs + static_cast<long>(NB) * i) * lda + c0, lda);
-        blk::stage_block(xj, A + (rs + static_cast<long>(NB) * j) * lda + c0, lda);
-        __syncthreads();
-        blk::v4d acc[2][2];
-        blk::mfma_nt(xi, xj, acc);
-        blk::sub_acc(A + (rs + static_cast<long>(NB) * i) * lda + rs + static_cast<long>(NB) * j, lda, acc, i == j);
+        const int c = e >> 6, r = e & 63;
+        out[static_cast<long>(c) * a.ldw + r] = buf0[r * blk::XS + c];
+    }
+    for (int j = k + 1; j < a.nb; j++)
+    {
+        double *z = blockW(a, k, j);
+        for (int e = threadIdx.x; e < NB * NB; e += THREADS)
+            z[static_cast<long>(e >> 6) * a.ldw + (e & 63)] = 0.0;
     }
 }
 
-// Inverse of a factored w x w diagonal block, k-major (W(r, c) at
-// Wb[c * ldw + r]), by doubling from its 64-wide inverses.
-// k_inv_place: the 64x64 diagonal blocks (row-major inverses, transposed
-// through LDS so both sides are coalesced) and zeros in every block above
-// the diagonal; G workgroups loop over the blocks rb <= cb.
-__global__ __launch_bounds__(THREADS) void k_inv_place(const double *__restrict__ Winv, int nb, double *__restrict__ Wb, long ldw)
+// W_ba = -Winv_b (L_ba Winv_a), b = a + 1; the intermediate stays on chip.
+__device__ __noinline__ void task_joinf(const PanelArgs &a, int ka, double *buf0, double *buf1)
 {
-    __shared__ double a[NB * blk::XS];
-    const int tid = threadIdx.x;
-    for (int t = blockIdx.x; t < nb * (nb + 1) / 2; t += gridDim.x)
-    {
-        int cb = 0, rb = t;  // t = cb (cb + 1) / 2 + rb, rb <= cb
-        while (rb > cb)
-        {
-            rb -= cb + 1;
-            cb++;
-        }
-        double *out = Wb + static_cast<long>(NB) * cb * ldw + NB * rb;
-        if (rb == cb)
-        {
-            __syncthreads();  // the previous block's LDS reads are done
-            blk::stage_block(a, Winv + static_cast<long>(rb) * NB * NB, NB);
-            __syncthreads();
-            for (int e = tid; e < NB * NB; e += THREADS)
-            {
-                const int c = e >> 6, r = e & 63;
-                out[static_cast<long>(c) * ldw + r] = a[r * blk::XS + c];
-            }
-        }
-        else
-        {
-            for (int e = tid; e < NB * NB; e += THREADS)
-                out[static_cast<long>(e >> 6) * ldw + (e & 63)] = 0.0;
-        }
-    }
+    const int kb = ka + 1;
+    blk::stage_block(buf0, blockA(a, kb, ka), a.lda);  // [r][k] = L_ba(r, k)
+    blk::stage_block_t(buf1, blockWinv(a, ka), NB);    // [c][k] = W_a(k, c)
+    __syncthreads();
+    blk::v4d acc[2][2];
+    blk::mfma_nt(buf0, buf1, acc);
+    __syncthreads();
+    blk::for_each(acc, [&](int r, int c, double v) { buf1[c * blk::XS + r] = v; });  // [c][k] = T(k, c)
+    blk::stage_block(buf0, blockWinv(a, kb), NB);                                     // [r][k] = W_b(r, k)
+    __syncthreads();
+    blk::mfma_nt(buf0, buf1, acc);
+    double *out = blockW(a, kb, ka);
+    blk::for_each(acc, [&](int r, int c, double v) { out[static_cast<long>(c) * a.ldw + r] = -v; });
 }
 
-// One doubling level: for every pair of adjacent size-wide halves a = [t0,
-// t0 + size), b = [t0 + size, t0 + size + sb) of the block (t0 = 2 size p),
-//   mode 0: T = L_ba W_a, kept transposed in tmp (pair p at p size^2,
-//           T(r, c) at [c * size + r]); W_a lower, so k runs from c's block;
-//   mode 1: W_ba = -W_b T into Wb; W_b lower, so k stops at r's block.
-// One workgroup per 64x64 output block: blockIdx.x = (p, rb, cb).
-__global__ __launch_bounds__(THREADS) void k_inv_join(const double *__restrict__ L,
-                                                       long lda,
-                                                       long p0,
-                                                       int w,
-                                                       int size,
-                                                       double *__restrict__ Wb,
-                                                       long ldw,
-                                                       double *__restrict__ tmp,
-                                                       int mode)
+// Doubling join at level h, pair p: a = blocks [2hp, 2hp + h), b = the
+// next h; the pair's T (64h x 64h) at tmp + w * 4096, T(r, c) at
+// [c * 64h + r].
+//   phase 0: T(rb, cb) = sum_{s >= cb} L_{b0+rb, a0+s} W_{a0+s, a0+cb} (W_a lower);
+//   phase 1: W_{b0+rb, a0+cb} = -sum_{s <= rb} W_{b0+rb, b0+s} T(s, cb) (W_b lower).
+__device__ __noinline__ void task_join(const PanelArgs &a, int phase, int h, int p, int rb, int cb, int w, double *buf0, double *buf1)
 {
-    __shared__ double a[NB * blk::XS];
-    __shared__ double b[NB * blk::XS];
-    const int nbs = size / NB;
-    const int pair = blockIdx.x / (nbs * nbs), rem = blockIdx.x % (nbs * nbs);
-    const int rb = rem / nbs, cb = rem % nbs;
-    const long t0 = 2L * size * pair;
-    const int sb = static_cast<int>(min(static_cast<long>(size), w - t0 - size));
-    if (NB * rb >= sb)
-        return;
-    double *tp = tmp + static_cast<long>(pair) * size * size;
+    const int a0 = 2 * h * p, b0 = a0 + h;
+    const long size = static_cast<long>(NB) * h;
+    double *tp = a.tmp + static_cast<long>(w) * NB * NB;
     blk::v4d acc[2][2];
     blk::zero(acc);
-    if (mode == 0)
+    if (phase == 0)
     {
-        for (int kb = cb; kb < nbs; kb++)
+        for (int s = cb; s < h; s++)
         {
             __syncthreads();
-            blk::stage_block(a, L + (p0 + t0 + size + NB * rb) * lda + p0 + t0 + NB * kb, lda);  // L_ba(r, k)
-            blk::stage_block(b, Wb + (t0 + NB * cb) * ldw + t0 + NB * kb, ldw);                // [c][k] = W_a(k, c)
+            blk::stage_block(buf0, blockA(a, b0 + rb, a0 + s), a.lda);  // [r][k] = L(r, k)
+            blk::stage_block(buf1, blockW(a, a0 + s, a0 + cb), a.ldw);  // [c][k] = W(k, c)
             __syncthreads();
-            blk::mfma_nt_acc(a, b, acc);
+            blk::mfma_nt_acc(buf0, buf1, acc);
         }
-        blk::for_each(acc, [&](int r, int c, double v) { tp[static_cast<long>(NB * cb + c) * size + NB * rb + r] = v; });
+        blk::for_each(acc, [&](int r, int c, double v) { tp[(NB * cb + c) * size + NB * rb + r] = v; });
     }
     else
     {
-        for (int kb = 0; kb <= rb; kb++)
+        for (int s = 0; s <= rb; s++)
         {
             __syncthreads();
-            blk::stage_block_t(a, Wb + (t0 + size + NB * kb) * ldw + t0 + size + NB * rb, ldw);  // [r][k] = W_b(r, k)
-            blk::stage_block(b, tp + static_cast<long>(NB * cb) * size + NB * kb, size);       // [c][k] = T(k, c)
+            blk::stage_block_t(buf0, blockW(a, b0 + rb, b0 + s), a.ldw);  // [r][k] = W(r, k)
+            blk::stage_block(buf1, tp + (NB * cb) * size + NB * s, size);  // [c][k] = T(k, c)
             __syncthreads();
-            blk::mfma_nt_acc(a, b, acc);
+            blk::mfma_nt_acc(buf0, buf1, acc);
         }
-        blk::for_each(acc, [&](int r, int c, double v) { Wb[(t0 + NB * cb + c) * ldw + t0 + size + NB * rb + r] = -v; });
+        double *out = blockW(a, b0 + rb, a0 + cb);
+        blk::for_each(acc, [&](int r, int c, double v) { out[static_cast<long>(c) * a.ldw + r] = -v; });
+    }
+}
+
+__global__ __launch_bounds__(THREADS) void k_panel_tasks(PanelArgs a)
+{
+    __shared__ double buf0[NB * blk::XS];
+    __shared__ double buf1[NB * blk::XS];
+    __shared__ double R[NB];
+    const int first = a.wg_off[blockIdx.x], last = a.wg_off[blockIdx.x + 1];
+    for (int t = first; t < last; t++)
+    {
+        const int4 d = a.tasks[t];
+        const int type = d.x & 0xff;
+        __syncthreads();  // the previous task's LDS reads are done
+        if (type == PT_FACTOR || type == PT_UFACTOR)
+            task_factor(a, d.y, type == PT_UFACTOR ? d.z : -1, buf0, buf1, R);
+        else if (type == PT_SOLVE)
+            task_solve(a, d.y, d.z, buf0, buf1);
+        else if (type == PT_UPDATE)
+            task_update(a, d.y, d.z, d.w, buf0, buf1);
+        else if (type == PT_PLACE)
+            task_place(a, d.y, buf0);
+        else if (type == PT_JOINF)
+            task_joinf(a, d.y, buf0, buf1);
+        else
+            task_join(a, type == PT_JOIN0 ? 0 : 1, d.x >> 8, d.y, d.z & 0xffff, d.z >> 16, d.w, buf0, buf1);
     }
 }
 
@@ -669,47 +705,13 @@ void to_f32(hipStream_t s, const double *src, float *dst, long count)
     GP_CHECK_LAUNCH();
 }
 
-void potrf64_inv(hipStream_t s, double *A, long lda, int c0, double *w, int *info)
+void panel_tasks(hipStream_t s, double *A, long lda, long p0, int nb, double *Winv, double *Wb, long ldw, double *tmp, int *info,
+                 const int *tasks, const int *wg_off, int groups)
 {
-    require(c0 >= 0 && c0 % NB == 0 && lda % 2 == 0, "potrf64_inv: bad shape");
-    hipLaunchKernelGGL(k_potrf64_inv, dim3(1), dim3(THREADS), 0, s, A, lda, c0, w, info);
-    GP_CHECK_LAUNCH();
-}
-
-void invert_block_doubling(hipStream_t s, const double *L, long lda, const double *Winv, int p0, int w, double *Wb, long ldw, double *tmp,
-                           int max_wgs)
-{
-    require(w > 0 && w % NB == 0 && p0 % NB == 0 && ldw >= w && ldw % 2 == 0 && lda % 2 == 0 && max_wgs >= 1,
-            "invert_block_doubling: bad shape");
-    const int nb = w / NB;
-    hipLaunchKernelGGL(k_inv_place, dim3(std::min(nb * (nb + 1) / 2, max_wgs)), dim3(THREADS), 0, s, Winv + static_cast<long>(p0 / NB) * NB * NB,
-                       nb, Wb, ldw);
-    GP_CHECK_LAUNCH();
-    for (int size = NB; size < w; size *= 2)
-    {
-        const int pairs = static_cast<int>(ceil_div(static_cast<long>(w - size), 2L * size));  // t0 = 2 size p with t0 + size < w
-        const int nbs = size / NB;
-        for (int mode = 0; mode < 2; mode++)
-        {
-            hipLaunchKernelGGL(k_inv_join, dim3(pairs * nbs * nbs), dim3(THREADS), 0, s, L, lda, static_cast<long>(p0), w, size, Wb, ldw, tmp,
-                               mode);
-            GP_CHECK_LAUNCH();
-        }
-    }
-}
-
-void panel_solve64(hipStream_t s, double *A, long lda, long r0, long c0, int rows, const double *W)
-{
-    require(rows > 0 && rows % NB == 0 && c0 % NB == 0 && lda % 2 == 0, "panel_solve64: bad shape");
-    hipLaunchKernelGGL(k_panel_solve64, dim3(rows / NB), dim3(THREADS), 0, s, A, lda, r0, c0, W);
-    GP_CHECK_LAUNCH();
-}
-
-void panel_update64(hipStream_t s, double *A, long lda, long rs, long c0, int rows, int max_wgs)
-{
-    require(rows > 0 && rows % NB == 0 && c0 % NB == 0 && rs % NB == 0 && lda % 2 == 0 && max_wgs >= 1, "panel_update64: bad shape");
-    const int nb = rows / NB, pairs = nb * (nb + 1) / 2;
-    hipLaunchKernelGGL(k_panel_update64, dim3(std::min(pairs, max_wgs)), dim3(THREADS), 0, s, A, lda, rs, c0, nb);
+    require(p0 >= 0 && p0 % NB == 0 && nb >= 1 && lda % 2 == 0 && ldw % 2 == 0 && ldw >= static_cast<long>(nb) * NB && groups >= 1,
+            "panel_tasks: bad shape");
+    PanelArgs a{ A, lda, p0, Winv, Wb, ldw, tmp, info, reinterpret_cast<const int4 *>(tasks), wg_off, nb };
+    hipLaunchKernelGGL(k_panel_tasks, dim3(groups), dim3(THREADS), 0, s, a);
     GP_CHECK_LAUNCH();
 }
 

@@ -175,28 +175,16 @@ void batch_fit(hipStream_t s, const BatchGP *gps, const int *order, int count, i
 
 // ---- triangular family (k_triangular.hip) ---------------------------------
 
-// Factor the 64x64 diagonal block at (c0, c0) in place (lower triangle) and
-// write its inverse W = L_cc^-1 (64x64 row-major, zero upper) to w. info
-// (device int) takes the smallest global column whose pivot is not positive.
-void potrf64_inv(hipStream_t s, double *A, long lda, int c0, double *w, int *info);
-// Inside a diagonal block, after potrf64_inv of the step at column c0:
-// panel_solve64: X = A W^T in place for A[r0 : r0 + rows, c0 : c0 + 64], W
-// the step's 64x64 row-major inverse (one workgroup per 64 rows);
-// panel_update64:
-// A[rs + i, rs + j] -= sum_k X[i][k] X[j][k] for the lower (i >= j) part of
-// the rows x rows region at (rs, rs), X = A[rs :, c0 : c0 + 64], on at most
-// max_wgs workgroups. rows, c0 and rs are multiples of 64.
-// Wb (k-major, W(r, c) at Wb[c * ldw + r], upper part zero) = inverse of
-// the factored w x w diagonal block L[p0 :, p0 :], built by doubling from
-// the 64-wide row-major inverses in Winv: W_ba = -W_b L_ba W_a for adjacent
-// halves of 64, 128, 256, ... columns. tmp holds at least (w/2)^2 doubles.
-// The diagonal placement and zero fill run on at most max_wgs workgroups;
-// the joins use one workgroup per 64x64 output block, at most (w/128)^2 per
-// launch.
-void invert_block_doubling(hipStream_t s, const double *L, long lda, const double *Winv, int p0, int w, double *Wb, long ldw, double *tmp,
-                           int max_wgs);
-void panel_solve64(hipStream_t s, double *A, long lda, long r0, long c0, int rows, const double *W);
-void panel_update64(hipStream_t s, double *A, long lda, long rs, long c0, int rows, int max_wgs);
+// One launch of a diagonal block's task schedule (panel_plan.hpp): the
+// nb x nb block diagonal block of A at (p0, p0) is factored in place (lower
+// triangle), the row-major inverses of its 64x64 diagonal blocks go to
+// Winv + (p0 / 64 + k) * 4096 and its whole inverse, k-major (W(r, c) at
+// Wb[c * ldw + r], zero above the diagonal), to Wb; tmp holds the plan's
+// tmp_doubles. tasks (4 ints each) and wg_off (groups + 1 offsets) are
+// device copies of the plan's arrays for this launch. info (device int)
+// takes the smallest global column whose pivot is not positive.
+void panel_tasks(hipStream_t s, double *A, long lda, long p0, int nb, double *Winv, double *Wb, long ldw, double *tmp, int *info,
+                 const int *tasks, const int *wg_off, int groups);
 
 // Block steps of the vector solves with the saved diagonal-block inverses
 // Winv (block J at Winv + J*64*64). Forward: out[J] = W_JJ rhs[J],
