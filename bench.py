@@ -45,12 +45,15 @@ def outer_block(n_pad: int) -> int:
     env = os.environ.get("GPRAT_CHOLESKY_BLOCK")
     if env and int(env) >= 128 and int(env) % 128 == 0:
         return int(env)
-    return 256 if n_pad <= 4096 else 512
+    return 256 if n_pad <= 4096 else (512 if n_pad < 32768 else 768)
 
 
 def trailing_update_flops(n_pad: int):
     """Algorithmic flops of each trailing-update launch, in launch order:
-    2 * K * (number of lower-triangle elements updated)."""
+    2 * K * (number of elements updated). Step k's launch applies panel k
+    (K = its width) to the rows below the next diagonal block: the next
+    panel's columns whole, the columns after it lower (the next diagonal
+    block itself is updated on the panel stream)."""
     bo = outer_block(n_pad)
     out = []
     p0 = 0
@@ -59,12 +62,10 @@ def trailing_update_flops(n_pad: int):
         if pe >= n_pad:
             break
         q1 = min(n_pad, pe + bo)
-        K = pe - p0
-        M, N = n_pad - pe, q1 - pe
-        out.append(2.0 * K * (N * M - N * (N - 1) / 2))
         if q1 < n_pad:
+            K = pe - p0
             S = n_pad - q1
-            out.append(2.0 * K * (S * (S + 1) / 2))
+            out.append(2.0 * K * (S * (q1 - pe) + S * (S + 1) / 2))
         p0 = pe
     return out
 

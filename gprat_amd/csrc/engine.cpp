@@ -284,10 +284,11 @@ static int outer_block(int n_pad)
         if (v >= 128 && v % 128 == 0)
             return v;
     }
-    // measured at n = 16384 / 32768 (s): 512-wide panels 39.1 ms / 231.4 ms,
-    // 1024-wide 39.7 ms / 221.4 ms; the wider panel's K = 1024 trailing
-    // update pays once it dominates the diagonal-block chain
-    return n_pad <= 4096 ? 256 : (n_pad < 32768 ? 512 : 1024);
+    // factorisation measured at n = 16384 / 32768: 512-wide panels
+    // 35.6 / 210.6 ms, 768-wide 36.1 / 204.9 ms, 1024-wide 39.2 / 207.2 ms;
+    // wider panels give the trailing update a longer K but lengthen the
+    // diagonal-block chain and the panel stream's head work
+    return n_pad <= 4096 ? 256 : (n_pad < 32768 ? 512 : 768);
 }
 
 // Workgroup slots (out of two per CU) that look-ahead launches on the
@@ -301,15 +302,26 @@ static int trailing_reserve()
     return 16;
 }
 
-// C[i][j] -= sum_k P[k][off + i] P[k][off + j] for i < M, j < N, i >= j, with
-// P the k-major copy of a factored panel (leading dimension ldp).
-static void trailing_kmajor(hipStream_t s, const double *P, long ldp, long off, double *C, long ldc, int M, int N, int K,
-                            int max_wgs)
+// Workgroups of the panel stream's head launches (the next diagonal
+// block's rows of the panel and its update); GPRAT_HEAD_WORKGROUPS
+// overrides.
+static int head_workgroups()
+{
+    if (const char *e = std::getenv("GPRAT_HEAD_WORKGROUPS"))
+        return std::max(1, std::atoi(e));
+    return 64;
+}
+
+// C[i][j] -= sum_k P[k][a_off + i] P[k][b_off + j] for i < M, j < N,
+// i + diag_off >= j, with P the k-major copy of a factored panel (leading
+// dimension ldp).
+static void trailing_kmajor(hipStream_t s, const double *P, long ldp, long a_off, long b_off, long diag_off, double *C, long ldc, int M, int N,
+                            int K, int max_wgs, int *queue)
 {
     k::GemmArgs g{};
-    g.A = P + off;
+    g.A = P + a_off;
     g.lda = ldp;
-    g.B = P + off;
+    g.B = P + b_off;
     g.ldb = ldp;
     g.C = C;
     g.ldc = ldc;
@@ -319,8 +331,10 @@ static void trailing_kmajor(hipStream_t s, const double *P, long ldp, long off, 
     g.a_trans = true;
     g.b_trans = false;
     g.lower = true;
+    g.diag_off = diag_off;
     g.role = k::GemmRole::TrailingT;
     g.max_wgs = max_wgs;
+    g.queue = queue;
     k::gemm_f64(s, g);
 }
 
@@ -332,10 +346,24 @@ const Context::PanelPlanDev &Context::panel_plan(int nb, int chain_wgs, int tail
     chain_wgs = std::clamp(chain_wgs, 1, 4096);
     tail_wgs = std::clamp(tail_wgs, 1, 4096);
     const long key = (static_cast<long>(nb) << 26) | (static_cast<long>(chain_wgs) << 13) | tail_wgs;
+    return cached_plan(key, [&] { return build_panel_plan(nb, chain_wgs, tail_wgs); });
+}
+
+const Context::PanelPlanDev &Context::head_plan(int nbh, int nb, int wgs)
+{
+    std::lock_guard<std::recursive_mutex> g(mu);
+    wgs = std::clamp(wgs, 1, 4096);
+    const long key = (1L << 50) | (static_cast<long>(nbh) << 26) | (static_cast<long>(nb) << 13) | wgs;
+    return cached_plan(key, [&] { return build_head_plan(nbh, nb, wgs); });
+}
+
+const Context::PanelPlanDev &Context::cached_plan(long key, const std::function<PanelPlan()> &build)
+{
+    std::lock_guard<std::recursive_mutex> g(mu);
     auto it = panel_plans.find(key);
     if (it != panel_plans.end())
         return it->second;
-    const PanelPlan p = build_panel_plan(nb, chain_wgs, tail_wgs);
+    const PanelPlan p = build();
     PanelPlanDev d;
     d.tasks = DevBuf<int>(*this, p.tasks.size() * 4);
     d.wg_off = DevBuf<int>(*this, p.wg_off.size());
@@ -381,29 +409,41 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
     else
         p64 = DevBuf<double>(c, static_cast<std::size_t>(Bo) * n_pad);
     // inverses of the panels' diagonal blocks: kept (Wpanel, block kb at
-    // kb * Bo * Bo) or one scratch block reused by every panel
+    // kb * Bo * Bo) or two scratch blocks used by alternate panels (the
+    // panel stream factors panel k+2 only after the trailing stream's last
+    // read of panel k's inverse)
     const int chain_wgs = ahead ? std::max(1, trailing_reserve()) : 4096, tail_wgs = ahead ? 64 : 4096;
+    const int head_wgs = ahead ? head_workgroups() : 4096;
     DevBuf<double> Wscratch, tmp(c, static_cast<std::size_t>(std::max(1L, c.panel_plan(Bo / 64, chain_wgs, tail_wgs).tmp_doubles)));
     DevBuf<double> Xs(c, static_cast<std::size_t>(Bo) * n_pad);  // the panel's rows below, before the copy back
+    DevBuf<double> Xh(c, static_cast<std::size_t>(Bo) * Bo);     // the next diagonal block's rows of the panel, solved
+    DevBuf<int> queues(c, static_cast<std::size_t>(nblk));          // tile counters of the capped trailing updates, one per step
     if (!Wpanel)
-        Wscratch = DevBuf<double>(c, static_cast<std::size_t>(Bo) * Bo);
-    auto Wblock = [&](int p0) { return Wpanel ? Wpanel + static_cast<long>(p0 / Bo) * Bo * Bo : Wscratch.get(); };
+        Wscratch = DevBuf<double>(c, static_cast<std::size_t>(2) * Bo * Bo);
+    auto Wblock = [&](int p0)
+    {
+        const long kb = p0 / Bo;
+        return Wpanel ? Wpanel + kb * Bo * Bo : Wscratch.get() + (kb % 2) * Bo * Bo;
+    };
 
     std::vector<hipEvent_t> used;
-    auto ev = [&]()
+    auto record = [&](hipStream_t s)
     {
         hipEvent_t e = c.event();
         used.push_back(e);
+        GP_HIP(hipEventRecord(e, s));
         return e;
     };
-    auto hand_over = [&](hipStream_t from, hipStream_t to)
+    auto wait = [&](hipStream_t s, hipEvent_t e, hipStream_t from)
     {
-        if (from == to)
-            return;
-        hipEvent_t e = ev();
-        GP_HIP(hipEventRecord(e, from));
-        GP_HIP(hipStreamWaitEvent(to, e, 0));
+        if (e && s != from)
+            GP_HIP(hipStreamWaitEvent(s, e, 0));
     };
+    k::PanelTaskArgs args{};
+    args.A = A;
+    args.lda = lda;
+    args.Winv = Winv;
+    args.info = c.info.get();
     // The diagonal block: its factor (in place), the inverses of its 64x64
     // blocks (Winv) and its whole inverse (Wblock), as the task schedule's
     // launches; the chain launches stay within the slots the capped
@@ -411,23 +451,48 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
     auto diagonal = [&](int p0, int pe)
     {
         Timed t(c, P, "panel");
-        const int nbk = (pe - p0) / 64;
-        const auto &plan = c.panel_plan(nbk, chain_wgs, tail_wgs);
+        k::PanelTaskArgs a = args;
+        a.p0 = p0;
+        a.nb = (pe - p0) / 64;
+        a.Wb = Wblock(p0);
+        a.ldw = Bo;
+        a.tmp = tmp.get();
+        const auto &plan = c.panel_plan(a.nb, chain_wgs, tail_wgs);
         for (const auto &l : plan.launches)
-            k::panel_tasks(P, A, lda, p0, nbk, Winv, Wblock(p0), Bo, tmp.get(), c.info.get(), plan.tasks.get(), plan.wg_off.get() + l.first,
-                           l.groups);
+            k::panel_tasks(P, a, plan.tasks.get(), plan.wg_off.get() + l.first, l.groups);
+    };
+    // Panel [p0, pe)'s contribution to the next diagonal block [pe, q1),
+    // on the panel stream: the block's rows of the panel solved with the
+    // panel's inverse into Xh (then `solved` is recorded: the trailing
+    // stream may overwrite those rows), and the block's update from them.
+    auto head = [&](int p0, int pe, int q1, hipEvent_t &solved)
+    {
+        Timed t(c, P, "panel");
+        k::PanelTaskArgs a = args;
+        a.p0 = pe;
+        a.nb = (q1 - pe) / 64;
+        a.hp0 = p0;
+        a.nbh = (pe - p0) / 64;
+        a.Wh = Wblock(p0);
+        a.ldwh = Bo;
+        a.Xh = Xh.get();
+        a.ldx = Bo;
+        const auto &plan = c.head_plan(a.nbh, a.nb, head_wgs);
+        require(plan.launches.size() == 2, "cholesky: unexpected head plan");
+        k::panel_tasks(P, a, plan.tasks.get(), plan.wg_off.get() + plan.launches[0].first, plan.launches[0].groups);
+        solved = record(P);
+        k::panel_tasks(P, a, plan.tasks.get(), plan.wg_off.get() + plan.launches[1].first, plan.launches[1].groups);
     };
     // The panel's rows below its diagonal block: X = A W^T with W the
     // inverse of the diagonal block (k-major), one launch into scratch (each
-    // column tile stops K at its last column: W is lower triangular), then
-    // one pass that writes the k-major copy for the trailing update and X
-    // back into the panel.
-    auto rows_below = [&](int p0, int pe)
+    // column tile stops K at its last column: W is lower triangular), then,
+    // once the panel stream has read the next block's rows (`solved`), one
+    // pass that writes the k-major copy for the trailing update and X back
+    // into the panel.
+    auto rows_below = [&](int p0, int pe, hipEvent_t solved)
     {
         Timed t(c, T, "panel_rows");
         const int w = pe - p0, R = n_pad - pe;
-        if (R <= 0)
-            return;
         double *panel = A + static_cast<long>(pe) * lda + p0;
         k::GemmArgs g{};
         g.A = panel;
@@ -443,46 +508,54 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
         g.k_to_col = true;
         g.role = k::GemmRole::PanelX;
         k::gemm_f64(T, g);
+        wait(T, solved, P);
         if (mixed)
             k::panel_to_f32t(T, Xs.get(), w, 0, 0, R, w, p32.get(), n_pad, panel, lda);
         else
             k::panel_to_f64t(T, Xs.get(), w, 0, 0, R, w, p64.get(), n_pad, panel, lda);
     };
-    auto trailing = [&](long off, double *C, int M, int N, int K)
+    // Panel [p0, pe)'s update of everything below the next diagonal block:
+    // rows q1 .., columns pe .. (the next panel's columns whole, the rest
+    // lower), one capped launch.
+    auto trailing = [&](int p0, int pe, int q1)
     {
         Timed t(c, T, "syrk_trailing");
+        double *C = A + static_cast<long>(q1) * lda + pe;
+        const int M = n_pad - q1, N = n_pad - pe, K = pe - p0;
         if (mixed)
-            k::gemm_f32_trailing(T, p32.get(), n_pad, off, off, C, lda, M, N, K, 0, cap);
+            k::gemm_f32_trailing(T, p32.get(), n_pad, q1 - pe, 0, C, lda, M, N, K, q1 - pe, cap, queues.get() + p0 / Bo);
         else
-            trailing_kmajor(T, p64.get(), n_pad, off, C, lda, M, N, K, cap);
+            trailing_kmajor(T, p64.get(), n_pad, q1 - pe, 0, q1 - pe, C, lda, M, N, K, cap, queues.get() + p0 / Bo);
     };
     {
         Timed whole(c, c.main, "cholesky");
-        hand_over(c.main, P);
-        hand_over(c.main, T);
-        {
-            const int pe = std::min(n_pad, Bo);
-            diagonal(0, pe);
-            hand_over(P, T);
-            if (pe < n_pad)
-                rows_below(0, pe);
-        }
+        wait(P, record(c.main), c.main);
+        wait(T, record(c.main), c.main);
+        GP_HIP(hipMemsetAsync(queues.get(), 0, sizeof(int) * nblk, T));
+        diagonal(0, std::min(n_pad, Bo));
+        // Step k: the panel stream prepares and factors block k+1 (its head
+        // needs step k-1's trailing update); the trailing stream solves
+        // panel k's rows (they need block k's inverse) and applies panel k
+        // below block k+1.
+        hipEvent_t trailed = nullptr;
         for (int kb = 0; kb + 1 < nblk; kb++)
         {
-            const int p0 = kb * Bo, pe = std::min(n_pad, p0 + Bo);
-            const int q1 = std::min(n_pad, pe + Bo);
-            // panel k+1's columns first, then hand its diagonal block over
-            trailing(0, A + static_cast<long>(pe) * lda + pe, n_pad - pe, q1 - pe, pe - p0);
-            hand_over(T, P);
+            const int p0 = kb * Bo, pe = std::min(n_pad, p0 + Bo), q1 = std::min(n_pad, pe + Bo);
+            hipEvent_t factored = record(P), solved = nullptr;
+            wait(P, trailed, T);
+            head(p0, pe, q1, solved);
             diagonal(pe, q1);
+            wait(T, factored, P);
+            rows_below(p0, pe, solved);
+            trailed = nullptr;
             if (q1 < n_pad)
-                trailing(q1 - pe, A + static_cast<long>(q1) * lda + q1, n_pad - q1, n_pad - q1, pe - p0);
-            hand_over(P, T);
-            if (q1 < n_pad)
-                rows_below(pe, q1);
+            {
+                trailing(p0, pe, q1);
+                trailed = record(T);
+            }
         }
-        hand_over(T, c.main);
-        hand_over(P, c.main);
+        wait(c.main, record(T), T);
+        wait(c.main, record(P), P);
     }
     c.sync();
     for (auto e : used)

@@ -3,6 +3,7 @@
 // blocked algorithms that schedule the kernels.
 #pragma once
 
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -118,6 +119,8 @@ struct Context
     };
     std::map<long, PanelPlanDev> panel_plans;
     const PanelPlanDev &panel_plan(int nb, int chain_wgs, int tail_wgs);
+    const PanelPlanDev &head_plan(int nbh, int nb, int wgs);
+    const PanelPlanDev &cached_plan(long key, const std::function<PanelPlan()> &build);
 
     void make_current() const;
     void sync();

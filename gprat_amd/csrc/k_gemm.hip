@@ -48,6 +48,9 @@ struct Params
     int set;
     int k_to_col;
     int k_to_row;
+    int paired;  // k_gemm_f64 of the panel rows / TRSM diagonal steps: two tiles per workgroup
+    int *queue;  // k_gemm_f64_capped: tile counter (0 at launch) or null for fixed shares
+    int tiles;   // k_gemm_f64_capped: tiles holding part of the update
 };
 
 // Per-thread geometry of one BK-deep slab of a 128-wide operand, fixed for
@@ -273,14 +276,42 @@ __device__ __forceinline__ void gemm_tile(const Params &p, long r0, long c0, dou
         store_quarter<false>(p, acc, cw, rb, cb, rows_a, rows_b, r0, c0);
 }
 
-// One workgroup per tile (grid x = column tiles, y = row tiles).
+// One workgroup per tile (grid x = column tiles, y = row tiles). The
+// panel rows (K ends at each column tile's last column) and the TRSM
+// diagonal steps (K ends at each row tile's last row) give tile t of n a K
+// range of (t + 1) tiles: when the grid exceeds the device's workgroup
+// slots (p.paired), a workgroup takes tiles t and n - 1 - t, so every
+// workgroup does the same work (grid halved along that axis); a grid that
+// fits in one round keeps one tile per workgroup (shorter critical tile).
 template <bool AT, bool BT, int ROLE>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gemm_f64(Params p)
 {
     __shared__ double As[2][BK * LDS_STRIDE];
     __shared__ double Bs[2][BK * LDS_STRIDE];
-    const long r0 = static_cast<long>(blockIdx.y) * BM;
-    const long c0 = static_cast<long>(blockIdx.x) * BN;
+    long r0 = static_cast<long>(blockIdx.y) * BM;
+    long c0 = static_cast<long>(blockIdx.x) * BN;
+    if constexpr (ROLE == static_cast<int>(GemmRole::PanelX) || ROLE == static_cast<int>(GemmRole::LeafT))
+    {
+        if (p.paired)
+        {
+            constexpr bool by_col = ROLE == static_cast<int>(GemmRole::PanelX);
+            const int n = by_col ? (p.N + BN - 1) / BN : (p.M + BM - 1) / BM;
+            const int a = by_col ? blockIdx.x : blockIdx.y, b = n - 1 - a;
+            if (!(p.lower && r0 + (BM - 1) + p.diag_off < c0))
+                gemm_tile<AT, BT>(p, r0, c0, As, Bs);
+            if (b <= a)
+                return;
+            if (by_col)
+                c0 = static_cast<long>(b) * BN;
+            else
+                r0 = static_cast<long>(b) * BM;
+            if (p.lower && r0 + (BM - 1) + p.diag_off < c0)
+                return;
+            __syncthreads();  // the first tile's LDS reads are done
+            gemm_tile<AT, BT>(p, r0, c0, As, Bs);
+            return;
+        }
+    }
     if (p.lower && r0 + (BM - 1) + p.diag_off < c0)
         return;  // tile entirely above the diagonal
     gemm_tile<AT, BT>(p, r0, c0, As, Bs);
@@ -288,54 +319,115 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
 
 // Capped launch: G workgroups share the tiles that hold part of the update
 // (for a lower update, only tiles touching the lower band), numbered row by
-// row: workgroup w takes tiles w, w + G, ..., so every workgroup gets the
-// same number of tiles within one and the tiles in flight together are
-// neighbours. The launch leaves workgroup slots free for kernels on other
-// streams. Two workgroups share a CU and their tiles take equal time, so left
-// alone they would reach their epilogues (memory-bound, no matrix-core work)
-// together. For C -= updates the upper half of the grid therefore starts
-// with the first half of its first tile's K range and finishes that tile's
-// second half last: its epilogues fall half a tile after the lower half's.
+// row, so the tiles in flight together are neighbours. The launch leaves
+// workgroup slots free for kernels on other streams. With a queue, each
+// workgroup takes the next tile number from a device counter when it is
+// ready, so one that starts late (its slot was still held by another
+// stream's kernel) or shares its CU takes fewer tiles; without one,
+// workgroup w takes tiles w, w + G, .... Two workgroups share a CU and their
+// tiles take equal time, so left alone they would reach their epilogues
+// (memory-bound, no matrix-core work) together. For C -= updates the tiles
+// numbered G/2 .. G-1 (the first tiles of the upper half of the grid) are
+// therefore run as two halves of their K range, the first at once and the
+// second when the workgroup has no other tile left: those epilogues fall
+// half a tile after the others'. Which tiles are split depends only on
+// their numbers, so the rounding of every element does not depend on which
+// workgroup ran it.
 template <bool AT, bool BT, int ROLE>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gemm_f64_capped(Params p)
 {
     __shared__ double As[2][BK * LDS_STRIDE];
     __shared__ double Bs[2][BK * LDS_STRIDE];
+    __shared__ int next;
     const int gx = (p.N + BN - 1) / BN;
     const int gy = (p.M + BM - 1) / BM;
     const int G = gridDim.x;
-    const bool stagger = !p.set && static_cast<int>(blockIdx.x) >= G / 2;
-    long first_r = -1, first_c = 0;
-    int rem = blockIdx.x;  // this workgroup's next tile, counted from the current row's first
-    for (int tr = 0; tr < gy; tr++)
+    long first_r = -1, first_c = 0;  // this workgroup's deferred second half
+    // column tiles of tile row tr that touch the update
+    auto row_tiles = [&](int tr)
     {
-        const long r0 = static_cast<long>(tr) * BM;
-        int nc = gx;
-        if (p.lower)
+        if (!p.lower)
+            return gx;
+        const long last = static_cast<long>(tr) * BM + (BM - 1) + p.diag_off;  // last column the band reaches
+        return last < 0 ? 0 : static_cast<int>(min(static_cast<long>(gx), last / BN + 1));
+    };
+    auto run = [&](int t, long r0, long c0)
+    {
+        if (p.set || t < G / 2 || t >= G)
+            gemm_tile<AT, BT>(p, r0, c0, As, Bs, 0);
+        else if (first_r < 0)
         {
-            const long last = r0 + (BM - 1) + p.diag_off;  // last column the band reaches in this tile row
-            nc = last < 0 ? 0 : static_cast<int>(min(static_cast<long>(gx), last / BN + 1));
+            first_r = r0;
+            first_c = c0;
+            gemm_tile<AT, BT>(p, r0, c0, As, Bs, 1);
         }
-        for (; rem < nc; rem += G)
+        else
         {
-            const long c0 = static_cast<long>(rem) * BN;
-            __syncthreads();  // the previous tile's last slab has been read by every wavefront
-            if (stagger && first_r < 0)
+            // a second split tile for this workgroup: both halves now
+            gemm_tile<AT, BT>(p, r0, c0, As, Bs, 1);
+            __syncthreads();
+            gemm_tile<AT, BT>(p, r0, c0, As, Bs, 2);
+        }
+    };
+    if (p.queue)
+    {
+        // tile numbers only grow, so the row walk is incremental
+        int tr = 0, row_first = 0, nc = gy > 0 ? row_tiles(0) : 0;
+        for (;;)
+        {
+            __syncthreads();  // the previous tile's LDS reads and `next` reads are done
+            if (threadIdx.x == 0)
+                next = atomicAdd(p.queue, 1);
+            __syncthreads();
+            const int t = next;
+            if (t >= p.tiles)
+                break;
+            while (t >= row_first + nc && tr + 1 < gy)
             {
-                first_r = r0;
-                first_c = c0;
-                gemm_tile<AT, BT>(p, r0, c0, As, Bs, 1);
+                row_first += nc;
+                nc = row_tiles(++tr);
             }
-            else
-                gemm_tile<AT, BT>(p, r0, c0, As, Bs, 0);
+            run(t, static_cast<long>(tr) * BM, static_cast<long>(t - row_first) * BN);
         }
-        rem -= nc;
+    }
+    else
+    {
+        int rem = blockIdx.x;  // this workgroup's next tile, counted from the current row's first
+        int row_first = 0;     // number of the current row's first tile
+        for (int tr = 0; tr < gy; tr++)
+        {
+            const int nc = row_tiles(tr);
+            for (; rem < nc; rem += G)
+            {
+                __syncthreads();  // the previous tile's last slab has been read by every wavefront
+                run(row_first + rem, static_cast<long>(tr) * BM, static_cast<long>(rem) * BN);
+            }
+            rem -= nc;
+            row_first += nc;
+        }
     }
     if (first_r >= 0)
     {
         __syncthreads();
         gemm_tile<AT, BT>(p, first_r, first_c, As, Bs, 2);
     }
+}
+
+// Compute units of the current device (queried once per device).
+int device_cus()
+{
+    static int cus[64] = {};
+    int dev = 0;
+    GP_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64)
+        return 256;
+    if (cus[dev] == 0)
+    {
+        int v = 0;
+        GP_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev));
+        cus[dev] = v > 0 ? v : 256;
+    }
+    return cus[dev];
 }
 
 }  // namespace
@@ -352,7 +444,7 @@ void gemm_f64(hipStream_t s, const GemmArgs &g)
     require(reinterpret_cast<uintptr_t>(g.A) % 16 == 0 && reinterpret_cast<uintptr_t>(g.B) % 16 == 0,
             "gemm_f64: operands must be 16-byte aligned");
     Params p{ g.A, g.lda, g.B, g.ldb, g.C, g.ldc, g.M, g.N, g.K, g.lower ? 1 : 0, g.diag_off, g.k_from_tile ? 1 : 0,
-              g.kstart_off, g.set ? 1 : 0, g.k_to_col ? 1 : 0, g.k_to_row ? 1 : 0 };
+              g.kstart_off, g.set ? 1 : 0, g.k_to_col ? 1 : 0, g.k_to_row ? 1 : 0, 0, nullptr, 0 };
     dim3 grid(static_cast<unsigned>(ceil_div(g.N, BN)), static_cast<unsigned>(ceil_div(g.M, BM)));
     require(grid.y <= 65535, "gemm_f64: M too large for one launch");
     // tiles holding part of the update, counted as the capped kernel numbers them
@@ -366,9 +458,29 @@ void gemm_f64(hipStream_t s, const GemmArgs &g)
     {
         // only the trailing update runs capped
         require(g.a_trans && !g.b_trans && g.role == GemmRole::TrailingT, "gemm_f64: capped launch for this role");
+        require(tiles < (1L << 30), "gemm_f64: too many tiles");
+        p.queue = g.queue;
+        p.tiles = static_cast<int>(tiles);
         hipLaunchKernelGGL((k_gemm_f64_capped<true, false, 5>), dim3(static_cast<unsigned>(g.max_wgs)), dim3(THREADS), 0, s, p);
         GP_CHECK_LAUNCH();
         return;
+    }
+    // paired tiles (see k_gemm_f64) when the grid needs more than one
+    // round of the device's workgroup slots (two per CU): half the grid
+    // along the paired axis
+    if ((g.role == GemmRole::PanelX || g.role == GemmRole::LeafT) && tiles > 2L * device_cus())
+    {
+        if (g.role == GemmRole::PanelX)
+        {
+            require(g.k_to_col && !g.k_to_row, "gemm_f64: panel rows pair column tiles by their K range");
+            grid.x = (grid.x + 1) / 2;
+        }
+        else
+        {
+            require(g.k_to_row && !g.k_to_col, "gemm_f64: TRSM diagonal steps pair row tiles by their K range");
+            grid.y = (grid.y + 1) / 2;
+        }
+        p.paired = 1;
     }
     // Instantiated combinations: (N,N) panel rows; (T,N) with both operands
     // k-major for the updates, the TRSM steps and W^T W.

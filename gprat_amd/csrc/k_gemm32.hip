@@ -42,6 +42,8 @@ struct Params32
     int set;  // C = acc instead of C -= acc
     int max_wgs;  // > 0: at most this many workgroups, looping over tiles
     int k_to_row;  // A(i, k) known zero for k > i: each row tile stops K at its last row
+    int *queue;    // capped: tile counter (0 at launch) or null for fixed shares
+    int tiles;     // capped: tiles holding part of the update
 };
 
 // Element offsets (within one slab) of this thread's four 16-byte loads,
@@ -229,38 +231,76 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2))) vo
         return;
     }
     // Capped: the tiles holding part of the update (for a lower update, those
-    // touching the band) are numbered row by row and workgroup w takes w,
-    // w + G, ...: equal shares within one tile, neighbours in flight
-    // together. The upper half of the grid starts with the first K half of
-    // its first tile and ends with the second, so the two workgroups of a CU
-    // do not reach their epilogues together.
+    // touching the band) are numbered row by row, neighbours in flight
+    // together. With a queue, each workgroup takes the next tile number from
+    // a device counter when it is ready (one that starts late or shares its
+    // CU takes fewer); without one, workgroup w takes w, w + G, .... For C -=
+    // updates the tiles numbered G/2 .. G-1 run as two K halves, the second
+    // when the workgroup has nothing else left, so the two workgroups of a
+    // CU do not reach their epilogues together; the choice depends only on
+    // tile numbers, so rounding does not depend on scheduling.
+    __shared__ int next;
     const int G = gridDim.x;
-    const bool stagger = !p.set && static_cast<int>(blockIdx.x) >= G / 2;
-    long first_r = -1, first_c = 0;
-    int rem = blockIdx.x;  // this workgroup's next tile, counted from the current row's first
-    for (int tr = 0; tr < gy; tr++)
+    long first_r = -1, first_c = 0;  // this workgroup's deferred second half
+    auto row_tiles = [&](int tr)
     {
-        const long r0 = static_cast<long>(tr) * BM;
-        int nc = gx;
-        if (p.lower)
+        if (!p.lower)
+            return gx;
+        const long last = static_cast<long>(tr) * BM + (BM - 1) + p.diag_off;
+        return last < 0 ? 0 : static_cast<int>(min(static_cast<long>(gx), last / BM + 1));
+    };
+    auto run = [&](int t, long r0, long c0)
+    {
+        if (p.set || t < G / 2 || t >= G)
+            gemm_tile32<TC>(p, r0, c0, As, Bs, 0);
+        else if (first_r < 0)
         {
-            const long last = r0 + (BM - 1) + p.diag_off;
-            nc = last < 0 ? 0 : static_cast<int>(min(static_cast<long>(gx), last / BM + 1));
+            first_r = r0;
+            first_c = c0;
+            gemm_tile32<TC>(p, r0, c0, As, Bs, 1);
         }
-        for (; rem < nc; rem += G)
+        else
         {
-            const long c0 = static_cast<long>(rem) * BM;
-            __syncthreads();  // the previous tile's last slab has been read by every wavefront
-            if (stagger && first_r < 0)
+            gemm_tile32<TC>(p, r0, c0, As, Bs, 1);
+            __syncthreads();
+            gemm_tile32<TC>(p, r0, c0, As, Bs, 2);
+        }
+    };
+    if (p.queue)
+    {
+        int tr = 0, row_first = 0, nc = gy > 0 ? row_tiles(0) : 0;
+        for (;;)
+        {
+            __syncthreads();  // the previous tile's LDS reads and `next` reads are done
+            if (threadIdx.x == 0)
+                next = atomicAdd(p.queue, 1);
+            __syncthreads();
+            const int t = next;
+            if (t >= p.tiles)
+                break;
+            while (t >= row_first + nc && tr + 1 < gy)
             {
-                first_r = r0;
-                first_c = c0;
-                gemm_tile32<TC>(p, r0, c0, As, Bs, 1);
+                row_first += nc;
+                nc = row_tiles(++tr);
             }
-            else
-                gemm_tile32<TC>(p, r0, c0, As, Bs, 0);
+            run(t, static_cast<long>(tr) * BM, static_cast<long>(t - row_first) * BM);
         }
-        rem -= nc;
+    }
+    else
+    {
+        int rem = blockIdx.x;  // this workgroup's next tile, counted from the current row's first
+        int row_first = 0;
+        for (int tr = 0; tr < gy; tr++)
+        {
+            const int nc = row_tiles(tr);
+            for (; rem < nc; rem += G)
+            {
+                __syncthreads();  // the previous tile's last slab has been read by every wavefront
+                run(row_first + rem, static_cast<long>(tr) * BM, static_cast<long>(rem) * BM);
+            }
+            rem -= nc;
+            row_first += nc;
+        }
     }
     if (first_r >= 0)
     {
@@ -343,24 +383,28 @@ static void launch_f32_tn(hipStream_t s, const Params32<TC> &p)
     if (active == 0)
         return;
     Params32<TC> q = p;
+    q.tiles = static_cast<int>(std::min(active, (1L << 30)));
     if (q.max_wgs >= active)
+    {
         q.max_wgs = 0;  // the cap does not bind: one workgroup per tile
+        q.queue = nullptr;
+    }
     dim3 grid(static_cast<unsigned>(q.max_wgs > 0 ? q.max_wgs : gx * gy));
     hipLaunchKernelGGL((k_gemm_f32_tn<TC, ROLE>), grid, dim3(THREADS), 0, s, q);
     GP_CHECK_LAUNCH();
 }
 
 void gemm_f32_trailing(hipStream_t s, const float *P, long ldp, long a_off, long b_off, double *C, long ldc, int M, int N, int K, long diag_off,
-                       int max_wgs)
+                       int max_wgs, int *queue)
 {
-    Params32<double> p{ P + a_off, ldp, P + b_off, ldp, C, ldc, M, N, K, 1, diag_off, 0, max_wgs, 0 };
+    Params32<double> p{ P + a_off, ldp, P + b_off, ldp, C, ldc, M, N, K, 1, diag_off, 0, max_wgs, 0, queue, 0 };
     launch_f32_tn<double, 0>(s, p);
 }
 
 void gemm_f32_tn(hipStream_t s, const float *A, long lda, const float *B, long ldb, float *C, long ldc, int M, int N, int K, bool set,
                  int max_wgs, bool a_lower)
 {
-    Params32<float> p{ A, lda, B, ldb, C, ldc, M, N, K, 0, 0, set ? 1 : 0, max_wgs, a_lower ? 1 : 0 };
+    Params32<float> p{ A, lda, B, ldb, C, ldc, M, N, K, 0, 0, set ? 1 : 0, max_wgs, a_lower ? 1 : 0, nullptr, 0 };
     if (set)
         launch_f32_tn<float, 1>(s, p);
     else

@@ -44,26 +44,18 @@ __device__ __forceinline__ void load_lower_block(const double *__restrict__ A, l
 // task of its launch touches, so the tasks of a launch need no ordering.
 struct PanelArgs
 {
-    double *A;
-    long lda;
-    long p0;
-    double *Winv;
-    double *Wb;
-    long ldw;
-    double *tmp;
-    int *info;
+    PanelTaskArgs h;  // A, p0, Winv, Wb, tmp, info, head operands
     const int4 *tasks;
     const int *wg_off;
-    int nb;
 };
 
 __device__ __forceinline__ double *blockA(const PanelArgs &a, int i, int j)
 {
-    return a.A + (a.p0 + static_cast<long>(NB) * i) * a.lda + a.p0 + NB * j;
+    return a.h.A + (a.h.p0 + static_cast<long>(NB) * i) * a.h.lda + a.h.p0 + NB * j;
 }
-__device__ __forceinline__ double *blockWinv(const PanelArgs &a, int k) { return a.Winv + (a.p0 / NB + k) * static_cast<long>(NB * NB); }
+__device__ __forceinline__ double *blockWinv(const PanelArgs &a, int k) { return a.h.Winv + (a.h.p0 / NB + k) * static_cast<long>(NB * NB); }
 // k-major W block (i, j): element (r, c) at [c * ldw + r]
-__device__ __forceinline__ double *blockW(const PanelArgs &a, int i, int j) { return a.Wb + static_cast<long>(NB) * j * a.ldw + NB * i; }
+__device__ __forceinline__ double *blockW(const PanelArgs &a, int i, int j) { return a.h.Wb + static_cast<long>(NB) * j * a.h.ldw + NB * i; }
 
 // FACTOR / UFACTOR: block (k, k) (minus L_kq L_kq^T when q >= 0) factored
 // and inverted in LDS; L to A (lower triangle), the inverse to Winv_k
@@ -73,10 +65,10 @@ __device__ __noinline__ void task_factor(const PanelArgs &a, int k, int q, doubl
     double(*D)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(buf0);
     double(*Wl)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(buf1);
     double *blk = blockA(a, k, k);
-    load_lower_block(blk, a.lda, 0, D);
+    load_lower_block(blk, a.h.lda, 0, D);
     if (q >= 0)
     {
-        blk::stage_block(buf1, blockA(a, k, q), a.lda);
+        blk::stage_block(buf1, blockA(a, k, q), a.h.lda);
         __syncthreads();
         blk::v4d acc[2][2];
         blk::mfma_nt(buf1, buf1, acc);
@@ -88,13 +80,13 @@ __device__ __noinline__ void task_factor(const PanelArgs &a, int k, int q, doubl
                       });
     }
     __syncthreads();  // D final; every read of buf1 done before W lands there
-    factor_block64(D, Wl, R, a.info, static_cast<int>(a.p0) + NB * k);
+    factor_block64(D, Wl, R, a.h.info, static_cast<int>(a.h.p0) + NB * k);
     double *w = blockWinv(a, k);
     for (int e = threadIdx.x; e < NB * NB; e += THREADS)
     {
         const int r = e / NB, c = e % NB;
         if (c <= r)
-            blk[static_cast<long>(r) * a.lda + c] = D[r][c];
+            blk[static_cast<long>(r) * a.h.lda + c] = D[r][c];
         w[e] = (c <= r) ? Wl[r][c] : 0.0;
     }
 }
@@ -103,24 +95,24 @@ __device__ __noinline__ void task_factor(const PanelArgs &a, int k, int q, doubl
 __device__ __noinline__ void task_solve(const PanelArgs &a, int i, int k, double *buf0, double *buf1)
 {
     double *p = blockA(a, i, k);
-    blk::stage_block(buf0, p, a.lda);
+    blk::stage_block(buf0, p, a.h.lda);
     blk::stage_block(buf1, blockWinv(a, k), NB);
     __syncthreads();
     blk::v4d acc[2][2];
     blk::mfma_nt(buf0, buf1, acc);
-    blk::store_acc(p, a.lda, acc, true, false);
+    blk::store_acc(p, a.h.lda, acc, true, false);
 }
 
 // A_ij -= L_ik L_jk^T (lower half of a diagonal block).
 __device__ __noinline__ void task_update(const PanelArgs &a, int i, int j, int k, double *buf0, double *buf1)
 {
-    blk::stage_block(buf0, blockA(a, i, k), a.lda);
+    blk::stage_block(buf0, blockA(a, i, k), a.h.lda);
     if (i != j)
-        blk::stage_block(buf1, blockA(a, j, k), a.lda);
+        blk::stage_block(buf1, blockA(a, j, k), a.h.lda);
     __syncthreads();
     blk::v4d acc[2][2];
     blk::mfma_nt(buf0, i != j ? buf1 : buf0, acc);
-    blk::sub_acc(blockA(a, i, j), a.lda, acc, i == j);
+    blk::sub_acc(blockA(a, i, j), a.h.lda, acc, i == j);
 }
 
 // W_kk = Winv_k (transposed through LDS: both sides coalesced), W_kj = 0
@@ -133,13 +125,13 @@ __device__ __noinline__ void task_place(const PanelArgs &a, int k, double *buf0)
     for (int e = threadIdx.x; e < NB * NB; e += THREADS)
     {
         const int c = e >> 6, r = e & 63;
-        out[static_cast<long>(c) * a.ldw + r] = buf0[r * blk::XS + c];
+        out[static_cast<long>(c) * a.h.ldw + r] = buf0[r * blk::XS + c];
     }
-    for (int j = k + 1; j < a.nb; j++)
+    for (int j = k + 1; j < a.h.nb; j++)
     {
         double *z = blockW(a, k, j);
         for (int e = threadIdx.x; e < NB * NB; e += THREADS)
-            z[static_cast<long>(e >> 6) * a.ldw + (e & 63)] = 0.0;
+            z[static_cast<long>(e >> 6) * a.h.ldw + (e & 63)] = 0.0;
     }
 }
 
@@ -147,7 +139,7 @@ __device__ __noinline__ void task_place(const PanelArgs &a, int k, double *buf0)
 __device__ __noinline__ void task_joinf(const PanelArgs &a, int ka, double *buf0, double *buf1)
 {
     const int kb = ka + 1;
-    blk::stage_block(buf0, blockA(a, kb, ka), a.lda);  // [r][k] = L_ba(r, k)
+    blk::stage_block(buf0, blockA(a, kb, ka), a.h.lda);  // [r][k] = L_ba(r, k)
     blk::stage_block_t(buf1, blockWinv(a, ka), NB);    // [c][k] = W_a(k, c)
     __syncthreads();
     blk::v4d acc[2][2];
@@ -158,7 +150,7 @@ __device__ __noinline__ void task_joinf(const PanelArgs &a, int ka, double *buf0
     __syncthreads();
     blk::mfma_nt(buf0, buf1, acc);
     double *out = blockW(a, kb, ka);
-    blk::for_each(acc, [&](int r, int c, double v) { out[static_cast<long>(c) * a.ldw + r] = -v; });
+    blk::for_each(acc, [&](int r, int c, double v) { out[static_cast<long>(c) * a.h.ldw + r] = -v; });
 }
 
 // Doubling join at level h, pair p: a = blocks [2hp, 2hp + h), b = the
@@ -170,7 +162,7 @@ __device__ __noinline__ void task_join(const PanelArgs &a, int phase, int h, int
 {
     const int a0 = 2 * h * p, b0 = a0 + h;
     const long size = static_cast<long>(NB) * h;
-    double *tp = a.tmp + static_cast<long>(w) * NB * NB;
+    double *tp = a.h.tmp + static_cast<long>(w) * NB * NB;
     blk::v4d acc[2][2];
     blk::zero(acc);
     if (phase == 0)
@@ -178,8 +170,8 @@ __device__ __noinline__ void task_join(const PanelArgs &a, int phase, int h, int
         for (int s = cb; s < h; s++)
         {
             __syncthreads();
-            blk::stage_block(buf0, blockA(a, b0 + rb, a0 + s), a.lda);  // [r][k] = L(r, k)
-            blk::stage_block(buf1, blockW(a, a0 + s, a0 + cb), a.ldw);  // [c][k] = W(k, c)
+            blk::stage_block(buf0, blockA(a, b0 + rb, a0 + s), a.h.lda);  // [r][k] = L(r, k)
+            blk::stage_block(buf1, blockW(a, a0 + s, a0 + cb), a.h.ldw);  // [c][k] = W(k, c)
             __syncthreads();
             blk::mfma_nt_acc(buf0, buf1, acc);
         }
@@ -190,14 +182,53 @@ __device__ __noinline__ void task_join(const PanelArgs &a, int phase, int h, int
         for (int s = 0; s <= rb; s++)
         {
             __syncthreads();
-            blk::stage_block_t(buf0, blockW(a, b0 + rb, b0 + s), a.ldw);  // [r][k] = W(r, k)
+            blk::stage_block_t(buf0, blockW(a, b0 + rb, b0 + s), a.h.ldw);  // [r][k] = W(r, k)
             blk::stage_block(buf1, tp + (NB * cb) * size + NB * s, size);  // [c][k] = T(k, c)
             __syncthreads();
             blk::mfma_nt_acc(buf0, buf1, acc);
         }
         double *out = blockW(a, b0 + rb, a0 + cb);
-        blk::for_each(acc, [&](int r, int c, double v) { out[static_cast<long>(c) * a.ldw + r] = -v; });
+        blk::for_each(acc, [&](int r, int c, double v) { out[static_cast<long>(c) * a.h.ldw + r] = -v; });
     }
+}
+
+// Head of the diagonal block at p0 in the previous panel (columns hp0 ..,
+// nbh blocks): X_ic = sum_{t <= c} A(p0 + 64 i, hp0 + 64 t) W_P(c, t)^T,
+// W_P the panel's k-major inverse (lower), into Xh (row-major, ld ldx).
+__device__ __noinline__ void task_headx(const PanelArgs &a, int i, int c, double *buf0, double *buf1)
+{
+    const PanelTaskArgs &h = a.h;
+    const double *row = h.A + (h.p0 + static_cast<long>(NB) * i) * h.lda + h.hp0;
+    blk::v4d acc[2][2];
+    blk::zero(acc);
+    for (int t = 0; t <= c; t++)
+    {
+        __syncthreads();
+        blk::stage_block(buf0, row + NB * t, h.lda);                                           // [r][k] = A(r, k)
+        blk::stage_block_t(buf1, h.Wh + static_cast<long>(NB) * t * h.ldwh + NB * c, h.ldwh);  // [cc][k] = W_P(c, t)(cc, k)
+        __syncthreads();
+        blk::mfma_nt_acc(buf0, buf1, acc);
+    }
+    double *out = h.Xh + static_cast<long>(NB) * i * h.ldx + NB * c;
+    blk::store_acc(out, h.ldx, acc, true, false);
+}
+
+// A(p0 + 64 i, p0 + 64 j) -= sum_{t < nbh} X_it X_jt^T (lower half if i == j).
+__device__ __noinline__ void task_heads(const PanelArgs &a, int i, int j, double *buf0, double *buf1)
+{
+    const PanelTaskArgs &h = a.h;
+    blk::v4d acc[2][2];
+    blk::zero(acc);
+    for (int t = 0; t < h.nbh; t++)
+    {
+        __syncthreads();
+        blk::stage_block(buf0, h.Xh + static_cast<long>(NB) * i * h.ldx + NB * t, h.ldx);
+        if (i != j)
+            blk::stage_block(buf1, h.Xh + static_cast<long>(NB) * j * h.ldx + NB * t, h.ldx);
+        __syncthreads();
+        blk::mfma_nt_acc(buf0, i != j ? buf1 : buf0, acc);
+    }
+    blk::sub_acc(blockA(a, i, j), h.lda, acc, i == j);
 }
 
 __global__ __launch_bounds__(THREADS) void k_panel_tasks(PanelArgs a)
@@ -221,6 +252,10 @@ __global__ __launch_bounds__(THREADS) void k_panel_tasks(PanelArgs a)
             task_place(a, d.y, buf0);
         else if (type == PT_JOINF)
             task_joinf(a, d.y, buf0, buf1);
+        else if (type == PT_HEADX)
+            task_headx(a, d.y, d.z, buf0, buf1);
+        else if (type == PT_HEADS)
+            task_heads(a, d.y, d.z, buf0, buf1);
         else
             task_join(a, type == PT_JOIN0 ? 0 : 1, d.x >> 8, d.y, d.z & 0xffff, d.z >> 16, d.w, buf0, buf1);
     }
@@ -705,12 +740,14 @@ void to_f32(hipStream_t s, const double *src, float *dst, long count)
     GP_CHECK_LAUNCH();
 }
 
-void panel_tasks(hipStream_t s, double *A, long lda, long p0, int nb, double *Winv, double *Wb, long ldw, double *tmp, int *info,
-                 const int *tasks, const int *wg_off, int groups)
+void panel_tasks(hipStream_t s, const PanelTaskArgs &h, const int *tasks, const int *wg_off, int groups)
 {
-    require(p0 >= 0 && p0 % NB == 0 && nb >= 1 && lda % 2 == 0 && ldw % 2 == 0 && ldw >= static_cast<long>(nb) * NB && groups >= 1,
-            "panel_tasks: bad shape");
-    PanelArgs a{ A, lda, p0, Winv, Wb, ldw, tmp, info, reinterpret_cast<const int4 *>(tasks), wg_off, nb };
+    require(h.p0 >= 0 && h.p0 % NB == 0 && h.nb >= 1 && h.lda % 2 == 0 && groups >= 1, "panel_tasks: bad shape");
+    require(!h.Wb || (h.ldw % 2 == 0 && h.ldw >= static_cast<long>(h.nb) * NB), "panel_tasks: bad inverse layout");
+    require(!h.Xh || (h.hp0 >= 0 && h.hp0 % NB == 0 && h.nbh >= 1 && h.ldx % 2 == 0 && h.ldx >= static_cast<long>(h.nbh) * NB
+                      && h.ldwh % 2 == 0 && h.ldwh >= static_cast<long>(h.nbh) * NB && h.Wh),
+            "panel_tasks: bad head layout");
+    PanelArgs a{ h, reinterpret_cast<const int4 *>(tasks), wg_off };
     hipLaunchKernelGGL(k_panel_tasks, dim3(groups), dim3(THREADS), 0, s, a);
     GP_CHECK_LAUNCH();
 }

@@ -129,6 +129,10 @@ struct GemmArgs
     // > 0: at most this many workgroups, each looping over tiles, so the
     // launch leaves workgroup slots free for kernels on other streams.
     int max_wgs;
+    // With max_wgs: a device int holding 0 at launch; the workgroups then
+    // take tiles in order from this counter (a workgroup that starts late,
+    // or shares its CU, takes fewer) instead of a fixed share each.
+    int *queue = nullptr;
 };
 void gemm_f64(hipStream_t s, const GemmArgs &g);
 
@@ -142,9 +146,10 @@ void panel_to_f64t(hipStream_t s, const double *A, long lda, long r0, long p0, i
                    long ldback = 0);
 // C[i][j] -= sum_k P[k][a_off + i] P[k][b_off + j] (FP32 products and sums),
 // for i < M, j < N, i + diag_off >= j; K % 32 == 0.
-// max_wgs > 0 caps the launch (workgroups loop over tiles).
+// max_wgs > 0 caps the launch (workgroups loop over tiles; with a queue, a
+// device int holding 0 at launch, they take tiles in order from it).
 void gemm_f32_trailing(hipStream_t s, const float *P, long ldp, long a_off, long b_off, double *C, long ldc, int M, int N, int K, long diag_off,
-                       int max_wgs = 0);
+                       int max_wgs = 0, int *queue = nullptr);
 // C (= or -=) A B over FP32 operands, A(i,k) = A[k*lda + i], B(k,j) = B[k*ldb + j],
 // FP32 C; K % 32 == 0, M, N, lda, ldb multiples of 4, A and B 16-byte aligned.
 // a_lower: A(i, k) is known zero for k > i (each row tile stops K there)
@@ -180,11 +185,31 @@ void batch_fit(hipStream_t s, const BatchGP *gps, const int *order, int count, i
 // triangle), the row-major inverses of its 64x64 diagonal blocks go to
 // Winv + (p0 / 64 + k) * 4096 and its whole inverse, k-major (W(r, c) at
 // Wb[c * ldw + r], zero above the diagonal), to Wb; tmp holds the plan's
-// tmp_doubles. tasks (4 ints each) and wg_off (groups + 1 offsets) are
-// device copies of the plan's arrays for this launch. info (device int)
-// takes the smallest global column whose pivot is not positive.
-void panel_tasks(hipStream_t s, double *A, long lda, long p0, int nb, double *Winv, double *Wb, long ldw, double *tmp, int *info,
-                 const int *tasks, const int *wg_off, int groups);
+// tmp_doubles. info (device int) takes the smallest global column whose
+// pivot is not positive. Head tasks (build_head_plan) read the block's rows
+// of the previous panel (columns hp0 .., nbh blocks wide) and that panel's
+// k-major inverse Wh (ld ldwh), and use Xh (ld ldx, nb*64 x nbh*64) as
+// scratch. tasks (4 ints each) and wg_off (groups + 1 offsets) are device
+// copies of the plan's arrays for this launch.
+struct PanelTaskArgs
+{
+    double *A;
+    long lda;
+    long p0;
+    int nb;
+    double *Winv;
+    double *Wb;
+    long ldw;
+    double *tmp;
+    int *info;
+    long hp0;
+    int nbh;
+    const double *Wh;
+    long ldwh;
+    double *Xh;
+    long ldx;
+};
+void panel_tasks(hipStream_t s, const PanelTaskArgs &a, const int *tasks, const int *wg_off, int groups);
 
 // Block steps of the vector solves with the saved diagonal-block inverses
 // Winv (block J at Winv + J*64*64). Forward: out[J] = W_JJ rhs[J],

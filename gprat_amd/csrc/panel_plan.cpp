@@ -71,6 +71,32 @@ struct Launch
 
 PanelTask pt(int type, int y, int z = 0, int w = 0, int h = 0) { return PanelTask{ type | (h << 8), y, z, w }; }
 
+// Longest task first onto the least loaded workgroup; appends the launch
+// to the plan (nothing for an empty launch).
+void append_launch(PanelPlan &plan, Launch &l)
+{
+    if (l.tasks.empty())
+        return;
+    std::stable_sort(l.tasks.begin(), l.tasks.end(), [](const Task &a, const Task &b) { return a.cost > b.cost; });
+    const int g = std::min(static_cast<int>(l.tasks.size()), l.cap);
+    std::vector<double> load(g, 0.0);
+    std::vector<std::vector<PanelTask>> per(g);
+    for (const auto &t : l.tasks)
+    {
+        const int w = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+        load[w] += t.cost;
+        per[w].push_back(t.t);
+    }
+    PanelLaunch pl{ static_cast<int>(plan.wg_off.size()), g };
+    for (int w = 0; w < g; w++)
+    {
+        plan.wg_off.push_back(static_cast<int>(plan.tasks.size()));
+        plan.tasks.insert(plan.tasks.end(), per[w].begin(), per[w].end());
+    }
+    plan.wg_off.push_back(static_cast<int>(plan.tasks.size()));
+    plan.launches.push_back(pl);
+}
+
 }  // namespace
 
 PanelPlan build_panel_plan(int nb, int chain_wgs, int tail_wgs)
@@ -244,33 +270,30 @@ PanelPlan build_panel_plan(int nb, int chain_wgs, int tail_wgs)
         }
     }
 
-    // ---- workgroup assignment (longest task first onto the least loaded) --
     PanelPlan plan;
     plan.nb = nb;
     plan.tmp_doubles = tmp_blocks * 4096;
     for (auto &l : L)
-    {
-        if (l.tasks.empty())
-            continue;
-        std::stable_sort(l.tasks.begin(), l.tasks.end(), [](const Task &a, const Task &b) { return a.cost > b.cost; });
-        const int g = std::min(static_cast<int>(l.tasks.size()), l.cap);
-        std::vector<double> load(g, 0.0);
-        std::vector<std::vector<PanelTask>> per(g);
-        for (const auto &t : l.tasks)
-        {
-            const int w = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
-            load[w] += t.cost;
-            per[w].push_back(t.t);
-        }
-        PanelLaunch pl{ static_cast<int>(plan.wg_off.size()), g };
-        for (int w = 0; w < g; w++)
-        {
-            plan.wg_off.push_back(static_cast<int>(plan.tasks.size()));
-            plan.tasks.insert(plan.tasks.end(), per[w].begin(), per[w].end());
-        }
-        plan.wg_off.push_back(static_cast<int>(plan.tasks.size()));
-        plan.launches.push_back(pl);
-    }
+        append_launch(plan, l);
+    return plan;
+}
+
+PanelPlan build_head_plan(int nbh, int nb, int wgs)
+{
+    if (nbh < 1 || nb < 1 || nbh > 4096 || nb > 4096 || wgs < 1)
+        throw std::runtime_error("build_head_plan: bad arguments");
+    PanelPlan plan;
+    plan.nb = nb;
+    Launch x, u;
+    x.cap = u.cap = wgs;
+    for (int i = 0; i < nb; i++)
+        for (int c = 0; c < nbh; c++)
+            x.tasks.push_back(Task{ pt(PT_HEADX, i, c), cost_of(2 * (c + 1), c + 1), {}, {}, {}, {} });
+    for (int i = 0; i < nb; i++)
+        for (int j = 0; j <= i; j++)
+            u.tasks.push_back(Task{ pt(PT_HEADS, i, j), cost_of(2 * nbh, nbh, 1.5), {}, {}, {}, {} });
+    append_launch(plan, x);
+    append_launch(plan, u);
     return plan;
 }
 

@@ -22,6 +22,10 @@ enum PanelTaskType : int
     PT_JOINF = 5,    // (a): W_ba = -Winv_b L_ba Winv_a, b = a + 1
     PT_JOIN0 = 6,    // doubling level h, pair p, block (rb, cb): T = L_ba W_a (into tmp)
     PT_JOIN1 = 7,    // the same: W_ba = -W_b T
+    // head of the next diagonal block (its rows in the previous panel P,
+    // whose diagonal block has the k-major inverse W_P):
+    PT_HEADX = 8,  // (i, c): X_ic = sum_{t <= c} A_{i, P t} W_P(c, t)^T into scratch
+    PT_HEADS = 9,  // (i, j): A_ij -= sum_t X_it X_jt^T (lower half if i == j)
 };
 
 // Four ints per task (an int4 on the device):
@@ -30,7 +34,8 @@ enum PanelTaskType : int
 //   UPDATE y = i, z = j, w = k; PLACE y = k; JOINF y = a;
 //   JOIN0 / JOIN1 y = p (a = 2 h p, b = a + h), z = rb | cb << 16,
 //   w = offset of the pair's T in tmp, in 64x64 blocks (T(r, c) of the
-//   64h x 64h pair matrix at tmp[w * 4096 + c * 64 h + r]).
+//   64h x 64h pair matrix at tmp[w * 4096 + c * 64 h + r]);
+//   HEADX y = i, z = c; HEADS y = i, z = j.
 struct PanelTask
 {
     int x, y, z, w;
@@ -64,5 +69,11 @@ struct PanelPlan
 // workgroups, the later ones at most tail_wgs. Throws std::runtime_error
 // if the result violates its own dependency check.
 PanelPlan build_panel_plan(int nb, int chain_wgs, int tail_wgs);
+
+// The previous panel's (nbh blocks wide) contribution to the next
+// diagonal block (nb blocks): launch 0 the HEADX tasks (the block's rows of
+// the panel, solved), launch 1 the HEADS tasks (the rank-64 nbh update of
+// the block's lower half), each on at most wgs workgroups.
+PanelPlan build_head_plan(int nbh, int nb, int wgs);
 
 }  // namespace gpamd
