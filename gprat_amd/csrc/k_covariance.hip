@@ -48,6 +48,92 @@ __global__ __launch_bounds__(THREADS) void k_assemble_lower(const double *__rest
     }
 }
 
+// The same for window lengths R <= 16: a lane keeps its column's window in
+// registers, and the row's window (one row per wavefront) is read through
+// the scalar cache. Same operations in the same order as window_sqdist.
+template <int R>
+__global__ __launch_bounds__(THREADS) void k_assemble_lower_w(const double *__restrict__ x,
+                                                               double scale,
+                                                               double v,
+                                                               double noise,
+                                                               double *__restrict__ A,
+                                                               long lda,
+                                                               int n)
+{
+#pragma clang fp contract(off)
+    const int ti = blockIdx.y, tj = blockIdx.x;
+    if (tj > ti)
+        return;
+    const int c = threadIdx.x % TB;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / TB);
+    const int j = tj * TB + c;
+    const int jc = j < n ? j : n - 1;  // lanes past the end read a valid window and store the padding value
+    double xj[R];
+#pragma unroll
+    for (int k = 0; k < R; k++)
+        xj[k] = x[jc + k];
+    for (int rr = w; rr < TB; rr += THREADS / TB)
+    {
+        const int i = ti * TB + rr;
+        double val;
+        if (i < n && j < n)
+        {
+            double d = 0.0;
+#pragma unroll
+            for (int k = 0; k < R; k++)
+            {
+                const double t = x[i + k] - xj[k];
+                d += t * t;
+            }
+            val = se(scale, v, d);
+            if (i == j)
+                val += noise;
+        }
+        else
+            val = (i == j) ? 1.0 : 0.0;
+        A[static_cast<long>(i) * lda + j] = val;
+    }
+}
+
+template <class T, int R>
+__global__ __launch_bounds__(THREADS) void k_assemble_cross_t_w(const double *__restrict__ x,
+                                                                 const double *__restrict__ z,
+                                                                 double scale,
+                                                                 double v,
+                                                                 T *__restrict__ B,
+                                                                 long ldb,
+                                                                 int n,
+                                                                 int m)
+{
+#pragma clang fp contract(off)
+    const int ti = blockIdx.y, tj = blockIdx.x;
+    const int c = threadIdx.x % TB;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / TB);
+    const int j = tj * TB + c;  // test index
+    const int jc = j < m ? j : m - 1;
+    double zj[R];
+#pragma unroll
+    for (int k = 0; k < R; k++)
+        zj[k] = z[jc + k];
+    for (int rr = w; rr < TB; rr += THREADS / TB)
+    {
+        const int i = ti * TB + rr;  // train index
+        double val = 0.0;
+        if (i < n && j < m)
+        {
+            double d = 0.0;
+#pragma unroll
+            for (int k = 0; k < R; k++)
+            {
+                const double t = zj[k] - x[i + k];  // (test - train), as gp_algorithms.cpp:113-140
+                d += t * t;
+            }
+            val = se(scale, v, d);
+        }
+        B[static_cast<long>(i) * ldb + j] = static_cast<T>(val);
+    }
+}
+
 template <class T>
 __global__ __launch_bounds__(THREADS) void k_assemble_cross_t(const double *__restrict__ x,
                                                                const double *__restrict__ z,
@@ -351,11 +437,38 @@ void gradient_sums(hipStream_t s,
     GP_CHECK_LAUNCH();
 }
 
+template <int R>
+void launch_lower_w(hipStream_t s, dim3 grid, const double *x, double scale, double v, double noise, double *A, long lda, int n)
+{
+    hipLaunchKernelGGL(k_assemble_lower_w<R>, grid, dim3(THREADS), 0, s, x, scale, v, noise, A, lda, n);
+}
+using LowerLaunch = void (*)(hipStream_t, dim3, const double *, double, double, double, double *, long, int);
+constexpr LowerLaunch lower_w_table[16] = { launch_lower_w<1>,  launch_lower_w<2>,  launch_lower_w<3>,  launch_lower_w<4>,
+                                            launch_lower_w<5>,  launch_lower_w<6>,  launch_lower_w<7>,  launch_lower_w<8>,
+                                            launch_lower_w<9>,  launch_lower_w<10>, launch_lower_w<11>, launch_lower_w<12>,
+                                            launch_lower_w<13>, launch_lower_w<14>, launch_lower_w<15>, launch_lower_w<16> };
+
+template <class T, int R>
+void launch_cross_w(hipStream_t s, dim3 grid, const double *x, const double *z, double scale, double v, T *B, long ldb, int n, int m)
+{
+    hipLaunchKernelGGL((k_assemble_cross_t_w<T, R>), grid, dim3(THREADS), 0, s, x, z, scale, v, B, ldb, n, m);
+}
+template <class T>
+using CrossLaunch = void (*)(hipStream_t, dim3, const double *, const double *, double, double, T *, long, int, int);
+template <class T>
+constexpr CrossLaunch<T> cross_w_table[16] = { launch_cross_w<T, 1>,  launch_cross_w<T, 2>,  launch_cross_w<T, 3>,  launch_cross_w<T, 4>,
+                                               launch_cross_w<T, 5>,  launch_cross_w<T, 6>,  launch_cross_w<T, 7>,  launch_cross_w<T, 8>,
+                                               launch_cross_w<T, 9>,  launch_cross_w<T, 10>, launch_cross_w<T, 11>, launch_cross_w<T, 12>,
+                                               launch_cross_w<T, 13>, launch_cross_w<T, 14>, launch_cross_w<T, 15>, launch_cross_w<T, 16> };
+
 void assemble_lower(hipStream_t s, const double *x, int r, SEK sek, double *A, long lda, int n, int n_pad)
 {
     require(n_pad % TB == 0 && n <= n_pad && lda >= n_pad && r >= 1, "assemble_lower: bad shape");
     dim3 grid(n_pad / TB, n_pad / TB);
-    hipLaunchKernelGGL(k_assemble_lower, grid, dim3(THREADS), 0, s, x, r, sek.scale(), sek.vertical, sek.noise, A, lda, n);
+    if (r <= 16 && n >= 1 && n_pad < (1 << 30) - 16)
+        lower_w_table[r - 1](s, grid, x, sek.scale(), sek.vertical, sek.noise, A, lda, n);
+    else
+        hipLaunchKernelGGL(k_assemble_lower, grid, dim3(THREADS), 0, s, x, r, sek.scale(), sek.vertical, sek.noise, A, lda, n);
     GP_CHECK_LAUNCH();
 }
 
@@ -364,7 +477,10 @@ static void assemble_cross_t_impl(hipStream_t s, const double *x, const double *
 {
     require(n_pad % TB == 0 && m_pad % TB == 0 && n <= n_pad && m <= m_pad && ldb >= m_pad, "assemble_cross_t: bad shape");
     dim3 grid(m_pad / TB, n_pad / TB);
-    hipLaunchKernelGGL(k_assemble_cross_t<T>, grid, dim3(THREADS), 0, s, x, z, r, sek.scale(), sek.vertical, B, ldb, n, m);
+    if (r <= 16 && m >= 1 && n_pad < (1 << 30) - 16 && m_pad < (1 << 30) - 16)
+        cross_w_table<T>[r - 1](s, grid, x, z, sek.scale(), sek.vertical, B, ldb, n, m);
+    else
+        hipLaunchKernelGGL(k_assemble_cross_t<T>, grid, dim3(THREADS), 0, s, x, z, r, sek.scale(), sek.vertical, B, ldb, n, m);
     GP_CHECK_LAUNCH();
 }
 
