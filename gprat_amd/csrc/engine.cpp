@@ -756,19 +756,53 @@ static void blocked_solve(Context &c, const double *L, long lda, const typename 
 {
     using T = typename Ops::T;
     hipStream_t s = c.main;
+    // the k-major copies of L's panels are made on the panel stream, one
+    // step ahead into alternating buffers, so they overlap the diagonal
+    // steps (which fill half the device) instead of preceding each update
+    hipStream_t side = c.panel;
     const int nblk = static_cast<int>(ceil_div(n_pad, NBK));
-    DevBuf<T> V(c, static_cast<std::size_t>(NBK) * m_pad), P(c, static_cast<std::size_t>(NBK) * n_pad);
+    DevBuf<T> V(c, static_cast<std::size_t>(NBK) * m_pad);
+    DevBuf<T> P[2] = { DevBuf<T>(c, static_cast<std::size_t>(NBK) * n_pad), DevBuf<T>(c, static_cast<std::size_t>(NBK) * n_pad) };
+    std::vector<hipEvent_t> used;
+    auto record = [&](hipStream_t st)
+    {
+        hipEvent_t e = c.event();
+        used.push_back(e);
+        GP_HIP(hipEventRecord(e, st));
+        return e;
+    };
+    hipEvent_t freed[2] = { record(s), nullptr };  // the buffer's last update is done (side waits)
+    freed[1] = freed[0];
+    auto copy = [&](int J)
+    {
+        const int j0 = J * NBK, nb = std::min(NBK, n_pad - j0), j1 = j0 + nb;
+        if (side != s)
+            GP_HIP(hipStreamWaitEvent(side, freed[J % 2], 0));
+        Ops::panel(side, L, lda, j1, j0, n_pad - j1, nb, P[J % 2].get(), n_pad);
+        return record(side);
+    };
+    hipEvent_t ready = (NBK < n_pad) ? copy(0) : nullptr;
     for (int J = 0; J < nblk; J++)
     {
         const int j0 = J * NBK, nb = std::min(NBK, n_pad - j0), j1 = j0 + nb;
+        hipEvent_t next = nullptr;
+        if (j1 + NBK < n_pad)  // step J+1 has rows below its block
+            next = copy(J + 1);
         Ops::leaf(s, W + static_cast<long>(J) * NBK * NBK, NBK, B + static_cast<long>(j0) * m_pad, m_pad, V.get(), nb, m_pad);
         consume(s, static_cast<const T *>(V.get()), nb);
         if (j1 >= n_pad)
             break;
-        Ops::panel(s, L, lda, j1, j0, n_pad - j1, nb, P.get(), n_pad);
-        Ops::update(s, P.get(), n_pad, V.get(), m_pad, B + static_cast<long>(j1) * m_pad, m_pad, n_pad - j1, m_pad, nb);
+        if (side != s)
+            GP_HIP(hipStreamWaitEvent(s, ready, 0));
+        Ops::update(s, P[J % 2].get(), n_pad, V.get(), m_pad, B + static_cast<long>(j1) * m_pad, m_pad, n_pad - j1, m_pad, nb);
+        freed[J % 2] = record(s);
+        ready = next;
     }
+    if (side != s)
+        GP_HIP(hipStreamWaitEvent(s, record(side), 0));
     c.sync();
+    for (auto e : used)
+        c.recycle(e);
 }
 
 // colsq[j] = sum_i V[i][j]^2 over the solve above, for j < mt; squares are
