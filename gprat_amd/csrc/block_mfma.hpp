@@ -58,6 +58,42 @@ __device__ __forceinline__ void stage_block_t(double *dst, const double *__restr
     }
 }
 
+// stage_block / stage_block_t in two halves, so the loads of the next
+// block can be in flight while the current one is multiplied: load_block
+// issues this thread's eight 16-byte loads, store_block / store_block_t
+// write them to LDS (stride XS; transposed: dst[c][r] = src[r][c]).
+__device__ __forceinline__ void load_block(const double *__restrict__ src, long ld, d2 (&v)[8])
+{
+#pragma unroll
+    for (int s = 0; s < 8; s++)
+    {
+        const int e = threadIdx.x + THREADS * s;
+        v[s] = *reinterpret_cast<const d2 *>(src + static_cast<long>(e >> 5) * ld + (e & 31) * 2);
+    }
+}
+
+__device__ __forceinline__ void store_block(double *dst, const d2 (&v)[8])
+{
+#pragma unroll
+    for (int s = 0; s < 8; s++)
+    {
+        const int e = threadIdx.x + THREADS * s;
+        *reinterpret_cast<d2 *>(dst + (e >> 5) * XS + (e & 31) * 2) = v[s];
+    }
+}
+
+__device__ __forceinline__ void store_block_t(double *dst, const d2 (&v)[8])
+{
+#pragma unroll
+    for (int s = 0; s < 8; s++)
+    {
+        const int e = threadIdx.x + THREADS * s;
+        const int r = e >> 5, c = (e & 31) * 2;
+        dst[c * XS + r] = v[s][0];
+        dst[(c + 1) * XS + r] = v[s][1];
+    }
+}
+
 __device__ __forceinline__ void zero(v4d (&acc)[2][2])
 {
 #pragma unroll

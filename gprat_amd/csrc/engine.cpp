@@ -484,8 +484,9 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
         k::panel_tasks(P, a, plan.tasks.get(), plan.wg_off.get() + plan.launches[1].first, plan.launches[1].groups);
     };
     // The panel's rows below its diagonal block: X = A W^T with W the
-    // inverse of the diagonal block (k-major), one launch into scratch (each
-    // column tile stops K at its last column: W is lower triangular), then,
+    // inverse of the diagonal block (k-major), one launch of 64x64 block
+    // products into scratch (W is lower triangular, so column block c needs
+    // c + 1 products; the launch pairs c with its mirror), then,
     // once the panel stream has read the next block's rows (`solved`), one
     // pass that writes the k-major copy for the trailing update and X back
     // into the panel.
@@ -494,20 +495,7 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
         Timed t(c, T, "panel_rows");
         const int w = pe - p0, R = n_pad - pe;
         double *panel = A + static_cast<long>(pe) * lda + p0;
-        k::GemmArgs g{};
-        g.A = panel;
-        g.lda = lda;
-        g.B = Wblock(p0);  // B(k, j) = W(j, k)
-        g.ldb = Bo;
-        g.C = Xs.get();
-        g.ldc = w;
-        g.M = R;
-        g.N = w;
-        g.K = w;
-        g.set = true;
-        g.k_to_col = true;
-        g.role = k::GemmRole::PanelX;
-        k::gemm_f64(T, g);
+        k::rows_solve(T, A, lda, pe, p0, R, w, Wblock(p0), Bo, Xs.get(), w);
         wait(T, solved, P);
         if (mixed)
             k::panel_to_f32t(T, Xs.get(), w, 0, 0, R, w, p32.get(), n_pad, panel, lda);

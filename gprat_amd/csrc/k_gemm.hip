@@ -11,9 +11,8 @@
 // accumulators stay in place and two workgroups share a CU.
 //
 // The same kernel serves the Cholesky trailing update (both operands the
-// k-major panel copy, lower triangle only), the panel rows times the
-// diagonal block's inverse (both operands row-major), the blocked TRSM's
-// diagonal steps and updates, and W^T W (triangular K range).
+// k-major panel copy, lower triangle only), the blocked TRSM's diagonal
+// steps and updates, and W^T W (triangular K range).
 #include "kernels.hpp"
 
 namespace gpamd::k
@@ -46,9 +45,8 @@ struct Params
     int k_from_tile;
     long kstart_off;
     int set;
-    int k_to_col;
     int k_to_row;
-    int paired;  // k_gemm_f64 of the panel rows / TRSM diagonal steps: two tiles per workgroup
+    int paired;  // k_gemm_f64 of the TRSM diagonal steps: two row tiles per workgroup
     int *queue;  // k_gemm_f64_capped: tile counter (0 at launch) or null for fixed shares
     int tiles;   // k_gemm_f64_capped: tiles holding part of the update
 };
@@ -190,8 +188,6 @@ __device__ __forceinline__ void gemm_tile(const Params &p, long r0, long c0, dou
         kbeg = (kbeg / BK) * BK;
     }
     long kend = p.K;
-    if (p.k_to_col)
-        kend = min(kend, c0 + BN);
     if (p.k_to_row)
         kend = min(kend, r0 + BM);
     if (part != 0)
@@ -276,12 +272,11 @@ __device__ __forceinline__ void gemm_tile(const Params &p, long r0, long c0, dou
         store_quarter<false>(p, acc, cw, rb, cb, rows_a, rows_b, r0, c0);
 }
 
-// One workgroup per tile (grid x = column tiles, y = row tiles). The
-// panel rows (K ends at each column tile's last column) and the TRSM
-// diagonal steps (K ends at each row tile's last row) give tile t of n a K
-// range of (t + 1) tiles: when the grid exceeds the device's workgroup
-// slots (p.paired), a workgroup takes tiles t and n - 1 - t, so every
-// workgroup does the same work (grid halved along that axis); a grid that
+// One workgroup per tile (grid x = column tiles, y = row tiles). The TRSM
+// diagonal steps (K ends at each row tile's last row) give row tile t of n
+// a K range of (t + 1) tiles: when the grid exceeds the device's workgroup
+// slots (p.paired), a workgroup takes row tiles t and n - 1 - t, so every
+// workgroup does the same work (grid halved along the rows); a grid that
 // fits in one round keeps one tile per workgroup (shorter critical tile).
 template <bool AT, bool BT, int ROLE>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gemm_f64(Params p)
@@ -289,22 +284,18 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
     __shared__ double As[2][BK * LDS_STRIDE];
     __shared__ double Bs[2][BK * LDS_STRIDE];
     long r0 = static_cast<long>(blockIdx.y) * BM;
-    long c0 = static_cast<long>(blockIdx.x) * BN;
-    if constexpr (ROLE == static_cast<int>(GemmRole::PanelX) || ROLE == static_cast<int>(GemmRole::LeafT))
+    const long c0 = static_cast<long>(blockIdx.x) * BN;
+    if constexpr (ROLE == static_cast<int>(GemmRole::LeafT))
     {
         if (p.paired)
         {
-            constexpr bool by_col = ROLE == static_cast<int>(GemmRole::PanelX);
-            const int n = by_col ? (p.N + BN - 1) / BN : (p.M + BM - 1) / BM;
-            const int a = by_col ? blockIdx.x : blockIdx.y, b = n - 1 - a;
+            const int n = (p.M + BM - 1) / BM;
+            const int a = blockIdx.y, b = n - 1 - a;
             if (!(p.lower && r0 + (BM - 1) + p.diag_off < c0))
                 gemm_tile<AT, BT>(p, r0, c0, As, Bs);
             if (b <= a)
                 return;
-            if (by_col)
-                c0 = static_cast<long>(b) * BN;
-            else
-                r0 = static_cast<long>(b) * BM;
+            r0 = static_cast<long>(b) * BM;
             if (p.lower && r0 + (BM - 1) + p.diag_off < c0)
                 return;
             __syncthreads();  // the first tile's LDS reads are done
@@ -444,7 +435,7 @@ void gemm_f64(hipStream_t s, const GemmArgs &g)
     require(reinterpret_cast<uintptr_t>(g.A) % 16 == 0 && reinterpret_cast<uintptr_t>(g.B) % 16 == 0,
             "gemm_f64: operands must be 16-byte aligned");
     Params p{ g.A, g.lda, g.B, g.ldb, g.C, g.ldc, g.M, g.N, g.K, g.lower ? 1 : 0, g.diag_off, g.k_from_tile ? 1 : 0,
-              g.kstart_off, g.set ? 1 : 0, g.k_to_col ? 1 : 0, g.k_to_row ? 1 : 0, 0, nullptr, 0 };
+              g.kstart_off, g.set ? 1 : 0, g.k_to_row ? 1 : 0, 0, nullptr, 0 };
     dim3 grid(static_cast<unsigned>(ceil_div(g.N, BN)), static_cast<unsigned>(ceil_div(g.M, BM)));
     require(grid.y <= 65535, "gemm_f64: M too large for one launch");
     // tiles holding part of the update, counted as the capped kernel numbers them
@@ -465,28 +456,17 @@ void gemm_f64(hipStream_t s, const GemmArgs &g)
         GP_CHECK_LAUNCH();
         return;
     }
-    // paired tiles (see k_gemm_f64) when the grid needs more than one
-    // round of the device's workgroup slots (two per CU): half the grid
-    // along the paired axis
-    if ((g.role == GemmRole::PanelX || g.role == GemmRole::LeafT) && tiles > 2L * device_cus())
+    // paired row tiles (see k_gemm_f64) when the grid needs more than one
+    // round of the device's workgroup slots (two per CU)
+    if (g.role == GemmRole::LeafT && tiles > 2L * device_cus())
     {
-        if (g.role == GemmRole::PanelX)
-        {
-            require(g.k_to_col && !g.k_to_row, "gemm_f64: panel rows pair column tiles by their K range");
-            grid.x = (grid.x + 1) / 2;
-        }
-        else
-        {
-            require(g.k_to_row && !g.k_to_col, "gemm_f64: TRSM diagonal steps pair row tiles by their K range");
-            grid.y = (grid.y + 1) / 2;
-        }
+        require(g.k_to_row, "gemm_f64: TRSM diagonal steps pair row tiles by their K range");
+        grid.y = (grid.y + 1) / 2;
         p.paired = 1;
     }
-    // Instantiated combinations: (N,N) panel rows; (T,N) with both operands
-    // k-major for the updates, the TRSM steps and W^T W.
-    if (!g.a_trans && !g.b_trans && g.role == GemmRole::PanelX)
-        hipLaunchKernelGGL((k_gemm_f64<false, false, 8>), grid, dim3(THREADS), 0, s, p);
-    else if (g.a_trans && !g.b_trans && g.role == GemmRole::TrailingT)
+    // Instantiated combinations: (T,N) with both operands k-major for the
+    // updates, the TRSM steps and W^T W.
+    if (g.a_trans && !g.b_trans && g.role == GemmRole::TrailingT)
         hipLaunchKernelGGL((k_gemm_f64<true, false, 5>), grid, dim3(THREADS), 0, s, p);
     else if (g.a_trans && !g.b_trans && g.role == GemmRole::SolveT)
         hipLaunchKernelGGL((k_gemm_f64<true, false, 6>), grid, dim3(THREADS), 0, s, p);

@@ -261,6 +261,55 @@ __global__ __launch_bounds__(THREADS) void k_panel_tasks(PanelArgs a)
     }
 }
 
+// Panel rows below a factored diagonal block: X = A W^T for the R rows at
+// r0 of the panel's columns [p0, p0 + 64 nbw), W the block's k-major
+// inverse (lower: X(i, c) = sum_{t <= c} A(i, t) W(c, t)^T over 64x64
+// blocks), into X (row-major, ld ldx). Column block c costs c + 1 block
+// products, so workgroup (p, i) computes column blocks p and nbw - 1 - p of
+// row block i: every workgroup does nbw + 1 products (one extra for the
+// middle block of an odd nbw). The next pair of blocks is loaded into
+// registers while the current pair is multiplied.
+__global__ __launch_bounds__(THREADS) void k_rows_solve(const double *__restrict__ A,
+                                                         long lda,
+                                                         long r0,
+                                                         long p0,
+                                                         int nbw,
+                                                         const double *__restrict__ Wb,
+                                                         long ldw,
+                                                         double *__restrict__ X,
+                                                         long ldx)
+{
+    __shared__ double a[NB * blk::XS];
+    __shared__ double b[NB * blk::XS];
+    const int pair = blockIdx.x, i = blockIdx.y;
+    const double *row = A + (r0 + static_cast<long>(NB) * i) * lda + p0;
+    for (int half = 0; half < 2; half++)
+    {
+        const int c = half == 0 ? pair : nbw - 1 - pair;
+        if (half == 1 && c <= pair)
+            break;
+        blk::v4d acc[2][2];
+        blk::zero(acc);
+        blk::d2 va[8], vb[8];
+        blk::load_block(row, lda, va);
+        blk::load_block(Wb + static_cast<long>(NB) * c, ldw, vb);  // t = 0: rows k of W(c, 0)^T
+        for (int t = 0; t <= c; t++)
+        {
+            __syncthreads();  // the previous product's LDS reads are done
+            blk::store_block(a, va);    // [r][k] = A(r, k)
+            blk::store_block_t(b, vb);  // [cc][k] = W(c, t)(cc, k)
+            __syncthreads();
+            if (t < c)
+            {
+                blk::load_block(row + NB * (t + 1), lda, va);
+                blk::load_block(Wb + static_cast<long>(NB) * (t + 1) * ldw + NB * c, ldw, vb);
+            }
+            blk::mfma_nt_acc(a, b, acc);
+        }
+        blk::store_acc(X + static_cast<long>(NB) * i * ldx + NB * c, ldx, acc, true, false);
+    }
+}
+
 // x_J = W_JJ rhs_J (or W_JJ^T rhs_J), four column quarters per row combined
 // in a fixed order; every workgroup of a solve step computes it.
 template <bool TRANSPOSED>
@@ -749,6 +798,17 @@ void panel_tasks(hipStream_t s, const PanelTaskArgs &h, const int *tasks, const 
             "panel_tasks: bad head layout");
     PanelArgs a{ h, reinterpret_cast<const int4 *>(tasks), wg_off };
     hipLaunchKernelGGL(k_panel_tasks, dim3(groups), dim3(THREADS), 0, s, a);
+    GP_CHECK_LAUNCH();
+}
+
+void rows_solve(hipStream_t s, const double *A, long lda, long r0, long p0, int rows, int width, const double *Wb, long ldw, double *X, long ldx)
+{
+    require(rows > 0 && rows % NB == 0 && width > 0 && width % NB == 0 && lda % 2 == 0 && ldw % 2 == 0 && ldx % 2 == 0 && ldw >= width
+                && ldx >= width && p0 % 2 == 0,
+            "rows_solve: bad shape");
+    const int nbw = width / NB;
+    require(rows / NB <= 65535, "rows_solve: too many rows for one launch");
+    hipLaunchKernelGGL(k_rows_solve, dim3((nbw + 1) / 2, rows / NB), dim3(THREADS), 0, s, A, lda, r0, p0, nbw, Wb, ldw, X, ldx);
     GP_CHECK_LAUNCH();
 }
 

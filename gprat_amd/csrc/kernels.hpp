@@ -92,7 +92,6 @@ enum class GemmRole
     TrailingT = 5,  // Cholesky trailing update from a k-major panel copy
     SolveT = 6,     // TRSM update from a k-major panel copy
     LeafT = 7,      // TRSM diagonal step: k-major block inverse times the right-hand side
-    PanelX = 8,     // Cholesky panel rows times the diagonal block's inverse
 };
 
 struct GemmArgs
@@ -117,9 +116,6 @@ struct GemmArgs
     // are known zero and skipped (triangular operands, e.g. W^T W).
     bool k_from_tile;
     long kstart_off;
-    // B(k, j) is known zero for k > j (a lower-triangular matrix seen as
-    // B(k, j) = W(j, k)): each column tile stops K at its last column.
-    bool k_to_col;
     // A(i, k) is known zero for k > i (A lower triangular): each row tile
     // stops K at its last row.
     bool k_to_row;
@@ -210,6 +206,12 @@ struct PanelTaskArgs
     long ldx;
 };
 void panel_tasks(hipStream_t s, const PanelTaskArgs &a, const int *tasks, const int *wg_off, int groups);
+
+// Rows below a factored diagonal block: X = A W^T for A[r0 : r0 + rows,
+// p0 : p0 + width] with W the block's k-major inverse (W(r, c) at
+// Wb[c * ldw + r], lower triangular), into X (row-major, ld ldx). rows and
+// width are multiples of 64.
+void rows_solve(hipStream_t s, const double *A, long lda, long r0, long p0, int rows, int width, const double *Wb, long ldw, double *X, long ldx);
 
 // Block steps of the vector solves with the saved diagonal-block inverses
 // Winv (block J at Winv + J*64*64). Forward: out[J] = W_JJ rhs[J],
