@@ -673,6 +673,11 @@ struct VarianceF64
     using T = double;
     static void leaf(hipStream_t s, const T *W, int ldw, const T *B, long ldb, T *V, int nb, int cols)
     {
+        if (nb % 64 == 0 && cols % 64 == 0)
+        {
+            k::leaf_solve(s, W, ldw, nb, B, ldb, cols, V, ldb);
+            return;
+        }
         k::GemmArgs g{};
         g.A = W;
         g.lda = ldw;

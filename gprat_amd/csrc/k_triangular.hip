@@ -310,6 +310,50 @@ __global__ __launch_bounds__(THREADS) void k_rows_solve(const double *__restrict
     }
 }
 
+// Diagonal step of the blocked TRSM: V = W B with W lower triangular
+// (k-major: W(r, c) at Wk[c * ldw + r], nbr 64-blocks), B and V row-major.
+// V(i, j) = sum_{t <= i} W(i, t) B(t, j) over 64x64 blocks: row block i
+// costs i + 1 block products, so workgroup (j, p) computes row blocks p and
+// nbr - 1 - p of column block j, and loads the next pair of blocks into
+// registers while the current pair is multiplied.
+__global__ __launch_bounds__(THREADS) void k_leaf_solve(const double *__restrict__ Wk,
+                                                         long ldw,
+                                                         int nbr,
+                                                         const double *__restrict__ B,
+                                                         long ldb,
+                                                         double *__restrict__ V,
+                                                         long ldv)
+{
+    __shared__ double a[NB * blk::XS];
+    __shared__ double b[NB * blk::XS];
+    const int j = blockIdx.x, pair = blockIdx.y;
+    for (int half = 0; half < 2; half++)
+    {
+        const int i = half == 0 ? pair : nbr - 1 - pair;
+        if (half == 1 && i <= pair)
+            break;
+        blk::v4d acc[2][2];
+        blk::zero(acc);
+        blk::d2 va[8], vb[8];
+        blk::load_block(Wk + static_cast<long>(NB) * i, ldw, va);  // t = 0: rows k of W(i, 0)
+        blk::load_block(B + static_cast<long>(NB) * j, ldb, vb);   // rows k of B(0, j)
+        for (int t = 0; t <= i; t++)
+        {
+            __syncthreads();              // the previous product's LDS reads are done
+            blk::store_block_t(a, va);    // [r][k] = W(i, t)(r, k)
+            blk::store_block_t(b, vb);    // [cc][k] = B(t, j)(k, cc)
+            __syncthreads();
+            if (t < i)
+            {
+                blk::load_block(Wk + static_cast<long>(NB) * (t + 1) * ldw + NB * i, ldw, va);
+                blk::load_block(B + static_cast<long>(NB) * (t + 1) * ldb + NB * j, ldb, vb);
+            }
+            blk::mfma_nt_acc(a, b, acc);
+        }
+        blk::store_acc(V + static_cast<long>(NB) * i * ldv + NB * j, ldv, acc, true, false);
+    }
+}
+
 // x_J = W_JJ rhs_J (or W_JJ^T rhs_J), four column quarters per row combined
 // in a fixed order; every workgroup of a solve step computes it.
 template <bool TRANSPOSED>
@@ -809,6 +853,17 @@ void rows_solve(hipStream_t s, const double *A, long lda, long r0, long p0, int 
     const int nbw = width / NB;
     require(rows / NB <= 65535, "rows_solve: too many rows for one launch");
     hipLaunchKernelGGL(k_rows_solve, dim3((nbw + 1) / 2, rows / NB), dim3(THREADS), 0, s, A, lda, r0, p0, nbw, Wb, ldw, X, ldx);
+    GP_CHECK_LAUNCH();
+}
+
+void leaf_solve(hipStream_t s, const double *Wk, long ldw, int rows, const double *B, long ldb, int cols, double *V, long ldv)
+{
+    require(rows > 0 && rows % NB == 0 && cols > 0 && cols % NB == 0 && ldw % 2 == 0 && ldb % 2 == 0 && ldv % 2 == 0 && ldw >= rows
+                && ldb >= cols && ldv >= cols,
+            "leaf_solve: bad shape");
+    const int nbr = rows / NB;
+    require((nbr + 1) / 2 <= 65535, "leaf_solve: too many rows for one launch");
+    hipLaunchKernelGGL(k_leaf_solve, dim3(cols / NB, (nbr + 1) / 2), dim3(THREADS), 0, s, Wk, ldw, nbr, B, ldb, V, ldv);
     GP_CHECK_LAUNCH();
 }
 

@@ -213,6 +213,11 @@ void panel_tasks(hipStream_t s, const PanelTaskArgs &a, const int *tasks, const 
 // width are multiples of 64.
 void rows_solve(hipStream_t s, const double *A, long lda, long r0, long p0, int rows, int width, const double *Wb, long ldw, double *X, long ldx);
 
+// Diagonal step of the blocked TRSM: V = W B for W lower triangular,
+// k-major (W(r, c) at Wk[c * ldw + r]), rows x rows; B, V row-major, rows
+// x cols. rows and cols are multiples of 64.
+void leaf_solve(hipStream_t s, const double *Wk, long ldw, int rows, const double *B, long ldb, int cols, double *V, long ldv);
+
 // Block steps of the vector solves with the saved diagonal-block inverses
 // Winv (block J at Winv + J*64*64). Forward: out[J] = W_JJ rhs[J],
 // rhs[later] -= L[later, J] out[J]; run J = 0 .. n_pad/64-1 in order.
