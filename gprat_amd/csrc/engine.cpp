@@ -380,13 +380,14 @@ const Context::PanelPlanDev &Context::cached_plan(long key, const std::function<
 //
 // A panel of width Bo is factored in two parts. Its diagonal block (Bo x Bo)
 // is factored in 64-wide steps and inverted on the `panel` stream; the rows
-// below it are then one set of MFMA products with that inverse on the
+// below it are then one set of block products with that inverse on the
 // `trail` stream, followed by a k-major copy of the panel that feeds the
 // trailing update as the A^T operand. The latency-bound diagonal-block work
 // is a few small launches per panel, so it slots in beside the update.
-// Step k on `trail`: refresh panel k+1's columns, release its diagonal block
-// to `panel`, update the rest of the trailing matrix while that block is
-// being factored, then solve panel k+1's rows.
+// Step k: the `panel` stream applies panel k to diagonal block k+1 (once
+// step k-1's update has reached it) and factors that block, while the
+// `trail` stream solves panel k's rows and applies panel k to everything
+// below block k+1.
 void cholesky_fp64(Context &c, double *A, long lda, int n_pad, double *Winv) { cholesky(c, A, lda, n_pad, Winv, false); }
 
 int panel_width(int n_pad) { return outer_block(n_pad); }
