@@ -45,7 +45,7 @@ def outer_block(n_pad: int) -> int:
     env = os.environ.get("GPRAT_CHOLESKY_BLOCK")
     if env and int(env) >= 128 and int(env) % 128 == 0:
         return int(env)
-    return 256 if n_pad <= 4096 else (512 if n_pad < 32768 else 768)
+    return 256 if n_pad <= 4096 else (512 if n_pad < 12288 else 768)
 
 
 def trailing_update_flops(n_pad: int):

@@ -287,8 +287,10 @@ static int outer_block(int n_pad)
     // factorisation measured at n = 16384 / 32768: 512-wide panels
     // 35.6 / 210.6 ms, 768-wide 36.1 / 204.9 ms, 1024-wide 39.2 / 207.2 ms;
     // wider panels give the trailing update a longer K but lengthen the
-    // diagonal-block chain and the panel stream's head work
-    return n_pad <= 4096 ? 256 : (n_pad < 32768 ? 512 : 768);
+    // diagonal-block chain and the panel stream's head work. The variance
+    // solve steps through the same blocks, so a whole request at n = 16384
+    // (m = 4096) favours 768 (56.5 against 57.1 ms with 512)
+    return n_pad <= 4096 ? 256 : (n_pad < 12288 ? 512 : 768);
 }
 
 // Workgroup slots (out of two per CU) that look-ahead launches on the
