@@ -46,7 +46,6 @@ struct Task
 
 struct Launch
 {
-    bool chain = false;
     std::vector<Task> tasks;
     std::vector<double> loads;  // estimated per-workgroup work (placement only)
     int cap = 1;
@@ -107,7 +106,6 @@ PanelPlan build_panel_plan(int nb, int chain_wgs, int tail_wgs)
     auto new_launch = [&](bool chain)
     {
         Launch l;
-        l.chain = chain;
         l.cap = chain ? chain_wgs : tail_wgs;
         L.push_back(std::move(l));
         return static_cast<int>(L.size()) - 1;
