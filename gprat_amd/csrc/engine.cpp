@@ -308,12 +308,14 @@ static int trailing_reserve()
 
 // Workgroups of the panel stream's head launches (the next diagonal
 // block's rows of the panel and its update); GPRAT_HEAD_WORKGROUPS
-// overrides.
+// overrides. Request at n = 16384 (768-wide panels: 144 + 78 head tasks),
+// alternating runs: 32 → 57.3-57.5 ms, 64 → 55.9-56.4, 96 → 55.3-55.7,
+// 128 → 55.2-55.4, 160 → 55.7-55.8.
 static int head_workgroups()
 {
     if (const char *e = std::getenv("GPRAT_HEAD_WORKGROUPS"))
         return std::max(1, std::atoi(e));
-    return 64;
+    return 128;
 }
 
 // C[i][j] -= sum_k P[k][a_off + i] P[k][b_off + j] for i < M, j < N,
