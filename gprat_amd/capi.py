@@ -72,6 +72,8 @@ SIGNATURES = {
     "gprat_model_gradient": [_VP, _I, _D, _D],
     "gprat_model_bytes": [_VP, C.POINTER(_SZ)],
     "gprat_model_refinement": [_VP, _I, _D],
+    "gprat_model_set_variance_refinement": [_VP, C.c_int],
+    "gprat_model_variance_correction": [_VP, _D],
     "gprat_model_release": [_VP],
     "gprat_batch_fit": [_VP, C.c_int, C.c_void_p, C.c_int, C.POINTER(_VP)],
     "gprat_batch_losses": [_VP, _D],
@@ -345,6 +347,17 @@ class Model:
         last = C.c_double(0.0)
         _check(lib().gprat_model_refinement(self.handle, C.byref(it), C.byref(last)))
         return it.value, last.value
+
+    def set_variance_refinement(self, enabled: bool):
+        """Mixed-precision models: FP64 refinement of the latent variance
+        (on by default); off returns the FP32 solve's unrefined variance."""
+        _check(lib().gprat_model_set_variance_refinement(self.handle, 1 if enabled else 0))
+
+    def variance_correction(self):
+        """Largest FP64 correction |q1 - q0| / v of the last refined variance."""
+        v = C.c_double(0.0)
+        _check(lib().gprat_model_variance_correction(self.handle, C.byref(v)))
+        return v.value
 
     def nbytes(self):
         b = _SZ(0)

@@ -532,6 +532,27 @@ int gprat_model_refinement(const gprat_model *model, int *iterations, double *la
         });
 }
 
+int gprat_model_set_variance_refinement(gprat_model *model, int enabled)
+{
+    return guarded(
+        [&]
+        {
+            auto &md = model_ref(model);
+            std::lock_guard<std::recursive_mutex> g(md.ctx->mu);
+            md.refine_variance = enabled != 0;
+        });
+}
+
+int gprat_model_variance_correction(const gprat_model *model, double *correction)
+{
+    return guarded(
+        [&]
+        {
+            gpamd::require(correction != nullptr, "null argument");
+            *correction = model_ref(model).variance_correction;
+        });
+}
+
 int gprat_model_bytes(const gprat_model *model, size_t *bytes)
 {
     return guarded(

@@ -96,18 +96,26 @@ void DevicePool::release_cached() { trim(0); }
 
 void DevicePool::trim(std::size_t keep)
 {
-    std::lock_guard<std::mutex> g(mu_);
-    if (cached_ <= keep)
-        return;
-    hipDeviceSynchronize();
-    // free the largest cached blocks first
-    while (cached_ > keep && !free_.empty())
+    // the blocks leave the cache under the lock (largest first); the wait
+    // for kernels that may still use them and the frees happen after it is
+    // released, so put() and get() from other threads never wait on the device
+    std::vector<void *> drop;
     {
-        auto it = std::prev(free_.end());
-        hipFree(it->second);
-        cached_ -= it->first;
-        free_.erase(it);
+        std::lock_guard<std::mutex> g(mu_);
+        while (cached_ > keep && !free_.empty())
+        {
+            auto it = std::prev(free_.end());
+            drop.push_back(it->second);
+            cached_ -= it->first;
+            free_.erase(it);
+        }
     }
+    if (drop.empty())
+        return;
+    hipSetDevice(device_);
+    hipDeviceSynchronize();
+    for (void *p : drop)
+        hipFree(p);
 }
 
 template <class T>
@@ -846,6 +854,156 @@ static void variance_colsq_f32(Context &c, const FactorView &m, const double *z,
     variance_solve<VarianceF32>(c, m.L, m.lda, Wt32.get(), NBK, n_pad, B.get(), m_pad, mt, colsq);
 }
 
+// X = L^-T V in FP32 (V n_pad x m_pad in B, overwritten by X), blocked in
+// NBK-row steps from the last block up: X_J = W_J^T V_J (WT: the transposed
+// block inverses, k-major, so the product's A operand loads whole rows),
+// then V[above] -= L[J, above]^T X_J from an FP32 copy of L's row block J
+// (rows j0 .. j0 + nb, columns 0 .. j0: the k-major operand layout as it
+// stands). The row-block copies are made on the panel stream one step
+// ahead into alternating buffers.
+static void backward_solve_f32(Context &c, const double *L, long lda, const float *WT, int NBK, int n_pad, float *B, int m_pad)
+{
+    hipStream_t s = c.main;
+    hipStream_t side = c.panel;
+    const int nblk = static_cast<int>(ceil_div(n_pad, NBK));
+    DevBuf<float> T(c, static_cast<std::size_t>(NBK) * m_pad);
+    DevBuf<float> R[2] = { DevBuf<float>(c, static_cast<std::size_t>(NBK) * n_pad), DevBuf<float>(c, static_cast<std::size_t>(NBK) * n_pad) };
+    std::vector<hipEvent_t> used;
+    auto record = [&](hipStream_t st)
+    {
+        hipEvent_t e = c.event();
+        used.push_back(e);
+        GP_HIP(hipEventRecord(e, st));
+        return e;
+    };
+    hipEvent_t freed[2] = { record(s), nullptr };
+    freed[1] = freed[0];
+    auto copy = [&](int J)
+    {
+        const int j0 = J * NBK, nb = std::min(NBK, n_pad - j0);
+        if (side != s)
+            GP_HIP(hipStreamWaitEvent(side, freed[J % 2], 0));
+        k::rows_to_f32(side, L, lda, j0, nb, j0, R[J % 2].get(), n_pad);
+        return record(side);
+    };
+    hipEvent_t ready = nblk > 1 ? copy(nblk - 1) : nullptr;
+    for (int J = nblk - 1; J >= 0; J--)
+    {
+        const int j0 = J * NBK, nb = std::min(NBK, n_pad - j0);
+        hipEvent_t next = J - 1 >= 1 ? copy(J - 1) : nullptr;
+        float *BJ = B + static_cast<long>(j0) * m_pad;
+        k::gemm_f32_tn(s, WT + static_cast<long>(J) * NBK * NBK, NBK, BJ, m_pad, T.get(), m_pad, nb, m_pad, nb, true);
+        GP_HIP(hipMemcpyAsync(BJ, T.get(), sizeof(float) * nb * m_pad, hipMemcpyDeviceToDevice, s));
+        if (j0 == 0)
+            break;
+        if (side != s)
+            GP_HIP(hipStreamWaitEvent(s, ready, 0));
+        k::gemm_f32_tn(s, R[J % 2].get(), n_pad, T.get(), m_pad, B, m_pad, j0, m_pad, nb, false);
+        freed[J % 2] = record(s);
+        ready = next;
+    }
+    if (side != s)
+        GP_HIP(hipStreamWaitEvent(s, record(side), 0));
+    c.sync();
+    for (auto e : used)
+        c.recycle(e);
+}
+
+// Latent variance of a mixed-precision model, refined in FP64 (k_refine.hip):
+// X~ = K~^-1 K*^T by the FP32 blocked forward solve (its column squares are
+// the unrefined q0 = |L~^-1 k|^2) and the FP32 backward solve, on matrix
+// cores with FP32 copies of the block inverses; then, in FP64, the residual
+// product Z = K_sl X~ and per column q1 = 2 k^T x~ - (2 x~^T z + d |x~|^2),
+// var = v - q1. The result's error is e^T K e for the solves' error e, so
+// it is second order in the FP32 and factor errors. Returns max |q1 - q0| / v.
+static double variance_refined(Context &c, const FactorView &m, const double *z, int mt, int m_pad, double *var)
+{
+    hipStream_t s = c.main;
+    const int NBK = m.Wpanel ? m.wblock : 256;
+    const int n_pad = m.n_pad;
+    const int nblk = static_cast<int>(ceil_div(n_pad, NBK));
+    const std::size_t wcount = static_cast<std::size_t>(nblk) * NBK * NBK;
+    const std::size_t elems = static_cast<std::size_t>(n_pad) * m_pad;
+    DevBuf<float> W32(c, wcount), WT32(c, wcount);
+    DevBuf<double> Wt, Ktmp;
+    const double *W64 = m.Wpanel;
+    if (!W64)
+    {
+        Wt = DevBuf<double>(c, wcount);
+        k::invert_blocks256(s, m.L, m.lda, m.Winv, 0, n_pad, Wt.get(), NBK, static_cast<long>(NBK) * NBK);
+        W64 = Wt.get();
+    }
+    k::to_f32(s, W64, W32.get(), static_cast<long>(wcount));
+    k::transpose_blocks_f32(s, W64, WT32.get(), NBK, nblk);
+    const double *Kup = m.Kup;
+    long ldk = m.ldk;
+    if (!Kup)
+    {
+        Ktmp = DevBuf<double>(c, static_cast<std::size_t>(n_pad) * n_pad);
+        k::assemble_upper(s, m.x, m.r, m.sek, Ktmp.get(), n_pad, m.n, n_pad);
+        Kup = Ktmp.get();
+        ldk = n_pad;
+    }
+    DevBuf<float> B32(c, elems);
+    DevBuf<double> q0(c, m_pad);
+    {
+        Timed t(c, s, "assemble");
+        k::assemble_cross_t_f32(s, m.x, z, m.r, m.sek, B32.get(), m_pad, m.n, n_pad, mt, m_pad);
+    }
+    {
+        Timed t(c, s, "trsm");
+        k::fill(s, q0.get(), m_pad, 0.0);
+        int J = 0;
+        float *b32 = B32.get();
+        blocked_solve<VarianceF32>(c, m.L, m.lda, W32.get(), NBK, n_pad, b32, m_pad,
+                                   [&](hipStream_t st, const float *V, int rows)
+                                   {
+                                       k::colsq_accumulate_f32(st, V, m_pad, rows, mt, q0.get());
+                                       // B_J has been consumed by the diagonal step: V_J takes its place
+                                       GP_HIP(hipMemcpyAsync(b32 + static_cast<long>(J) * NBK * m_pad, V, sizeof(float) * rows * m_pad,
+                                                             hipMemcpyDeviceToDevice, st));
+                                       J++;
+                                   });
+        backward_solve_f32(c, m.L, m.lda, WT32.get(), NBK, n_pad, b32, m_pad);
+    }
+    DevBuf<double> X64(c, elems), B64(c, elems), Z(c, elems);
+    k::to_f64(s, B32.get(), X64.get(), static_cast<long>(elems));
+    {
+        Timed t(c, s, "assemble");
+        k::assemble_cross_t(s, m.x, z, m.r, m.sek, B64.get(), m_pad, m.n, n_pad, mt, m_pad);
+    }
+    const int splits = k::quad_form_splits(n_pad, m_pad);
+    DevBuf<double> part(c, static_cast<std::size_t>(splits) * 3 * m_pad), corr(c, m_pad);
+    {
+        Timed t(c, s, "refine");
+        k::GemmArgs g{};
+        g.A = Kup;  // A(i, k) = Kup[k][i] = K(i, k) for k < i
+        g.lda = ldk;
+        g.a_trans = true;
+        g.B = X64.get();
+        g.ldb = m_pad;
+        g.C = Z.get();
+        g.ldc = m_pad;
+        g.M = n_pad;
+        g.N = m_pad;
+        g.K = n_pad;
+        g.set = true;
+        g.k_before_tile = true;
+        g.role = k::GemmRole::Residual;
+        k::gemm_f64(s, g);
+        k::diag_strict_product(s, Kup, ldk, X64.get(), m_pad, Z.get(), m_pad, n_pad, m_pad);
+        k::quad_form(s, B64.get(), X64.get(), Z.get(), m_pad, n_pad, m_pad, m.sek.vertical + m.sek.noise, m.sek.vertical, q0.get(), mt,
+                     part.get(), splits, var, corr.get());
+    }
+    std::vector<double> hc(mt);
+    GP_HIP(hipMemcpyAsync(hc.data(), corr.get(), sizeof(double) * mt, hipMemcpyDefault, s));
+    c.sync();
+    double worst = 0.0;
+    for (double v : hc)
+        worst = std::max(worst, std::fabs(v));
+    return m.sek.vertical != 0.0 ? worst / std::fabs(m.sek.vertical) : worst;
+}
+
 // ---------------------------------------------------------------------------
 // Model operations
 
@@ -882,6 +1040,13 @@ void factor(Context &c, Model &m, const double *x, std::size_t x_len, int n, int
         k::assemble_lower(c.main, m.x.get(), r, m.sek, m.L.get(), m.lda, n, m.n_pad);
     }
     cholesky(c, m.L.get(), m.lda, m.n_pad, m.Winv.get(), precision == GPRAT_PRECISION_MIXED, m.Wpanel.get());
+    if (precision == GPRAT_PRECISION_MIXED)
+    {
+        // the covariance's strictly lower part, for the FP64 residual of the
+        // variance refinement, goes to the factor's free upper triangle
+        Timed t(c, c.main, "assemble");
+        k::assemble_upper(c.main, m.x.get(), r, m.sek, m.L.get(), m.lda, n, m.n_pad);
+    }
 }
 
 // FP64 iterative refinement of alpha against a factor of reduced accuracy:
@@ -909,7 +1074,6 @@ void refine_alpha(Model &m, double *work)
             k::residual_finish(s, m.y.get(), kx.get(), m.alpha.get(), m.sek.noise, n, r.get());
         }
         solve_spd(c, m.L.get(), m.lda, m.Winv.get(), m.n_pad, r.get(), work, delta.get(), m.Wpanel.get(), m.wblock);
-        k::add_vec(s, m.alpha.get(), delta.get(), n);
         GP_HIP(hipMemcpyAsync(hd.data(), delta.get(), sizeof(double) * n, hipMemcpyDefault, s));
         GP_HIP(hipMemcpyAsync(ha.data(), m.alpha.get(), sizeof(double) * n, hipMemcpyDefault, s));
         c.sync();
@@ -919,11 +1083,16 @@ void refine_alpha(Model &m, double *work)
             dmax = std::max(dmax, std::fabs(hd[i]));
             amax = std::max(amax, std::fabs(ha[i]));
         }
+        // a correction that did not shrink is not applied: alpha keeps the
+        // best iterate so far
+        if (it > 0 && dmax >= prev)
+            break;
+        k::add_vec(s, m.alpha.get(), delta.get(), n);
         m.refine_iterations = it + 1;
         m.refine_last = amax > 0.0 ? dmax / amax : 0.0;
         // the next correction is about (dmax / prev) * dmax: when that is
         // already below the target, this step has reached it
-        const bool converged = dmax <= 1e-12 * amax || (it > 0 && dmax < prev && dmax * (dmax / prev) <= 1e-12 * amax);
+        const bool converged = dmax <= 1e-12 * amax || (it > 0 && dmax * (dmax / prev) <= 1e-12 * amax);
         if (!converged && dmax < 0.5 * prev)
         {
             prev = dmax;
@@ -986,7 +1155,16 @@ void predict_view(const FactorView &m, const double *z, std::size_t z_len, int m
         k::predict_mean(c.main, m.x, dz.get(), m.alpha, m.r, m.sek, m.n, mt, partial.get(), splits, dmean.get());
     }
     DevBuf<double> dvar;
-    if (var && m.mixed)
+    if (var && m.mixed && m.refine_variance)
+    {
+        const int m_pad = static_cast<int>(round_up(mt, 64));
+        dvar = DevBuf<double>(c, mt);
+        const double corr = variance_refined(c, m, dz.get(), mt, m_pad, dvar.get());
+        if (m.variance_correction)
+            *m.variance_correction = corr;
+        GP_HIP(hipMemcpyAsync(var, dvar.get(), sizeof(double) * mt, hipMemcpyDefault, c.main));
+    }
+    else if (var && m.mixed)
     {
         const int m_pad = static_cast<int>(round_up(mt, 64));
         DevBuf<double> colsq(c, m_pad);
@@ -1092,11 +1270,24 @@ void predict_full_cov(Model &m, const double *z, std::size_t z_len, int mt, doub
                           z, z_len, mt, mean, cov);
 }
 
+FactorView view(const Model &m)
+{
+    FactorView v{ m.ctx, m.x.get(), m.n, m.n_pad, m.r, m.lda, m.sek, m.L.get(), m.Winv.get(), m.alpha.get(),
+                  m.precision == GPRAT_PRECISION_MIXED, m.Wpanel.get(), m.wblock };
+    v.refine_variance = m.refine_variance;
+    if (m.precision == GPRAT_PRECISION_MIXED)
+    {
+        v.Kup = m.L.get();  // mixed models keep the covariance in the factor's strict upper triangle
+        v.ldk = m.lda;
+    }
+    return v;
+}
+
 void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean, double *var)
 {
-    predict_view(FactorView{ m.ctx, m.x.get(), m.n, m.n_pad, m.r, m.lda, m.sek, m.L.get(), m.Winv.get(), m.alpha.get(),
-                             m.precision == GPRAT_PRECISION_MIXED, m.Wpanel.get(), m.wblock },
-                 z, z_len, mt, mean, var);
+    FactorView v = view(m);
+    v.variance_correction = &m.variance_correction;
+    predict_view(v, z, z_len, mt, mean, var);
 }
 
 // ---------------------------------------------------------------------------

@@ -27,10 +27,14 @@ class DevicePool
     void release_cached();
     // Free cached blocks until at most `keep` bytes stay cached.
     void trim(std::size_t keep);
-    std::size_t cached_bytes() const { return cached_; }
+    std::size_t cached_bytes() const
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        return cached_;
+    }
 
   private:
-    std::mutex mu_;
+    mutable std::mutex mu_;
     int device_;
     std::multimap<std::size_t, void *> free_;
     std::map<void *, std::size_t> live_;
@@ -136,6 +140,12 @@ struct Model
     int precision = GPRAT_PRECISION_FP64;
     int refine_iterations = 0;   // mixed precision: refinement steps taken for alpha
     double refine_last = 0.0;    // last max|delta| / max|alpha|
+    // mixed precision: the latent variance is refined in FP64 (the strict
+    // upper triangle of L then holds the covariance, written after the
+    // factorisation); variance_correction is the last prediction's
+    // max |q1 - q0| / v, the FP64 correction of the FP32 solves' result
+    bool refine_variance = true;
+    double variance_correction = 0.0;
     DevBuf<double> x;      // training series (n + r - 1)
     DevBuf<double> y;      // targets, zero padded to n_pad
     DevBuf<double> L;      // n_pad x lda, lower triangle holds the factor
@@ -160,6 +170,14 @@ struct FactorView
     bool mixed = false;               // variance solve in FP32 products (mixed-precision models)
     const double *Wpanel = nullptr;   // k-major wblock-wide diagonal-block inverses, if kept
     int wblock = 0;
+    // mixed: refine the variance in FP64 (Kup: the covariance in the strict
+    // upper triangle of an n_pad x n_pad matrix with leading dimension ldk,
+    // or null to assemble one for the call); variance_correction receives
+    // the prediction's max |q1 - q0| / v when given
+    bool refine_variance = true;
+    const double *Kup = nullptr;
+    long ldk = 0;
+    double *variance_correction = nullptr;
 };
 
 // Many GPs fitted by one persistent launch; factors stay in one arena.
@@ -204,6 +222,8 @@ void fit(Context &c,
          int precision);
 void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean, double *var);
 void predict_view(const FactorView &m, const double *z, std::size_t z_len, int mt, double *mean, double *var);
+// The view of a resident model (its context; no variance_correction output).
+FactorView view(const Model &m);
 // Mean and the whole mt x mt latent posterior covariance (row-major).
 void predict_full_cov(Model &m, const double *z, std::size_t z_len, int mt, double *mean, double *cov);
 void predict_full_cov_view(const FactorView &m, const double *z, std::size_t z_len, int mt, double *mean, double *cov);

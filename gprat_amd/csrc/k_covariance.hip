@@ -51,7 +51,9 @@ __global__ __launch_bounds__(THREADS) void k_assemble_lower(const double *__rest
 // The same for window lengths R <= 16: a lane keeps its column's window in
 // registers, and the row's window (one row per wavefront) is read through
 // the scalar cache. Same operations in the same order as window_sqdist.
-template <int R>
+// UPPER: the 64x64 blocks on or above the diagonal instead, and only the
+// elements strictly above it (K(i, j), i < j; zero in the padding).
+template <int R, bool UPPER = false>
 __global__ __launch_bounds__(THREADS) void k_assemble_lower_w(const double *__restrict__ x,
                                                                double scale,
                                                                double v,
@@ -62,7 +64,7 @@ __global__ __launch_bounds__(THREADS) void k_assemble_lower_w(const double *__re
 {
 #pragma clang fp contract(off)
     const int ti = blockIdx.y, tj = blockIdx.x;
-    if (tj > ti)
+    if (UPPER ? tj < ti : tj > ti)
         return;
     const int c = threadIdx.x % TB;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / TB);
@@ -91,7 +93,8 @@ __global__ __launch_bounds__(THREADS) void k_assemble_lower_w(const double *__re
         }
         else
             val = (i == j) ? 1.0 : 0.0;
-        A[static_cast<long>(i) * lda + j] = val;
+        if (!UPPER || i < j)
+            A[static_cast<long>(i) * lda + j] = val;
     }
 }
 
@@ -437,16 +440,40 @@ void gradient_sums(hipStream_t s,
     GP_CHECK_LAUNCH();
 }
 
-template <int R>
+template <int R, bool UPPER = false>
 void launch_lower_w(hipStream_t s, dim3 grid, const double *x, double scale, double v, double noise, double *A, long lda, int n)
 {
-    hipLaunchKernelGGL(k_assemble_lower_w<R>, grid, dim3(THREADS), 0, s, x, scale, v, noise, A, lda, n);
+    hipLaunchKernelGGL((k_assemble_lower_w<R, UPPER>), grid, dim3(THREADS), 0, s, x, scale, v, noise, A, lda, n);
 }
 using LowerLaunch = void (*)(hipStream_t, dim3, const double *, double, double, double, double *, long, int);
 constexpr LowerLaunch lower_w_table[16] = { launch_lower_w<1>,  launch_lower_w<2>,  launch_lower_w<3>,  launch_lower_w<4>,
                                             launch_lower_w<5>,  launch_lower_w<6>,  launch_lower_w<7>,  launch_lower_w<8>,
                                             launch_lower_w<9>,  launch_lower_w<10>, launch_lower_w<11>, launch_lower_w<12>,
                                             launch_lower_w<13>, launch_lower_w<14>, launch_lower_w<15>, launch_lower_w<16> };
+constexpr LowerLaunch upper_w_table[16] = { launch_lower_w<1, true>,  launch_lower_w<2, true>,  launch_lower_w<3, true>,
+                                            launch_lower_w<4, true>,  launch_lower_w<5, true>,  launch_lower_w<6, true>,
+                                            launch_lower_w<7, true>,  launch_lower_w<8, true>,  launch_lower_w<9, true>,
+                                            launch_lower_w<10, true>, launch_lower_w<11, true>, launch_lower_w<12, true>,
+                                            launch_lower_w<13, true>, launch_lower_w<14, true>, launch_lower_w<15, true>,
+                                            launch_lower_w<16, true> };
+
+// Generic window length: the strict upper triangle, as the windowed kernel above.
+__global__ __launch_bounds__(THREADS) void k_assemble_upper(const double *__restrict__ x, int r, double scale, double v,
+                                                             double *__restrict__ A, long lda, int n)
+{
+    const int ti = blockIdx.y, tj = blockIdx.x;
+    if (tj < ti)
+        return;
+    const int c = threadIdx.x % TB;
+    const long j = static_cast<long>(tj) * TB + c;
+    for (int rr = threadIdx.x / TB; rr < TB; rr += THREADS / TB)
+    {
+        const long i = static_cast<long>(ti) * TB + rr;
+        if (i >= j)
+            continue;
+        A[i * lda + j] = (i < n && j < n) ? se(scale, v, window_sqdist(x, i, x, j, r)) : 0.0;
+    }
+}
 
 template <class T, int R>
 void launch_cross_w(hipStream_t s, dim3 grid, const double *x, const double *z, double scale, double v, T *B, long ldb, int n, int m)
@@ -469,6 +496,17 @@ void assemble_lower(hipStream_t s, const double *x, int r, SEK sek, double *A, l
         lower_w_table[r - 1](s, grid, x, sek.scale(), sek.vertical, sek.noise, A, lda, n);
     else
         hipLaunchKernelGGL(k_assemble_lower, grid, dim3(THREADS), 0, s, x, r, sek.scale(), sek.vertical, sek.noise, A, lda, n);
+    GP_CHECK_LAUNCH();
+}
+
+void assemble_upper(hipStream_t s, const double *x, int r, SEK sek, double *A, long lda, int n, int n_pad)
+{
+    require(n_pad % TB == 0 && n <= n_pad && lda >= n_pad && r >= 1, "assemble_upper: bad shape");
+    dim3 grid(n_pad / TB, n_pad / TB);
+    if (r <= 16 && n >= 1 && n_pad < (1 << 30) - 16)
+        upper_w_table[r - 1](s, grid, x, sek.scale(), sek.vertical, sek.noise, A, lda, n);
+    else
+        hipLaunchKernelGGL(k_assemble_upper, grid, dim3(THREADS), 0, s, x, r, sek.scale(), sek.vertical, A, lda, n);
     GP_CHECK_LAUNCH();
 }
 

@@ -46,6 +46,7 @@ struct Params
     long kstart_off;
     int set;
     int k_to_row;
+    int k_before_tile;  // K ends at the row tile's first row; row tiles launched last first
     int paired;  // k_gemm_f64 of the TRSM diagonal steps: two row tiles per workgroup
     int *queue;  // k_gemm_f64_capped: tile counter (0 at launch) or null for fixed shares
     int tiles;   // k_gemm_f64_capped: tiles holding part of the update
@@ -190,6 +191,8 @@ __device__ __forceinline__ void gemm_tile(const Params &p, long r0, long c0, dou
     long kend = p.K;
     if (p.k_to_row)
         kend = min(kend, r0 + BM);
+    if (p.k_before_tile)
+        kend = min(kend, r0);
     if (part != 0)
     {
         const long mid = kbeg < kend ? kbeg + ((kend - kbeg) / BK / 2) * BK : kend;
@@ -285,6 +288,8 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
     __shared__ double Bs[2][BK * LDS_STRIDE];
     long r0 = static_cast<long>(blockIdx.y) * BM;
     const long c0 = static_cast<long>(blockIdx.x) * BN;
+    if (p.k_before_tile)
+        r0 = static_cast<long>(gridDim.y - 1 - blockIdx.y) * BM;  // longest K ranges first
     if constexpr (ROLE == static_cast<int>(GemmRole::LeafT))
     {
         if (p.paired)
@@ -435,7 +440,8 @@ void gemm_f64(hipStream_t s, const GemmArgs &g)
     require(reinterpret_cast<uintptr_t>(g.A) % 16 == 0 && reinterpret_cast<uintptr_t>(g.B) % 16 == 0,
             "gemm_f64: operands must be 16-byte aligned");
     Params p{ g.A, g.lda, g.B, g.ldb, g.C, g.ldc, g.M, g.N, g.K, g.lower ? 1 : 0, g.diag_off, g.k_from_tile ? 1 : 0,
-              g.kstart_off, g.set ? 1 : 0, g.k_to_row ? 1 : 0, 0, nullptr, 0 };
+              g.kstart_off, g.set ? 1 : 0, g.k_to_row ? 1 : 0, g.k_before_tile ? 1 : 0, 0, nullptr, 0 };
+    require(!g.k_before_tile || (g.max_wgs <= 0 && g.role == GemmRole::Residual), "gemm_f64: K before the row tile is for the residual product");
     dim3 grid(static_cast<unsigned>(ceil_div(g.N, BN)), static_cast<unsigned>(ceil_div(g.M, BM)));
     require(grid.y <= 65535, "gemm_f64: M too large for one launch");
     // tiles holding part of the update, counted as the capped kernel numbers them
@@ -474,6 +480,8 @@ void gemm_f64(hipStream_t s, const GemmArgs &g)
         hipLaunchKernelGGL((k_gemm_f64<true, false, 7>), grid, dim3(THREADS), 0, s, p);
     else if (g.a_trans && !g.b_trans && g.role == GemmRole::Inverse)
         hipLaunchKernelGGL((k_gemm_f64<true, false, 3>), grid, dim3(THREADS), 0, s, p);
+    else if (g.a_trans && !g.b_trans && g.role == GemmRole::Residual)
+        hipLaunchKernelGGL((k_gemm_f64<true, false, 8>), grid, dim3(THREADS), 0, s, p);
     else
         throw Error(GPRAT_ERR_INTERNAL, "gemm_f64: operand layout and role not instantiated");
     GP_CHECK_LAUNCH();

@@ -15,6 +15,9 @@ namespace gpamd::k
 // identity block, so the Cholesky factor of the padded matrix is
 // blockdiag(L, I).
 void assemble_lower(hipStream_t s, const double *x, int r, SEK sek, double *A, long lda, int n, int n_pad);
+// Strict upper triangle of the same matrix: A[i][j] = K(i, j) for i < j
+// (zero where i or j >= n); the lower triangle and diagonal are not touched.
+void assemble_upper(hipStream_t s, const double *x, int r, SEK sek, double *A, long lda, int n, int n_pad);
 
 // B = K*^T (n_pad x m_pad, ldb): B[i][j] = k(test j, train i); zero outside n x m.
 void assemble_cross_t(hipStream_t s,
@@ -92,6 +95,7 @@ enum class GemmRole
     TrailingT = 5,  // Cholesky trailing update from a k-major panel copy
     SolveT = 6,     // TRSM update from a k-major panel copy
     LeafT = 7,      // TRSM diagonal step: k-major block inverse times the right-hand side
+    Residual = 8,   // FP64 product with the strictly lower covariance (variance refinement)
 };
 
 struct GemmArgs
@@ -119,6 +123,11 @@ struct GemmArgs
     // A(i, k) is known zero for k > i (A lower triangular): each row tile
     // stops K at its last row.
     bool k_to_row;
+    // K ends at each row tile's first row (Residual role only): with A(i, k)
+    // read from the strict upper triangle of a symmetric matrix (a_trans),
+    // each tile takes the part of the strictly lower triangle left of its
+    // 128-row diagonal block.
+    bool k_before_tile;
     // true: C = acc; false: C -= acc.
     bool set;
     GemmRole role;
@@ -278,6 +287,29 @@ void fill_narrow(hipStream_t s, double *p, long count, double value);
 void set_diagonal(hipStream_t s, double *A, long lda, int n, double value);
 // Store one int (used to reset the pivot-failure flag without a memset node).
 void set_int(hipStream_t s, int *p, int value);
+
+// ---- FP64 refinement of the variance solves (k_refine.hip) ---------------
+
+// WT[b][k][i] = (float) W[b][i][k] for `count` square nb x nb blocks stored
+// one after another (nb a multiple of 64).
+void transpose_blocks_f32(hipStream_t s, const double *W, float *WT, int nb, int count);
+// R[k][i] = (float) A[(r0 + k) * lda + i] for k < rows, i < cols: a row block
+// of A as the k-major A operand of gemm_f32_tn.
+void rows_to_f32(hipStream_t s, const double *A, long lda, long r0, int rows, int cols, float *R, long ldr);
+// dst[i] = (double) src[i]
+void to_f64(hipStream_t s, const float *src, double *dst, long count);
+// Z[i][c] += sum_{k = t0}^{i-1} U[k * ldu + i] X[k][c], t0 = 128 floor(i / 128),
+// for i < rows, c < cols (cols % 64 == 0): the part of K_sl X inside the
+// 128-row diagonal blocks, K_sl read from the strict upper triangle of U.
+void diag_strict_product(hipStream_t s, const double *U, long ldu, const double *X, long ldx, double *Z, long ldz, int rows, int cols);
+// Refined quadratic forms, column c < mt: s1 = sum_i B[i][c] X[i][c],
+// s2 = sum_i X[i][c] Z[i][c], s3 = sum_i X[i][c]^2 over i < rows (fixed
+// order, two passes through `part`: quad_form_splits(rows, m) * 3 * m
+// doubles); q = 2 s1 - (2 s2 + d s3); var[c] = v - q and, when corr is
+// given, corr[c] = q - q0[c].
+int quad_form_splits(int rows, int m);
+void quad_form(hipStream_t s, const double *B, const double *X, const double *Z, long ld, int rows, int m, double d, double v, const double *q0,
+               int mt, double *part, int splits, double *var, double *corr);
 
 // ---- calibration (k_calibrate.hip) ---------------------------------------
 

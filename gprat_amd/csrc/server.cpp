@@ -78,7 +78,7 @@ long long Server::submit_fit(const gprat_gp_desc &d)
     e->n = d.n;
     e->r = d.n_reg;
     std::copy(d.theta, d.theta + 3, e->theta);
-    e->batched_path = cfg_.batch_max_n > 0 && d.n <= cfg_.batch_max_n;
+    e->small = cfg_.batch_max_n > 0 && d.n <= cfg_.batch_max_n;
     e->x.assign(d.train_in, d.train_in + d.n + d.n_reg - 1);
     e->y.assign(d.train_out, d.train_out + d.n);
     Request r;
@@ -214,7 +214,15 @@ std::size_t Server::predict_need(const Entry &e, int m) const
     // K*^T, the staged V block and the panel copy (panel width wide), block
     // inverses built for a factor without kept ones, the mean's partial sums
     const std::size_t bo = e.batched_path ? 256 : static_cast<std::size_t>(panel_width(static_cast<int>(np)));
-    return 8 * (np * mp + bo * mp + 4 * mp + blocks * 65536 + bo * np + splits * m + m + e.r) + (std::size_t(1) << 20);
+    std::size_t bytes = 8 * (np * mp + bo * mp + 4 * mp + blocks * 65536 + bo * np + splits * m + m + e.r) + (std::size_t(1) << 20);
+    if (!e.small && cfg_.precision == GPRAT_PRECISION_MIXED)
+    {
+        // FP64-refined variance: FP32 K*^T / solution, FP64 K*^T, solution
+        // and residual product, FP32 block inverses and their transposes,
+        // the FP32 panel and row-block copies and the staged blocks
+        bytes += 28 * np * mp + 24 * np * bo + 8 * bo * mp;
+    }
+    return bytes;
 }
 
 // Estimated flops of a request: the factorisation n^3/3 for a fit (or a
@@ -227,6 +235,56 @@ double Server::cost(const Request &r, const Entry &e) const
     if (r.kind == 1)
         f += n * n * r.m;
     return f;
+}
+
+// Cost model of the small-fit routing (milliseconds, FP64, n <= 2048),
+// measured on MI355X (host-timed fits, n = 256 .. 2048): a single-GP fit
+// uses the whole device but is bound by its launches' latency, about linear
+// in n (0.26 ms at n = 256, 1.91 ms at 2048); the batched kernel runs each GP
+// as one workgroup's chain of 64x64 block products on one CU, so a launch
+// lasts about as long as its largest GP's chain (0.37 ms at n = 256, 26.8 ms
+// at 2048 for a launch of one GP).
+static double single_fit_ms(int n) { return 0.05 + 0.00092 * n; }
+static double chain_ms(int n)
+{
+    const double f = static_cast<double>(n) * n * n / 3.0;
+    return 0.3 + f / 9.5e7;
+}
+
+// Largest n of the small fits that should share one batched launch now,
+// from the small fits waiting at and after `first` (up to batch_max), or 0
+// for none. For each cut n_c, the fits with n <= n_c would take chain_ms(n_c)
+// as one launch against their single-GP times spread over the slots; the cut
+// with the largest saving wins (at least two fits). A lone small fit, or a
+// large small fit beside a few small ones, takes the single-GP path.
+int Server::batch_cut(std::deque<Request>::iterator first)
+{
+    std::vector<int> ns;
+    for (auto it = first; it != queue_.end() && static_cast<int>(ns.size()) < cfg_.batch_max; ++it)
+    {
+        if (it->kind != 0)
+            continue;
+        auto qm = models_.find(it->model);
+        if (qm != models_.end() && qm->second->small)
+            ns.push_back(qm->second->n);
+    }
+    std::sort(ns.begin(), ns.end());
+    double single = 0.0, best = 0.0;
+    int cut = 0;
+    for (std::size_t k = 0; k < ns.size(); k++)
+    {
+        single += single_fit_ms(ns[k]);
+        const int count = static_cast<int>(k) + 1;
+        if (count < 2 || (k + 1 < ns.size() && ns[k + 1] == ns[k]))
+            continue;
+        const double saving = single / std::min(count, cfg_.slots) - chain_ms(ns[k]);
+        if (saving > best)
+        {
+            best = saving;
+            cut = ns[k];
+        }
+    }
+    return cut;
 }
 
 // Fits the budget, evicting idle resident factors least recently used first.
@@ -290,6 +348,7 @@ bool Server::pick(Job &job)
         }
         Entry &e = *mit->second;
         std::size_t need;
+        int cut = 0;
         if (r.kind == 1)
         {
             if (e.state == State::Pending || e.rebuilding)
@@ -311,6 +370,12 @@ bool Server::pick(Job &job)
         }
         else
         {
+            // a small fit's path is chosen now, from what is waiting with it
+            if (e.small)
+            {
+                cut = batch_cut(it);
+                e.batched_path = e.n <= cut;
+            }
             need = fit_need(e);
         }
         const bool heavy = cost(r, e) > cfg_.heavy_flops && !(r.kind == 0 && e.batched_path);
@@ -319,7 +384,14 @@ bool Server::pick(Job &job)
             ++it;  // keeps its place; short requests behind it may start
             continue;
         }
-        if (!admit(need))
+        // a predict's own factor must not be the one its admission evicts
+        // (need assumed it resident): it is pinned first
+        if (r.kind == 1)
+            e.pins++;
+        const bool fits = admit(need);
+        if (r.kind == 1 && !fits)
+            e.pins--;
+        if (!fits)
         {
             if (in_flight_ > 0)
                 return false;
@@ -346,7 +418,6 @@ bool Server::pick(Job &job)
         reserved_ += need;
         if (r.kind == 1)
         {
-            e.pins++;
             r.rebuild = !e.model;
             if (r.rebuild)
                 e.rebuilding = true;
@@ -359,14 +430,18 @@ bool Server::pick(Job &job)
         {
             Request &q = *it;
             auto qm = models_.find(q.model);
-            if (q.kind != 0 || qm == models_.end() || !qm->second->batched_path)
+            if (q.kind != 0 || qm == models_.end() || !qm->second->small || qm->second->n > cut)
             {
                 ++it;
                 continue;
             }
+            qm->second->batched_path = true;
             const std::size_t qn = fit_need(*qm->second);
             if (!admit(qn))
+            {
+                qm->second->batched_path = false;
                 break;
+            }
             q.need = qn;
             q.admitted = now;
             reserved_ += qn;
@@ -431,7 +506,9 @@ void Server::worker(int slot)
 void Server::build_single(Context &c, Job &job, Entry &e)
 {
     auto m = std::make_unique<Model>();
-    fit(c, *m, e.x.data(), e.x.size(), e.y.data(), e.y.size(), e.n, e.r, e.theta, cfg_.precision);
+    // small fits are FP64 on either path (mixed precision gains nothing at
+    // sizes bound by launch latency, and both paths then agree to rounding)
+    fit(c, *m, e.x.data(), e.x.size(), e.y.data(), e.y.size(), e.n, e.r, e.theta, e.small ? GPRAT_PRECISION_FP64 : cfg_.precision);
     job.built.emplace(e.id, std::move(m));
 }
 
@@ -544,9 +621,9 @@ void Server::run(Context &c, Job &job, std::vector<RequestResult> &res)
                 m = b->second.get();
                 res[0].refit = 1;
             }
-            predict_view(FactorView{ &c, m->x.get(), m->n, m->n_pad, m->r, m->lda, m->sek, m->L.get(), m->Winv.get(), m->alpha.get(),
-                                     m->precision == GPRAT_PRECISION_MIXED, m->Wpanel.get(), m->wblock },
-                         r0.z.data(), r0.z.size(), r0.m, r0.mean, r0.var);
+            FactorView v = view(*m);
+            v.ctx = &c;  // this slot's stream and pool
+            predict_view(v, r0.z.data(), r0.z.size(), r0.m, r0.mean, r0.var);
         }
     }
     catch (const std::exception &ex)

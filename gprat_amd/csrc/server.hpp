@@ -93,7 +93,8 @@ class Server
         long long id = 0;
         int n = 0, r = 0;
         double theta[3] = { 0, 0, 0 };
-        bool batched_path = false;  // built (and rebuilt) by the batched kernel
+        bool small = false;         // n <= batch_max_n: FP64, batched or single-GP path by load
+        bool batched_path = false;  // built (and rebuilt) by the batched kernel (chosen when the fit starts)
         std::vector<double> x, y;   // host copies for rebuilding after eviction
         std::unique_ptr<Model> model;
         std::size_t bytes = 0;
@@ -133,6 +134,7 @@ class Server
     void finish(Job &job, std::vector<RequestResult> &res);
     void complete(const Request &r, RequestResult res);
     double cost(const Request &r, const Entry &e) const;
+    int batch_cut(std::deque<Request>::iterator first);
     std::size_t fit_need(const Entry &e) const;
     std::size_t predict_need(const Entry &e, int m) const;
 

@@ -50,8 +50,14 @@ enum
 {
     GPRAT_PRECISION_FP64 = 0,
     /* FP64 panel (POTRF/TRSM) and log-det, FP32-MFMA trailing updates,
-     * FP64 iterative refinement of alpha; the variance solves run in FP64
-     * against the mixed-precision factor. */
+     * FP64 iterative refinement of alpha (residual y - K alpha in FP64).
+     * Latent variance: the solves K~^-1 K*^T on FP32 matrix cores with the
+     * mixed-precision factor, then one FP64 residual (K regenerated into the
+     * factor's free upper triangle at fit time) that corrects the quadratic
+     * form k^T K^-1 k to second order; gprat_model_set_variance_refinement
+     * switches that correction off (FP32 forward solve only). The loss's
+     * log-determinant and the gradient's K^-1 come from the mixed-precision
+     * factor unrefined. */
     GPRAT_PRECISION_MIXED = 1
 };
 
@@ -260,6 +266,16 @@ int gprat_model_gradient(gprat_model *model, const int trainable[3], double *los
 /* Mixed-precision models: refinement steps taken for alpha and the last
  * relative correction max|delta| / max|alpha| (0 and 0.0 for FP64 models). */
 int gprat_model_refinement(const gprat_model *model, int *iterations, double *last_correction);
+
+/* Mixed-precision models: enabled != 0 (the default) refines the latent
+ * variance in FP64 as described at GPRAT_PRECISION_MIXED; 0 returns the
+ * FP32 forward solve's unrefined variance (faster, FP32-level error). No
+ * effect on FP64 models. No counterpart in the reference (FP64 only). */
+int gprat_model_set_variance_refinement(gprat_model *model, int enabled);
+
+/* The last variance prediction's largest FP64 correction |q1 - q0| / v
+ * (0.0 for FP64 models and before any refined prediction). */
+int gprat_model_variance_correction(const gprat_model *model, double *correction);
 
 /* Device bytes held by the model. */
 int gprat_model_bytes(const gprat_model *model, size_t *bytes);

@@ -1,18 +1,33 @@
 """Mixed precision (FP32 matrix-core trailing updates, FP64 diagonal blocks,
-FP64 iterative refinement of alpha; variance solve on FP32 matrix cores)
-against the FP64 path on the GPU.
+FP64 iterative refinement of alpha, FP64-refined latent variance) against the
+FP64 oracle.
 
-Bar: predictive mean within 1e-8 (normwise) of FP64. The variance and the
-loss come from the reduced-accuracy factor without refinement; their errors
-are bounded here at the measured levels: variance 1e-5 relative (normwise)
-and loss 1e-6 absolute (the loss is normalised per training point and sits
-near zero, so its relative error is inflated by cancellation)."""
+Bars, checked against the oracle (live at n = 4096 and 16384; from
+tests/golden/oracle_n32768.npz at n = 32768, where the oracle's gradient
+needs minutes of host BLAS):
+  * predictive mean: 1e-8 normwise (alpha refined to 1e-12);
+  * latent variance: 1e-7 normwise with the FP64 refinement (its error is
+    second order in the FP32 solves' error: 3e-10 .. 5e-8 measured, the worse
+    end for 8-long windows, whose covariance is worse conditioned), and the
+    refinement must improve on the FP32 solve alone by at least 10x;
+  * loss: 1e-6 absolute (its log-determinant comes from the mixed-precision
+    factor, unrefined; the loss is normalised per training point and sits
+    near zero, so its relative error is inflated by cancellation);
+  * gradient: 1e-5 relative (K^-1 from the mixed-precision factor).
+At n = 65536 the oracle is out of reach on the host (17 GB of tiles and
+minutes per factorisation); there the FP64 GPU path, itself pinned to the
+oracle, is the reference for mean, variance and loss.
+"""
+import os
+
 import numpy as np
 import pytest
 
-from conftest import norm_rel
+from conftest import GOLDEN, norm_rel
 
 pytestmark = pytest.mark.gpu
+
+MEAN, VAR, LOSS_ABS, GRAD = 1e-8, 1e-7, 1e-6, 1e-5
 
 
 @pytest.fixture(scope="module")
@@ -23,48 +38,99 @@ def ctx():
     c.close()
 
 
-@pytest.mark.parametrize("n,r,m,noise", [(4096, 8, 1024, 0.1), (16384, 8, 4096, 0.1), (32768, 16, 2048, 0.05),
-                                         (65536, 16, 8192, 0.05)])
-def test_mixed_vs_fp64(ctx, n, r, m, noise):
+def _oracle_grad(oracle, x, y, T, b, r, theta):
+    g = oracle.GP(list(x), list(y), T, b, r, list(theta), [True, True, True])
+    g.optimize_step(oracle.AdamParams(0.1, 0.9, 0.999, 1e-8, 1), 0)
+    return np.array(g.adam_moments[0]) / (1.0 - 0.9)
+
+
+def _check(mix, z, m, om, ov, ol, og, label):
+    mmx, vmx = mix.predict(z, m)
+    corr = mix.variance_correction()
+    lmx, gmx = mix.gradient()
+    mix.set_variance_refinement(False)
+    _, vun = mix.predict(z, m)
+    mix.set_variance_refinement(True)
+    it, last = mix.refinement()
+    em, ev, eu = norm_rel(mmx, om), norm_rel(vmx, ov), norm_rel(vun, ov)
+    eg = float(np.max(np.abs(np.asarray(gmx) - og) / np.abs(og)))
+    print(f"{label}: mean {em:.2e} var {ev:.2e} (unrefined {eu:.2e}, correction {corr:.1e}) "
+          f"loss abs {abs(lmx - ol):.2e} grad {eg:.2e}; alpha refinement {it} steps (last {last:.1e})")
+    assert em < MEAN
+    assert ev < VAR and ev * 10 <= eu
+    assert abs(lmx - ol) < LOSS_ABS
+    assert eg < GRAD
+    assert 1 <= it <= 30
+
+
+@pytest.mark.parametrize("T,b,r,m,noise", [(8, 512, 8, 1024, 0.1), (32, 512, 16, 2048, 0.05)])
+def test_mixed_vs_oracle(ctx, oracle, T, b, r, m, noise):
     from gprat_amd import capi, datagen
+    n = T * b
     theta = (1.0, 1.0, noise)
     x, y, z = datagen.gp_problem(n, m, r)
-    f64 = capi.Model(ctx, x, y, n, r, theta, capi.PRECISION_FP64)
+    go = oracle.GP(list(x), list(y), T, b, r, list(theta), [True, True, True])
+    mt, mb = oracle.compute_test_tiles(m, T, b)
+    assert mt * mb == m
+    om, ov = go.predict_with_uncertainty(list(z), mt, mb)
+    ol = go.compute_loss()
+    og = _oracle_grad(oracle, x, y, T, b, r, theta)
     mix = capi.Model(ctx, x, y, n, r, theta, capi.PRECISION_MIXED)
     try:
-        m64, v64 = f64.predict(z, m)
-        mmx, vmx = mix.predict(z, m)
-        it, last = mix.refinement()
-        em, ev = norm_rel(mmx, m64), norm_rel(vmx, v64)
-        l64, lmx = f64.loss(), mix.loss()
-        print(f"n={n}: mean {em:.3e} var {ev:.3e} loss abs {abs(lmx - l64):.3e} rel {abs(lmx - l64) / abs(l64):.3e} "
-              f"refinement {it} steps (last {last:.1e})")
-        assert em < 1e-8
-        assert ev < 1e-5
-        assert abs(lmx - l64) < 1e-6
-        assert 1 <= it <= 30
+        _check(mix, z, m, om, ov, ol, og, f"n={n} r={r}")
     finally:
-        f64.release()
         mix.release()
 
 
-def test_mixed_gradient_vs_fp64(ctx):
-    """Gradient from a mixed-precision factor (K^-1 = W^T W of the reduced-
-    accuracy factor, refined alpha) against the FP64 model's."""
+def test_mixed_n32768_vs_oracle(ctx):
+    """n = 32768 in 32 tiles of 1024, n_reg = 16, noise 0.05, the first 2048
+    test points (the mixed-precision configuration's training size)."""
     from gprat_amd import capi, datagen
-    n, r, theta = 8192, 8, (1.0, 1.0, 0.1)
-    x, y, _ = datagen.gp_problem(n, 1, r)
+    ref = np.load(os.path.join(GOLDEN, "oracle_n32768.npz"))
+    T, b, r, m = (int(v) for v in ref["shape"])
+    theta = tuple(float(v) for v in ref["theta"])
+    x, y, z = datagen.gp_problem(T * b, m, r)
+    mix = capi.Model(ctx, x, y, T * b, r, theta, capi.PRECISION_MIXED)
+    try:
+        _check(mix, z, m, ref["mean"], ref["var"], float(ref["loss"]), ref["grad"], "n=32768")
+    finally:
+        mix.release()
+
+
+def test_mixed_n65536_vs_fp64(ctx):
+    from gprat_amd import capi, datagen
+    n, r, m, theta = 65536, 16, 8192, (1.0, 1.0, 0.05)
+    x, y, z = datagen.gp_problem(n, m, r)
     f64 = capi.Model(ctx, x, y, n, r, theta, capi.PRECISION_FP64)
+    m64, v64 = f64.predict(z, m)
+    l64 = f64.loss()
+    f64.release()
     mix = capi.Model(ctx, x, y, n, r, theta, capi.PRECISION_MIXED)
     try:
-        l64, g64 = f64.gradient()
-        lmx, gmx = mix.gradient()
-        ref_loss, ref_grad = ctx.loss_gradient(x, y, theta, 8, 1024, r)
-        assert abs(l64 - ref_loss) <= 1e-12 * abs(ref_loss)
-        assert np.max(np.abs(g64 - ref_grad)) <= 1e-12 * np.max(np.abs(ref_grad))
-        err = np.max(np.abs(gmx - g64)) / np.max(np.abs(g64))
-        print(f"mixed gradient rel err {err:.2e}: {gmx} vs {g64}")
-        assert err < 5e-3  # K^-1 carries the FP32 factor's error (no refinement for a full inverse)
+        mmx, vmx = mix.predict(z, m)
+        lmx = mix.loss()
+        em, ev = norm_rel(mmx, m64), norm_rel(vmx, v64)
+        print(f"n=65536 vs FP64 GPU: mean {em:.2e} var {ev:.2e} loss abs {abs(lmx - l64):.2e} "
+              f"correction {mix.variance_correction():.1e}")
+        assert em < MEAN and ev < VAR and abs(lmx - l64) < LOSS_ABS
     finally:
-        f64.release()
         mix.release()
+
+
+def test_refined_variance_of_a_mixed_batch(ctx, oracle):
+    """The batched fit's mixed-precision GPs get the same FP64-refined
+    variance (the covariance for the residual is assembled per call)."""
+    from gprat_amd import capi, datagen
+    T, b, r, m = 3, 400, 8, 300
+    theta = (1.0, 1.0, 0.1)
+    x, y, z = datagen.gp_problem(T * b, m, r)
+    bt = capi.Batch(ctx, [(x, y, T * b, r, theta)], precision=capi.PRECISION_MIXED)
+    try:
+        mean, var = bt.predict(0, z, m)
+    finally:
+        bt.release()
+    go = oracle.GP(list(x), list(y), T, b, r, list(theta), [True, True, True])
+    mt, mb = oracle.compute_test_tiles(m, T, b)
+    om, ov = go.predict_with_uncertainty(list(z), mt, mb)
+    print(f"mixed batch GP n={T * b}: mean {norm_rel(mean, om):.2e} var {norm_rel(var, ov):.2e}")
+    assert norm_rel(mean, om) < MEAN and norm_rel(var, ov) < VAR
