@@ -1,29 +1,38 @@
 #!/usr/bin/env python3
 """Benchmark of the GP fit+predict path on one MI355X.
 
-A step is one fit+predict request at the plain-FP64 scale configuration:
-n_train = 16384 (32 tiles of 512), n_reg = 8, n_test = 4096, theta = (1, 1, 0.1),
-seeded mass-spring-damper data. The request assembles K, factors it, solves
-alpha = K^-1 y and returns the predictive mean and latent variance at the
-test points (the reference's predict_with_uncertainty). Inputs are placed in
-HBM before the timed region and outputs stay there, so `value` excludes PCIe.
+A step is one fit+predict request at the mixed-precision configuration:
+n_train = 32768 (32 tiles of 1024), n_reg = 16, n_test = 8192,
+theta = (1, 1, 0.05), seeded mass-spring-damper data. The request assembles
+K, factors it with FP64 diagonal blocks and FP32 matrix-core trailing
+updates, solves alpha = K^-1 y with FP64 iterative refinement, and returns
+the predictive mean and the latent variance, refined in FP64 (the
+reference's predict_with_uncertainty). Inputs are placed in HBM before the
+timed region and outputs stay there, so `value` excludes PCIe.
 
 Also reported on the same line:
-  * p50 / p99 request latency over the timed steps;
-  * the n = 32768 FP64 Cholesky rate (n^3/3 flops over the factorisation's
-    wall time, n_reg = 16, noise 0.05);
-  * `roofline` for the dominant kernel (the Cholesky trailing update on FP64
-    MFMA), from HIP events bracketing each of its launches in the timed
-    region, against the 78.6 TFLOP/s FP64 matrix peak;
+  * p50 / p99 request latency over the timed steps, and the request's error
+    against the oracle (mean and variance of the first 2048 test points,
+    loss and gradient; tests/golden/oracle_n32768.npz);
+  * the same request in plain FP64 and with the variance unrefined (FP32
+    solve only), and the speed-up of each mixed variant over FP64;
+  * the plain-FP64 scale configuration (n_train = 16384 in 32 tiles of 512,
+    n_reg 8, n_test 4096) as a request rate with p50 / p99, and its
+    10-iteration Adam optimize;
+  * the n = 32768 FP64 Cholesky rate, the n = 65536 mixed-precision request,
+    the batched many-GP fit and the serving loop (median of three runs);
+  * `roofline` for the request's dominant kernel, from HIP events bracketing
+    each of its launches in the timed region;
   * `cpu_baseline`: the CPU oracle (a restatement of the reference's HPX +
-    BLAS path, oneMKL sequential per tile on a host pool) timed on one full
-    request on this host's cores, and the GPU result's error against it.
+    BLAS path, oneMKL sequential per tile on a host pool) on a bounded
+    sample of the workload, scaled to a whole request by its flops.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--headline-only]
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -37,6 +46,10 @@ FP64_MFMA_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
+# the request of the headline (mixed-precision configuration, n = 32768)
+HN, HT, HR, HM = 32768, 32, 16, 8192
+HTHETA = (1.0, 1.0, 0.05)
+# the plain-FP64 scale configuration
 N_TILES, TILE, N_REG, N_TEST = 32, 512, 8, 4096
 THETA = (1.0, 1.0, 0.1)
 
@@ -70,12 +83,19 @@ def trailing_update_flops(n_pad: int):
     return out
 
 
-def measured_traffic():
-    """HBM bytes per launch of the trailing update at this workload, from the
-    committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE per
-    dispatch, each scaled by what the counter reads for a 1 GiB copy);
-    None when that summary is absent."""
-    path = os.path.join(ROOT, "profiles", "trailing_update_n16384_traffic.md")
+def residual_flops(n_pad: int, m_pad: int) -> float:
+    """Algorithmic flops of the variance refinement's FP64 product Z = K_sl X:
+    every 128-row tile t takes the strictly lower part left of its diagonal
+    block, 2 * (128 t) * 128 * m_pad flops."""
+    T = n_pad // 128
+    return 2.0 * m_pad * 128 * 128 * T * (T - 1) / 2.0
+
+
+def measured_traffic(name):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by what the
+    counter reads for a 1 GiB copy); None when that summary is absent."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -88,27 +108,58 @@ def percentile(values, q):
     return float(np.percentile(np.asarray(values), q))
 
 
-def cpu_baseline(x, y, z, gpu_mean, gpu_var):
+def host_description():
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return os.cpu_count() or 1, model
+
+
+def cpu_baseline(repeats=3):
+    """The oracle on a bounded sample of the headline workload: fit +
+    predict_with_uncertainty at n = 16384 (16 tiles of 1024, n_reg 16, noise
+    0.05) on 1024 test points, one warm-up and `repeats` timed runs, median.
+    A whole request (n = 32768, m = 8192) does F = n^3/3 + n^2 m flops of
+    tile BLAS against the sample's; `value` scales the sample's median time
+    by that ratio."""
     sys.path.insert(0, os.path.join(ROOT, "oracle", "_build"))
     import gprat_oracle
-    threads = min(os.cpu_count() or 1, 16)
+    from gprat_amd import datagen
+    nproc, model = host_description()
+    threads = min(nproc, 16)
     gprat_oracle.set_threads(threads)
-    mt, mb = gprat_oracle.compute_test_tiles(N_TEST, N_TILES, TILE)
-    gp = gprat_oracle.GP(list(x), list(y), N_TILES, TILE, N_REG, list(THETA), [True, True, True])
-    t0 = time.perf_counter()
-    mean, var = gp.predict_with_uncertainty(list(z), mt, mb)
-    dt = time.perf_counter() - t0
-    mean, var = np.asarray(mean), np.asarray(var)
-    err_mean = float(np.max(np.abs(gpu_mean - mean)) / np.max(np.abs(mean)))
-    err_var = float(np.max(np.abs(gpu_var - var)) / np.max(np.abs(var)))
+    T, b, r, m = 16, 1024, HR, 1024
+    x, y, z = datagen.gp_problem(T * b, m, r)
+    mt, mb = gprat_oracle.compute_test_tiles(m, T, b)
+    times = []
+    for rep in range(repeats + 1):
+        gp = gprat_oracle.GP(list(x), list(y), T, b, r, list(HTHETA), [True, True, True])
+        t0 = time.perf_counter()
+        gp.predict_with_uncertainty(list(z), mt, mb)
+        dt = time.perf_counter() - t0
+        if rep > 0:
+            times.append(dt)
+    med = float(np.median(times))
+    ns = T * b
+    scale = (HN ** 3 / 3.0 + HN ** 2 * HM) / (ns ** 3 / 3.0 + ns ** 2 * m)
     return {
-        "value": 1.0 / dt,
+        "value": 1.0 / (med * scale),
         "unit": "req/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"1 full fit+predict request (n={N_TILES * TILE}, m={N_TEST}) on the oracle, "
-                  f"{gprat_oracle.blas_name()} sequential per tile, {threads} pool threads; {dt:.2f} s",
-    }, {"mean": err_mean, "var": err_var}
+        "sample": f"oracle fit + predict_with_uncertainty at n={ns} (16 tiles of 1024, n_reg {r}), m={m}: "
+                  f"1 warm-up + {repeats} runs, median {med:.2f} s ({', '.join(f'{t:.2f}' for t in times)}); "
+                  f"scaled by the flop ratio {scale:.2f} to one n={HN}, m={HM} request; "
+                  f"{gprat_oracle.blas_name()} sequential per tile, {threads} pool threads (cap 16)",
+        "sample_req_s": 1.0 / med,
+        "nproc": nproc,
+        "cpu_model": model,
+    }
 
 
 def cholesky_rate(capi, ctx, n=32768, repeats=2):
@@ -135,46 +186,124 @@ def cholesky_rate(capi, ctx, n=32768, repeats=2):
     return {"n": n, "ms_best": best, "ms_mean": float(np.mean(times)), "tflops": flops / (best * 1e-3) / 1e12}
 
 
-def mixed_leg(capi, ctx, n=32768, m=8192, r=16, noise=0.05, repeats=2):
-    """Mixed-precision configuration: n in {32768, 65536} (tiles of 1024),
-    n_reg 16, m=8192, noise 0.05. Times one fit+predict request in FP64 and
-    in mixed precision (FP32 matrix-core trailing updates + FP64 refinement
-    of alpha, variance solve on FP32 matrix cores) and reports the mixed
-    result's error against the FP64 one."""
+def request_leg(capi, ctx, n, T, r, m, theta, precision, steps, warmup=1, refine_variance=True, families=()):
+    """Times `steps` fit+predict requests (inputs resident in HBM): latency
+    per request and, for `families`, the kernel-family stats of the timed
+    requests. Returns the timing and the last request's model (released by
+    the caller)."""
     from gprat_amd import datagen
     x, y, z = datagen.gp_problem(n, m, r)
-    theta = (1.0, 1.0, noise)
     dx, dy, dz = ctx.device_array(x), ctx.device_array(y), ctx.device_array(z)
-    out = {}
-    res = {}
-    for name, prec in (("fp64", capi.PRECISION_FP64), ("mixed", capi.PRECISION_MIXED)):
-        times = []
-        for rep in range(repeats + 1):
-            ctx.synchronize()
-            t0 = time.perf_counter()
-            mdl = capi.Model(ctx, dx, dy, n, r, theta, prec)
-            mean, var = mdl.predict(dz, m)
-            ctx.synchronize()
-            dt = time.perf_counter() - t0
-            if rep > 0:
-                times.append(dt)
-            if rep == repeats:
-                res[name] = (mean, var, mdl.loss(), mdl.refinement())
-            mdl.release()
-        out[name + "_ms"] = 1e3 * min(times)
-    m64, v64, l64, _ = res["fp64"]
-    mmx, vmx, lmx, (it, last) = res["mixed"]
-    out["speedup"] = out["fp64_ms"] / out["mixed_ms"]
-    out["rel_err_vs_fp64"] = {
-        "mean": float(np.max(np.abs(mmx - m64)) / np.max(np.abs(m64))),
-        "var": float(np.max(np.abs(vmx - v64)) / np.max(np.abs(v64))),
-        "loss": abs(lmx - l64) / abs(l64),
+    dmean, dvar = ctx.device_empty(m), ctx.device_empty(m)
+
+    def step(keep=False):
+        mdl = capi.Model(ctx, dx, dy, n, r, theta, precision)
+        if not refine_variance:
+            mdl.set_variance_refinement(False)
+        mdl.predict(dz, m, variance=True, out=(dmean, dvar))
+        if keep:
+            return mdl
+        mdl.release()
+        return None
+
+    for _ in range(warmup):
+        step()
+    ctx.synchronize()
+    if families:
+        ctx.set_profiling(True)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s = time.perf_counter()
+        step()
+        lat.append(time.perf_counter() - s)
+    ctx.synchronize()
+    total = time.perf_counter() - t0
+    fams = {f: ctx.kernel_stats(f) for f in families}
+    if families:
+        ctx.set_profiling(False)
+    model = step(keep=True)
+    out = {
+        "req_s": steps / total,
+        "ms_per_request": total / steps * 1e3,
+        "p50_ms": percentile(lat, 50) * 1e3,
+        "p99_ms": percentile(lat, 99) * 1e3,
+        "steps": steps,
+        "workload": f"fit+predict n={n} ({T} tiles of {n // T}), n_reg={r}, m={m}, theta={theta}",
     }
-    out["refinement_steps"] = it
-    out["workload"] = f"fit+predict n={n} ({n // 1024} tiles of 1024), n_reg={r}, m={m}, noise={noise}"
-    for a in (dx, dy, dz):
+    mean, var = dmean.numpy(), dvar.numpy()
+    for a in (dx, dy, dz, dmean, dvar):
         a.free()
-    return out
+    return out, fams, model, (mean, var, x, y, z)
+
+
+def vs_oracle_n32768(mean, var, loss, grad):
+    """Errors of the first 2048 test points' mean and variance, the loss and
+    the gradient against the oracle at n = 32768 (committed fixture)."""
+    ref = np.load(os.path.join(ROOT, "tests", "golden", "oracle_n32768.npz"))
+    k = len(ref["mean"])
+    return {
+        "mean": float(np.max(np.abs(mean[:k] - ref["mean"])) / np.max(np.abs(ref["mean"]))),
+        "var": float(np.max(np.abs(var[:k] - ref["var"])) / np.max(np.abs(ref["var"]))),
+        "loss_abs": abs(loss - float(ref["loss"])),
+        "loss_rel": abs(loss - float(ref["loss"])) / abs(float(ref["loss"])),
+        "grad_rel_max": float(np.max(np.abs(np.asarray(grad) - ref["grad"]) / np.abs(ref["grad"]))),
+        "points": int(k),
+        "norm": "max|a-b| / max|b| over the first 2048 test points (mean, var); scalar relative (loss, grad)",
+    }
+
+
+def mixed64k_leg(capi, ctx, n=65536, m=8192, r=16):
+    """n = 65536 (64 tiles of 1024): one FP64 and one mixed-precision
+    (refined) request, after one warm-up of the mixed one; the mixed
+    result's error against the FP64 one."""
+    theta = HTHETA
+    res = {}
+    for name, prec in (("mixed", capi.PRECISION_MIXED), ("fp64", capi.PRECISION_FP64)):
+        t, _, mdl, (mean, var, *_rest) = request_leg(capi, ctx, n, 64, r, m, theta, prec, steps=1, warmup=1 if name == "mixed" else 0)
+        res[name] = (t, mean, var, mdl.loss())
+        mdl.release()
+    (t32, m32, v32, l32), (t64, m64, v64, l64) = res["mixed"], res["fp64"]
+    return {
+        "fp64_ms": t64["ms_per_request"],
+        "mixed_ms": t32["ms_per_request"],
+        "speedup": t64["ms_per_request"] / t32["ms_per_request"],
+        "rel_err_vs_fp64": {"mean": float(np.max(np.abs(m32 - m64)) / np.max(np.abs(m64))),
+                            "var": float(np.max(np.abs(v32 - v64)) / np.max(np.abs(v64))),
+                            "loss_abs": abs(l32 - l64)},
+        "workload": t32["workload"],
+    }
+
+
+def optimize_leg(repeats=1):
+    """The plain-FP64 scale configuration's 10-iteration Adam optimize
+    (lr 0.1) through the gprat facade on the GPU, against the oracle's
+    losses and hyperparameters (committed fixture)."""
+    import gprat_amd as gprat
+    from gprat_amd import datagen
+    ref = np.load(os.path.join(ROOT, "tests", "golden", "oracle_n16384.npz"))
+    T, b, r = (int(v) for v in ref["shape"])
+    x, y, _ = datagen.gp_problem(T * b, 1, r)
+    theta = [float(v) for v in ref["theta"]]
+    gp = gprat.GP(list(x), list(y), T, b, r, theta, [True, True, True], 0, 1)
+    gp.optimize(gprat.AdamParams(0.1, 0.9, 0.999, 1e-8, 1))  # warm-up
+    gp = gprat.GP(list(x), list(y), T, b, r, theta, [True, True, True], 0, 1)
+    t0 = time.perf_counter()
+    losses = gp.optimize(gprat.AdamParams(0.1, 0.9, 0.999, 1e-8, 10))
+    dt = time.perf_counter() - t0
+    k = gp.kernel_params
+    th = np.array([k.lengthscale, k.vertical_lengthscale, k.noise_variance])
+    n = T * b
+    return {
+        "iterations": 10,
+        "ms_total": dt * 1e3,
+        "ms_per_iteration": dt * 1e2,
+        "rel_err_vs_oracle": {"losses": float(np.max(np.abs(np.asarray(losses) - ref["opt_losses"]) / np.abs(ref["opt_losses"]))),
+                              "theta": float(np.max(np.abs(th - ref["opt_theta"]) / np.abs(ref["opt_theta"])))},
+        "flops_per_iteration_this_algorithm": n ** 3 / 3.0 * 3,
+        "flops_per_iteration_reference_algorithm": n ** 3 / 3.0 + 2.0 * n ** 3,
+        "workload": f"optimize: n={n} ({T} tiles of {b}), n_reg={r}, 10 Adam iterations at lr 0.1 (K^-1 = W^T W per iteration)",
+    }
 
 
 def batched_leg(capi, ctx, count=2048, seed=4):
@@ -255,16 +384,17 @@ def serving_trace(count=192, seed=7):
     return reqs
 
 
-def serving_leg(capi, count=192, budget=200 << 30, slots=4):
+def serving_leg(capi, count=192, budget=200 << 30, slots=4, runs=3):
     """Request loop with continuous batching against the sequential plain
-    FP64 path on the same trace.
+    FP64 path on the same trace; every figure is the median of `runs` runs.
 
     * sequential: one request at a time through resident FP64 models (fit =
       assemble + factor + alpha + loss; predict = mean + variance on the
       resident factor). `sequential_refit_req_s` prices the reference's API
       instead, where nothing stays on the device and every predict refits.
-    * saturated: the whole trace queued at t = 0 on the server (mixed precision
-      for single-GP fits, batched FP64 launches for n <= 2048, 200 GB budget).
+    * saturated: the whole trace queued at t = 0 on the server (mixed
+      precision with FP64 refinement for fits with n > 2048; smaller fits in
+      FP64, batched when several wait together and that pays; 200 GB budget).
     * poisson: open-loop Poisson arrivals at 50% and 80% of the saturated
       rate; latency = completion - arrival.
     """
@@ -272,29 +402,29 @@ def serving_leg(capi, count=192, budget=200 << 30, slots=4):
     reqs = serving_trace(count)
     theta = (1.0, 1.0, 0.1)
 
-    # sequential plain FP64, on a context of its own whose cached blocks are
-    # freed before the server starts
-    seq = capi.Context(0, 1)
-    models, times = {}, []
-    t_start = time.perf_counter()
-    for i, r in enumerate(reqs):
-        s = time.perf_counter()
-        if r[0] == "fit":
-            m = capi.Model(seq, r[1], r[2], r[3], 8, theta, capi.PRECISION_FP64)
-            m.loss()
-            models[i] = m
-        else:
-            models[r[1]].predict(r[2], r[3], variance=True)
-        times.append(time.perf_counter() - s)
-    seq_total = time.perf_counter() - t_start
-    refit_total = seq_total + sum(times[r[1]] for r in reqs if r[0] == "predict")
-    for m in models.values():
-        m.release()
-    models.clear()
-    seq.close()
+    def sequential():
+        # on a context of its own whose cached blocks are freed afterwards
+        seq = capi.Context(0, 1)
+        models, times = {}, []
+        t_start = time.perf_counter()
+        for i, r in enumerate(reqs):
+            s = time.perf_counter()
+            if r[0] == "fit":
+                m = capi.Model(seq, r[1], r[2], r[3], 8, theta, capi.PRECISION_FP64)
+                m.loss()
+                models[i] = m
+            else:
+                models[r[1]].predict(r[2], r[3], variance=True)
+            times.append(time.perf_counter() - s)
+        total = time.perf_counter() - t_start
+        refit = total + sum(times[r[1]] for r in reqs if r[0] == "predict")
+        for m in models.values():
+            m.release()
+        seq.close()
+        return total, refit, times
 
     def run(srv, arrivals):
-        tickets, submit_t = [None] * len(reqs), [0.0] * len(reqs)
+        tickets = [None] * len(reqs)
         t0 = time.perf_counter()
 
         def submitter():
@@ -303,7 +433,6 @@ def serving_leg(capi, count=192, budget=200 << 30, slots=4):
                     d = t0 + arrivals[i] - time.perf_counter()
                     if d > 0:
                         time.sleep(d)
-                submit_t[i] = time.perf_counter()
                 if r[0] == "fit":
                     tickets[i] = srv.submit_fit(r[1], r[2], r[3], 8, theta)
                 else:
@@ -324,55 +453,90 @@ def serving_leg(capi, count=192, budget=200 << 30, slots=4):
                 srv.release_model(tickets[i])
         return total, lat, bad
 
+    seq_runs = [sequential() for _ in range(runs)]
+    seq_total = float(np.median([t for t, _, _ in seq_runs]))
+    refit_total = float(np.median([f for _, f, _ in seq_runs]))
+    seq_times = seq_runs[int(np.argsort([t for t, _, _ in seq_runs])[len(seq_runs) // 2])][2]
+    failed = 0
     with capi.Server(0, slots=slots, hbm_budget=budget, precision=capi.PRECISION_MIXED) as srv:
         run(srv, None)  # warm-up: device pools and code paths
-        sat_total, sat_lat, sat_bad = run(srv, None)
-        poisson, po_bad = [], 0
+        sat = [run(srv, None) for _ in range(runs)]
+        failed += sum(b for _, _, b in sat)
+        sat_total = float(np.median([t for t, _, _ in sat]))
+        sat_lat = sat[int(np.argsort([t for t, _, _ in sat])[len(sat) // 2])][1]
+        poisson = []
         for frac in (0.5, 0.8):
             rate = frac * len(reqs) / sat_total
             gaps = np.random.default_rng(11).exponential(1.0 / rate, len(reqs))
-            po_total, po_lat, bad = run(srv, np.cumsum(gaps) - gaps[0])
-            po_bad += bad
-            poisson.append({"offered_req_s": rate, "fraction_of_saturated": frac, "achieved_req_s": len(reqs) / po_total,
-                            "p50_ms": percentile(po_lat, 50) * 1e3, "p99_ms": percentile(po_lat, 99) * 1e3})
+            po = [run(srv, np.cumsum(gaps) - gaps[0]) for _ in range(runs)]
+            failed += sum(b for _, _, b in po)
+            poisson.append({"offered_req_s": rate, "fraction_of_saturated": frac,
+                            "achieved_req_s": float(np.median([len(reqs) / t for t, _, _ in po])),
+                            "p50_ms": float(np.median([percentile(l, 50) for _, l, _ in po])) * 1e3,
+                            "p99_ms": float(np.median([percentile(l, 99) for _, l, _ in po])) * 1e3,
+                            "p99_ms_runs": [percentile(l, 99) * 1e3 for _, l, _ in po]})
         stats = srv.stats()
     n_fit = sum(r[0] == "fit" for r in reqs)
     # dense work of the trace: n^3/3 per fit, n^2 m per predict (variance
     # solve); the request rate if all of it ran at the matrix-core peaks
     work = sum(r[3] ** 3 / 3.0 for r in reqs if r[0] == "fit") + sum(reqs[r[1]][3] ** 2 * r[3] for r in reqs if r[0] == "predict")
+    fp32_bound = len(reqs) / (work / (FP32_MFMA_PEAK_TFLOPS * 1e12))
     return {
         "requests": len(reqs),
         "fits": n_fit,
         "predicts": len(reqs) - n_fit,
         "slots": slots,
+        "runs": runs,
         "hbm_budget_gb": budget / 2 ** 30,
         "sequential_fp64_req_s": len(reqs) / seq_total,
+        "sequential_fp64_req_s_runs": [len(reqs) / t for t, _, _ in seq_runs],
         "sequential_refit_req_s": len(reqs) / refit_total,
         "saturated_req_s": len(reqs) / sat_total,
+        "saturated_req_s_runs": [len(reqs) / t for t, _, _ in sat],
         "speedup_vs_sequential": seq_total / sat_total,
         "speedup_vs_sequential_refit": refit_total / sat_total,
         "saturated_p50_ms": percentile(sat_lat, 50) * 1e3,
         "saturated_p99_ms": percentile(sat_lat, 99) * 1e3,
         "poisson": poisson,
-        "sequential_p50_ms": percentile(times, 50) * 1e3,
-        "sequential_p99_ms": percentile(times, 99) * 1e3,
-        "failed": sat_bad + po_bad,
+        "sequential_p50_ms": percentile(seq_times, 50) * 1e3,
+        "sequential_p99_ms": percentile(seq_times, 99) * 1e3,
+        "failed": failed,
         "trace_flops": work,
         "req_s_if_all_work_at_fp64_peak": len(reqs) / (work / (FP64_MFMA_PEAK_TFLOPS * 1e12)),
-        "req_s_if_all_work_at_fp32_peak": len(reqs) / (work / (FP32_MFMA_PEAK_TFLOPS * 1e12)),
+        "req_s_if_all_work_at_fp32_peak": fp32_bound,
+        "saturated_fraction_of_fp32_peak_bound": (len(reqs) / sat_total) / fp32_bound,
         "server_stats": {k: stats[k] for k in ("batched_fits", "batch_launches", "evictions", "refits")},
         "workload": "70% fits n log-uniform [256, 32768] (n_reg 8), 30% predicts m uniform [1, 4096] on earlier models",
+    }
+
+
+def roofline_of(kernel, flops_per_launch, fam, peak, traffic_file):
+    launches, ms = fam
+    if not launches or ms <= 0:
+        return None
+    achieved = flops_per_launch * launches / (ms * 1e-3) / 1e12
+    return {
+        "kernel": kernel,
+        "bound": "mfma",
+        "achieved": achieved,
+        "peak": peak,
+        "unit": "TFLOP/s",
+        "frac": achieved / peak,
+        "traffic": measured_traffic(traffic_file),
+        "launches": launches,
+        "avg_launch_ms": ms / launches,
+        "flops_per_launch_avg": flops_per_launch,
     }
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--headline-only", action="store_true", help="only the timed requests of the headline (for profiling)")
     ap.add_argument("--chol-n", type=int, default=32768, help="size of the Cholesky-rate leg (0 skips it)")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched many-GP leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop leg")
     args = ap.parse_args()
@@ -381,109 +545,98 @@ def main():
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
-    from gprat_amd import capi, datagen
+    from gprat_amd import capi
 
-    n = N_TILES * TILE
-    x, y, z = datagen.gp_problem(n, N_TEST, N_REG)
     ctx = capi.Context(0, 1)
-    dx, dy, dz = ctx.device_array(x), ctx.device_array(y), ctx.device_array(z)
-    dmean, dvar = ctx.device_empty(N_TEST), ctx.device_empty(N_TEST)
-
-    def step():
-        model = capi.Model(ctx, dx, dy, n, N_REG, THETA)
-        model.predict(dz, N_TEST, variance=True, out=(dmean, dvar))
-        model.release()
-
-    for _ in range(args.warmup):
-        step()
-    ctx.synchronize()
-    ctx.set_profiling(True)
-    lat = []
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s = time.perf_counter()
-        step()
-        lat.append(time.perf_counter() - s)
-    ctx.synchronize()
-    total = time.perf_counter() - t0
-    fams = {f: ctx.kernel_stats(f) for f in
-            ("syrk_trailing", "panel", "panel_rows", "cholesky", "assemble", "solve", "trsm", "mean")}
-    ctx.set_profiling(False)
-    gpu_mean, gpu_var = dmean.numpy(), dvar.numpy()
-
-    per_launch = trailing_update_flops(((n + 127) // 128) * 128)
-    launches, syrk_ms = fams["syrk_trailing"]
-    expect = len(per_launch) * args.steps
-    syrk_flops = sum(per_launch) * args.steps
-    achieved = syrk_flops / (syrk_ms * 1e-3) / 1e12 if syrk_ms > 0 else None
-    roofline = {
-        "kernel": "k_gemm_f64<true,false,5> (Cholesky trailing update from the k-major panel copy, FP64 MFMA)",
-        "bound": "mfma",
-        "achieved": achieved,
-        "peak": FP64_MFMA_PEAK_TFLOPS,
-        "unit": "TFLOP/s",
-        "frac": (achieved / FP64_MFMA_PEAK_TFLOPS) if achieved else None,
-        "traffic": measured_traffic(),
-        "launches": launches,
-        "launches_expected": expect,
-        "avg_launch_ms": syrk_ms / launches if launches else None,
-        "flops_per_launch_avg": syrk_flops / launches if launches else None,
-    }
-    chol_ms = fams["cholesky"][1] / max(fams["cholesky"][0], 1)
-
-    calib = ctx.calibrate()
-    roofline["peak_measured"] = calib["fp64_tflops"]
-    roofline["frac_of_measured"] = (achieved / calib["fp64_tflops"]) if achieved else None
-
-    chol = None
-    if args.chol_n > 0:
-        chol = cholesky_rate(capi, ctx, args.chol_n)
-    mixed = None if args.no_mixed else mixed_leg(capi, ctx)
-    mixed64k = None if args.no_mixed else mixed_leg(capi, ctx, n=65536, repeats=1)
-    batched = None if args.no_batched else batched_leg(capi, ctx)
-    for a in (dx, dy, dz, dmean, dvar):
-        a.free()
-    ctx.close()  # the serving leg brings its own contexts and HBM budget
-    serving = None if args.no_serving else serving_leg(capi)
+    families = ("syrk_trailing", "residual", "trsm", "panel", "panel_rows", "cholesky", "assemble", "solve", "mean", "refine")
+    head, fams, model, (mean, var, *_r) = request_leg(capi, ctx, HN, HT, HR, HM, HTHETA, capi.PRECISION_MIXED, args.steps,
+                                                      warmup=args.warmup, families=families)
+    n_pad = ((HN + 127) // 128) * 128
+    m_pad = ((HM + 63) // 64) * 64
+    trailing = trailing_update_flops(n_pad)
+    r_fp32 = roofline_of("k_gemm_f32_tn<double,0> (FP32 matrix-core trailing update of the mixed-precision Cholesky)",
+                         sum(trailing) / len(trailing), fams["syrk_trailing"], FP32_MFMA_PEAK_TFLOPS,
+                         "trailing_update_f32_n32768_traffic.md")
+    r_res = roofline_of("k_gemm_f64<true,false,8> (FP64 residual product K_sl X of the variance refinement)",
+                        residual_flops(n_pad, m_pad), fams["residual"], FP64_MFMA_PEAK_TFLOPS, "residual_n32768_traffic.md")
+    ranked = sorted([r for r in (r_fp32, r_res) if r], key=lambda r: -r["launches"] * r["avg_launch_ms"])
+    roofline = ranked[0] if ranked else None
+    loss = model.loss()
+    _, grad = model.gradient()
+    it, last = model.refinement()
+    vcorr = model.variance_correction()
+    model.release()
+    head["rel_err_vs_oracle"] = vs_oracle_n32768(mean, var, loss, grad)
+    head["alpha_refinement_steps"] = it
+    head["alpha_last_correction"] = last
+    head["variance_max_correction"] = vcorr
+    head["phase_ms_per_request"] = {k: v[1] / args.steps for k, v in fams.items()}
 
     result = {
         "metric": METRIC,
-        "value": args.steps / total,
+        "value": head["req_s"],
         "unit": "req/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": total / args.steps * 1e3,
+        "ms_per_step": head["ms_per_request"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64+f32 (FP32 matrix-core trailing updates and solves, FP64 refinement)",
         "data": "synthetic (seeded mass-spring-damper series, SplitMix64)",
         "config": {
-            "workload": f"fit+predict request: n_train={n} ({N_TILES} tiles of {TILE}), n_reg={N_REG}, "
-                        f"n_test={N_TEST}, theta={THETA}, FP64, inputs resident in HBM",
-            "precision": "fp64",
+            "workload": f"fit+predict request, mixed precision with FP64 refinement of alpha and of the variance: "
+                        f"n_train={HN} ({HT} tiles of {HN // HT}), n_reg={HR}, n_test={HM}, theta={HTHETA}, "
+                        f"inputs resident in HBM",
+            "precision": "mixed",
         },
-        "p50_ms": percentile(lat, 50) * 1e3,
-        "p99_ms": percentile(lat, 99) * 1e3,
-        "cholesky_n16384_ms": chol_ms,
-        "cholesky_n16384_tflops": (n ** 3 / 3.0) / (chol_ms * 1e-3) / 1e12 if chol_ms > 0 else None,
-        "cholesky_n32k": chol,
-        "mixed_precision_n32k": mixed,
-        "mixed_precision_n64k": mixed64k,
-        "batched_2048_gps": batched,
-        "serving": serving,
-        "phase_ms_per_step": {k: v[1] / args.steps for k, v in fams.items()},
-        "device_calibration": calib,
+        "p50_ms": head["p50_ms"],
+        "p99_ms": head["p99_ms"],
+        "headline": head,
         "roofline": roofline,
+        "roofline_other": ranked[1] if len(ranked) > 1 else None,
     }
-    if not args.no_cpu_baseline:
-        base, err = cpu_baseline(x, y, z, gpu_mean, gpu_var)
-        result["cpu_baseline"] = base
-        result["rel_err_vs_oracle"] = err
-    else:
-        result["cpu_baseline"] = None
+    if not args.headline_only:
+        f64, _, mdl, (m64, v64, *_r) = request_leg(capi, ctx, HN, HT, HR, HM, HTHETA, capi.PRECISION_FP64, 3, warmup=1)
+        f64["rel_err_vs_oracle"] = vs_oracle_n32768(m64, v64, mdl.loss(), mdl.gradient()[1])
+        mdl.release()
+        unr, _, mdl, (mu, vu, *_r) = request_leg(capi, ctx, HN, HT, HR, HM, HTHETA, capi.PRECISION_MIXED, 3, warmup=1,
+                                                 refine_variance=False)
+        unr["rel_err_vs_oracle"] = vs_oracle_n32768(mu, vu, mdl.loss(), mdl.gradient()[1])
+        mdl.release()
+        result["fp64_n32k"] = f64
+        result["mixed_unrefined_variance_n32k"] = unr
+        result["speedup_mixed_refined_vs_fp64"] = f64["ms_per_request"] / head["ms_per_request"]
+        result["speedup_mixed_unrefined_vs_fp64"] = f64["ms_per_request"] / unr["ms_per_request"]
+        c2fams = ("syrk_trailing", "cholesky", "panel", "panel_rows", "assemble", "solve", "trsm", "mean")
+        c2, c2f, mdl, _ = request_leg(capi, ctx, N_TILES * TILE, N_TILES, N_REG, N_TEST, THETA, capi.PRECISION_FP64, 10, warmup=2,
+                                      families=c2fams)
+        mdl.release()
+        tr16 = trailing_update_flops(N_TILES * TILE)
+        c2["roofline_trailing_update"] = roofline_of(
+            "k_gemm_f64<true,false,5> (FP64 trailing update)", sum(tr16) / len(tr16), c2f["syrk_trailing"],
+            FP64_MFMA_PEAK_TFLOPS, "trailing_update_n16384_traffic.md")
+        chol_ms = c2f["cholesky"][1] / max(c2f["cholesky"][0], 1)
+        c2["cholesky_ms"] = chol_ms
+        c2["cholesky_tflops"] = (N_TILES * TILE) ** 3 / 3.0 / (chol_ms * 1e-3) / 1e12
+        c2["phase_ms_per_request"] = {k: v[1] / 10 for k, v in c2f.items()}
+        result["fp64_n16k"] = c2
+        result["optimize_n16k"] = optimize_leg()
+        calib = ctx.calibrate()
+        result["device_calibration"] = calib
+        if roofline:
+            peak_meas = calib["fp64_tflops"] if "f64" in roofline["kernel"] else calib.get("fp32_tflops")
+            if peak_meas:
+                roofline["peak_measured"] = peak_meas
+                roofline["frac_of_measured"] = roofline["achieved"] / peak_meas
+        result["cholesky_n32k"] = cholesky_rate(capi, ctx, args.chol_n) if args.chol_n > 0 else None
+        result["mixed_precision_n64k"] = mixed64k_leg(capi, ctx)
+        result["batched_2048_gps"] = None if args.no_batched else batched_leg(capi, ctx)
+    ctx.close()  # the serving leg brings its own contexts and HBM budget
+    if not args.headline_only:
+        result["serving"] = None if args.no_serving else serving_leg(capi)
+    result["cpu_baseline"] = None if (args.no_cpu_baseline or args.headline_only) else cpu_baseline()
     print(json.dumps(result), flush=True)
 
 

@@ -975,7 +975,7 @@ static double variance_refined(Context &c, const FactorView &m, const double *z,
     const int splits = k::quad_form_splits(n_pad, m_pad);
     DevBuf<double> part(c, static_cast<std::size_t>(splits) * 3 * m_pad), corr(c, m_pad);
     {
-        Timed t(c, s, "refine");
+        Timed t(c, s, "residual");
         k::GemmArgs g{};
         g.A = Kup;  // A(i, k) = Kup[k][i] = K(i, k) for k < i
         g.lda = ldk;
@@ -991,6 +991,9 @@ static double variance_refined(Context &c, const FactorView &m, const double *z,
         g.k_before_tile = true;
         g.role = k::GemmRole::Residual;
         k::gemm_f64(s, g);
+    }
+    {
+        Timed t(c, s, "refine");
         k::diag_strict_product(s, Kup, ldk, X64.get(), m_pad, Z.get(), m_pad, n_pad, m_pad);
         k::quad_form(s, B64.get(), X64.get(), Z.get(), m_pad, n_pad, m_pad, m.sek.vertical + m.sek.noise, m.sek.vertical, q0.get(), mt,
                      part.get(), splits, var, corr.get());

@@ -393,7 +393,8 @@ int gprat_context_set_profiling(gprat_context *ctx, int enable);
 /* Launch count and summed duration (ms) for one kernel family since
  * profiling was enabled. Families: "cholesky" (whole factorisation),
  * "panel" (diagonal blocks), "panel_rows", "syrk_trailing", "assemble",
- * "solve", "trsm", "mean", "refine", "inverse", "gradient", "batch_fit". */
+ * "solve", "trsm", "mean", "refine", "residual" (the FP64 product of the
+ * variance refinement), "inverse", "gradient", "batch_fit". */
 int gprat_context_kernel_stats(gprat_context *ctx, const char *family, long *launches, double *total_ms);
 
 /* Measured ceilings of this device, to report rates against: FP64 and FP32
