@@ -46,35 +46,34 @@ static std::size_t size_class(std::size_t bytes)
 
 void *DevicePool::get(std::size_t bytes)
 {
-    std::lock_guard<std::mutex> g(mu_);
     const std::size_t want = size_class(bytes ? bytes : 1);
-    auto it = free_.lower_bound(want);
-    // reuse a cached block if it is not more than 25% larger than needed
-    if (it != free_.end() && it->first <= want + want / 4)
     {
-        void *p = it->second;
-        live_[p] = it->first;
-        cached_ -= it->first;
-        free_.erase(it);
-        return p;
+        std::lock_guard<std::mutex> g(mu_);
+        auto it = free_.lower_bound(want);
+        // reuse a cached block if it is not more than 25% larger than needed
+        if (it != free_.end() && it->first <= want + want / 4)
+        {
+            void *p = it->second;
+            live_[p] = it->first;
+            cached_ -= it->first;
+            free_.erase(it);
+            return p;
+        }
     }
     void *p = nullptr;
-    hipError_t e = hipMalloc(&p, want);
-    if (e != hipSuccess)
+    if (hipMalloc(&p, want) != hipSuccess)
     {
         (void)hipGetLastError();
-        hipDeviceSynchronize();
-        for (auto &kv : free_)
-            hipFree(kv.second);
-        free_.clear();
-        cached_ = 0;
-        e = hipMalloc(&p, want);
-        if (e != hipSuccess)
+        if (on_out_of_memory)
+            on_out_of_memory();
+        trim(0);
+        if (hipMalloc(&p, want) != hipSuccess)
         {
             (void)hipGetLastError();
             throw Error(GPRAT_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(want) + " bytes failed");
         }
     }
+    std::lock_guard<std::mutex> g(mu_);
     live_[p] = want;
     return p;
 }

@@ -22,6 +22,10 @@ class DevicePool
   public:
     explicit DevicePool(int device) : device_(device) { }
     ~DevicePool();
+    // When a device allocation fails, this is called (without the pool's
+    // lock) before the pool frees its own cache and tries once more: an
+    // owner of several pools can free the others' cached blocks too.
+    std::function<void()> on_out_of_memory;
     void *get(std::size_t bytes);
     void put(void *p);
     void release_cached();

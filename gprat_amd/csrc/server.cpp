@@ -41,13 +41,18 @@ Server::Server(const ServerConfig &cfg) : cfg_(cfg)
     GP_HIP(hipMemGetInfo(&free_b, &total_b));
     if (cfg_.budget == 0)
         cfg_.budget = total_b / 10 * 7;
-    // blocks cached by the slots' pools live outside the budget: together
-    // they may take at most half of the memory the budget leaves free
-    const std::size_t spare = total_b > cfg_.budget ? total_b - cfg_.budget : 0;
-    cache_cap_ = std::max<std::size_t>(std::size_t(256) << 20, spare / (2 * cfg_.slots));
+    // Blocks cached by the slots' pools live outside the budget and are kept
+    // (freeing synchronises the whole device): when an allocation fails,
+    // every slot's cache is freed and the allocation tried again.
     st_.budget = cfg_.budget;
     for (int s = 0; s < cfg_.slots; s++)
         ctx_.push_back(std::make_unique<Context>(cfg_.device, 1));
+    for (auto &c : ctx_)
+        c->pool.on_out_of_memory = [this]
+        {
+            for (auto &o : ctx_)
+                o->pool.trim(0);
+        };
     for (int s = 0; s < cfg_.slots; s++)
         workers_.emplace_back([this, s] { worker(s); });
 }
@@ -494,8 +499,6 @@ void Server::worker(int slot)
         }
         done_cv_.notify_all();
         work_cv_.notify_all();
-        if (c.pool.cached_bytes() > cache_cap_)
-            c.pool.trim(cache_cap_ / 2);
     }
 }
 

@@ -139,7 +139,6 @@ class Server
     std::size_t predict_need(const Entry &e, int m) const;
 
     ServerConfig cfg_;
-    std::size_t cache_cap_ = 0;
     std::vector<std::unique_ptr<Context>> ctx_;
     std::mutex mu_;
     std::condition_variable work_cv_, done_cv_;
