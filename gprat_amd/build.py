@@ -29,7 +29,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function",
     "-I", CSRC, "-I", INCLUDE,
 ]
-KERNEL_SOURCES = ["k_covariance.hip", "k_gemm.hip", "k_gemm32.hip", "k_triangular.hip", "k_batch.hip", "k_calibrate.hip", "k_refine.hip"]
+KERNEL_SOURCES = ["k_covariance.hip", "k_gemm.hip", "k_gemm32.hip", "k_triangular.hip", "k_batch.hip", "k_calibrate.hip", "k_refine.hip", "k_noalloc.hip"]
 HOST_SOURCES = ["engine.cpp", "panel_plan.cpp", "server.cpp", "capi.cpp"]
 
 

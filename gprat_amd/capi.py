@@ -68,6 +68,7 @@ SIGNATURES = {
     "gprat_model_fit": [_VP, _D, _SZ, _D, _SZ, C.c_int, C.c_int, _D, C.c_int, C.POINTER(_VP)],
     "gprat_model_predict": [_VP, _D, _SZ, C.c_int, _D, _D],
     "gprat_model_predict_full_cov": [_VP, _D, _SZ, C.c_int, _D, _D],
+    "gprat_model_predict_mean_noalloc": [_VP, _D, _SZ, C.c_int, _D],
     "gprat_model_loss": [_VP, _D],
     "gprat_model_gradient": [_VP, _I, _D, _D],
     "gprat_model_bytes": [_VP, C.POINTER(_SZ)],
@@ -347,6 +348,14 @@ class Model:
         last = C.c_double(0.0)
         _check(lib().gprat_model_refinement(self.handle, C.byref(it), C.byref(last)))
         return it.value, last.value
+
+    def predict_mean_noalloc(self, test_in, m):
+        """Mean at m test points through the library's fixed device scratch
+        (no device allocation)."""
+        z = _arr(test_in)
+        mean = np.empty(m)
+        _check(lib().gprat_model_predict_mean_noalloc(self.handle, _dptr(z), z.size, m, _dptr(mean)))
+        return mean
 
     def set_variance_refinement(self, enabled: bool):
         """Mixed-precision models: FP64 refinement of the latent variance

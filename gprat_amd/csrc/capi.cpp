@@ -485,6 +485,17 @@ int gprat_model_predict(gprat_model *model, const double *test_in, size_t test_i
         });
 }
 
+int gprat_model_predict_mean_noalloc(gprat_model *model, const double *test_in, size_t test_in_len, int m, double *mean_out)
+{
+    return guarded(
+        [&]
+        {
+            auto &md = model_ref(model);
+            std::lock_guard<std::recursive_mutex> g(md.ctx->mu);
+            gpamd::predict_mean_noalloc(md, test_in, test_in_len, m, mean_out);
+        });
+}
+
 int gprat_model_predict_full_cov(gprat_model *model, const double *test_in, size_t test_in_len, int m, double *mean_out, double *cov_out)
 {
     return guarded(

@@ -150,6 +150,7 @@ Context::Context(int dev, int ns) : device(dev), n_streams(ns), pool(dev)
         GP_HIP(hipStreamCreateWithFlags(&panel, hipStreamNonBlocking));
     info = DevBuf<int>(*this, 1);
     scalars = DevBuf<double>(*this, 64);
+    k::noalloc_scratch_init(device);
 }
 
 Context::~Context()
@@ -1270,6 +1271,39 @@ void predict_full_cov(Model &m, const double *z, std::size_t z_len, int mt, doub
     predict_full_cov_view(FactorView{ m.ctx, m.x.get(), m.n, m.n_pad, m.r, m.lda, m.sek, m.L.get(), m.Winv.get(), m.alpha.get(),
                                       m.precision == GPRAT_PRECISION_MIXED, m.Wpanel.get(), m.wblock },
                           z, z_len, mt, mean, cov);
+}
+
+// Predictive mean through the fixed device scratch (no device allocation):
+// the test series is staged at the scratch's start, then the mean's partial
+// sums and the mean itself, each 256-byte aligned. Every partial sum is
+// written before it is read, so nothing needs clearing. Calls on one GPU
+// share its scratch: they are serialised by the context lock of the model,
+// and calls on different contexts of one GPU must not overlap.
+void predict_mean_noalloc(Model &m, const double *z, std::size_t z_len, int mt, double *mean)
+{
+    Context &c = *m.ctx;
+    require(mt >= 1, "predict_mean_noalloc: need at least one test point");
+    require(z != nullptr && z_len >= static_cast<std::size_t>(mt) + m.r - 1,
+            "predict_mean_noalloc: test input must hold m + n_reg - 1 values");
+    require(mean != nullptr, "predict_mean_noalloc: mean output is null");
+    const std::size_t zl = static_cast<std::size_t>(mt) + m.r - 1;
+    const int splits = k::mean_splits(m.n, mt);
+    auto aligned = [](std::size_t bytes) { return (bytes + 255) / 256 * 256; };
+    const std::size_t off_partial = aligned(zl * sizeof(double));
+    const std::size_t off_mean = off_partial + aligned(static_cast<std::size_t>(splits) * mt * sizeof(double));
+    const std::size_t need = off_mean + aligned(static_cast<std::size_t>(mt) * sizeof(double));
+    if (need > k::noalloc_scratch_bytes())
+        throw Error(GPRAT_ERR_ARGUMENT, "predict_mean_noalloc: " + std::to_string(mt) + " test points need " + std::to_string(need)
+                                            + " bytes of scratch, more than the fixed " + std::to_string(k::noalloc_scratch_bytes()));
+    c.make_current();
+    char *base = static_cast<char *>(k::noalloc_scratch(c.device));
+    double *dz = reinterpret_cast<double *>(base);
+    double *partial = reinterpret_cast<double *>(base + off_partial);
+    double *dmean = reinterpret_cast<double *>(base + off_mean);
+    GP_HIP(hipMemcpyAsync(dz, z, zl * sizeof(double), hipMemcpyDefault, c.main));
+    k::predict_mean(c.main, m.x.get(), dz, m.alpha.get(), m.r, m.sek, m.n, mt, partial, splits, dmean);
+    GP_HIP(hipMemcpyAsync(mean, dmean, sizeof(double) * mt, hipMemcpyDefault, c.main));
+    GP_HIP(hipStreamSynchronize(c.main));
 }
 
 FactorView view(const Model &m)

@@ -226,6 +226,8 @@ void fit(Context &c,
          int precision);
 void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean, double *var);
 void predict_view(const FactorView &m, const double *z, std::size_t z_len, int mt, double *mean, double *var);
+// Predictive mean through the fixed per-GPU scratch: no device allocation.
+void predict_mean_noalloc(Model &m, const double *z, std::size_t z_len, int mt, double *mean);
 // The view of a resident model (its context; no variance_correction output).
 FactorView view(const Model &m);
 // Mean and the whole mt x mt latent posterior covariance (row-major).

@@ -311,6 +311,16 @@ int quad_form_splits(int rows, int m);
 void quad_form(hipStream_t s, const double *B, const double *X, const double *Z, long ld, int rows, int m, double d, double v, const double *q0,
                int mt, double *part, int splits, double *var, double *corr);
 
+// ---- fixed scratch of the no-allocation entry point (k_noalloc.hip) ------
+
+// Bytes of the per-GPU scratch array.
+std::size_t noalloc_scratch_bytes();
+// Looks up the array's device address on `device` (once per device; done
+// when a context is created there).
+void noalloc_scratch_init(int device);
+// The array's device address on `device` (after noalloc_scratch_init).
+void *noalloc_scratch(int device);
+
 // ---- calibration (k_calibrate.hip) ---------------------------------------
 
 // Back-to-back MFMAs from registers on `workgroups` x 256 threads (FP64

@@ -245,6 +245,18 @@ int gprat_model_predict(gprat_model *model,
                         double *mean_out,
                         double *var_out);
 
+/* Predictive mean at m test points, like gprat_model_predict with
+ * var_out == NULL, without any device allocation: the test series, the
+ * mean's partial sums and the staged mean live in a fixed 8 MiB scratch
+ * array that each GPU holds for the life of the process (m up to about
+ * 340000 for large models; larger m fails with GPRAT_ERR_ARGUMENT). No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation). The scratch is shared by every call on one GPU: calls
+ * on models of one context are serialised, calls through different contexts
+ * on the same GPU must not overlap (they would race on the scratch). */
+int gprat_model_predict_mean_noalloc(gprat_model *model, const double *test_in, size_t test_in_len, int m, double *mean_out);
+
 /* Mean and the whole m x m latent posterior covariance (row-major, m * m
  * doubles) at m test points; see gprat_predict_full_posterior. */
 int gprat_model_predict_full_cov(gprat_model *model,
