@@ -91,6 +91,27 @@ def residual_flops(n_pad: int, m_pad: int) -> float:
     return 2.0 * m_pad * 128 * 128 * T * (T - 1) / 2.0
 
 
+def variance_update_flops(n_pad: int, m_pad: int, nbk: int, directions: int):
+    """Algorithmic flops of the variance solves' update products over one
+    request: step J of the forward solve updates the n_pad - j1 rows below
+    its block, step J of the backward solve the j0 rows above it, each a
+    (rows x m_pad) += (rows x nb) (nb x m_pad) product; returns (total flops,
+    launches)."""
+    total, launches = 0.0, 0
+    j0 = 0
+    while j0 < n_pad:
+        nb = min(nbk, n_pad - j0)
+        j1 = j0 + nb
+        if j1 < n_pad:
+            total += 2.0 * (n_pad - j1) * nb * m_pad
+            launches += 1
+        if directions == 2 and j0 > 0:
+            total += 2.0 * j0 * nb * m_pad
+            launches += 1
+        j0 = j1
+    return total, launches
+
+
 def measured_traffic(name):
     """HBM bytes per launch of a kernel from a committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by what the
@@ -554,7 +575,8 @@ def main():
     from gprat_amd import capi
 
     ctx = capi.Context(0, 1)
-    families = ("syrk_trailing", "residual", "trsm", "panel", "panel_rows", "cholesky", "assemble", "solve", "mean", "refine")
+    families = ("syrk_trailing", "residual", "trsm", "trsm_update", "panel", "panel_rows", "cholesky", "assemble", "solve", "mean",
+                "refine")
     head, fams, model, (mean, var, *_r) = request_leg(capi, ctx, HN, HT, HR, HM, HTHETA, capi.PRECISION_MIXED, args.steps,
                                                       warmup=args.warmup, families=families)
     n_pad = ((HN + 127) // 128) * 128
@@ -565,7 +587,10 @@ def main():
                          "trailing_update_f32_n32768_traffic.md")
     r_res = roofline_of("k_gemm_f64<true,false,8> (FP64 residual product K_sl X of the variance refinement)",
                         residual_flops(n_pad, m_pad), fams["residual"], FP64_MFMA_PEAK_TFLOPS, "residual_n32768_traffic.md")
-    ranked = sorted([r for r in (r_fp32, r_res) if r], key=lambda r: -r["launches"] * r["avg_launch_ms"])
+    upd_flops, upd_launches = variance_update_flops(n_pad, m_pad, outer_block(n_pad), 2)
+    r_upd = roofline_of("k_gemm_f32_tn<float,2> (FP32 matrix-core updates of the variance's forward and backward solves)",
+                        upd_flops / upd_launches, fams["trsm_update"], FP32_MFMA_PEAK_TFLOPS, "trsm_update_f32_n32768_traffic.md")
+    ranked = sorted([r for r in (r_fp32, r_res, r_upd) if r], key=lambda r: -r["launches"] * r["avg_launch_ms"])
     roofline = ranked[0] if ranked else None
     loss = model.loss()
     _, grad = model.gradient()
@@ -601,7 +626,7 @@ def main():
         "p99_ms": head["p99_ms"],
         "headline": head,
         "roofline": roofline,
-        "roofline_other": ranked[1] if len(ranked) > 1 else None,
+        "roofline_other": ranked[1:],
     }
     if not args.headline_only:
         f64, _, mdl, (m64, v64, *_r) = request_leg(capi, ctx, HN, HT, HR, HM, HTHETA, capi.PRECISION_FP64, 3, warmup=1)

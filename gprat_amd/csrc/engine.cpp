@@ -802,7 +802,10 @@ static void blocked_solve(Context &c, const double *L, long lda, const typename 
             break;
         if (side != s)
             GP_HIP(hipStreamWaitEvent(s, ready, 0));
-        Ops::update(s, P[J % 2].get(), n_pad, V.get(), m_pad, B + static_cast<long>(j1) * m_pad, m_pad, n_pad - j1, m_pad, nb);
+        {
+            Timed t(c, s, "trsm_update");
+            Ops::update(s, P[J % 2].get(), n_pad, V.get(), m_pad, B + static_cast<long>(j1) * m_pad, m_pad, n_pad - j1, m_pad, nb);
+        }
         freed[J % 2] = record(s);
         ready = next;
     }
@@ -898,7 +901,10 @@ static void backward_solve_f32(Context &c, const double *L, long lda, const floa
             break;
         if (side != s)
             GP_HIP(hipStreamWaitEvent(s, ready, 0));
-        k::gemm_f32_tn(s, R[J % 2].get(), n_pad, T.get(), m_pad, B, m_pad, j0, m_pad, nb, false);
+        {
+            Timed t(c, s, "trsm_update");
+            k::gemm_f32_tn(s, R[J % 2].get(), n_pad, T.get(), m_pad, B, m_pad, j0, m_pad, nb, false);
+        }
         freed[J % 2] = record(s);
         ready = next;
     }

@@ -405,7 +405,8 @@ int gprat_context_set_profiling(gprat_context *ctx, int enable);
 /* Launch count and summed duration (ms) for one kernel family since
  * profiling was enabled. Families: "cholesky" (whole factorisation),
  * "panel" (diagonal blocks), "panel_rows", "syrk_trailing", "assemble",
- * "solve", "trsm", "mean", "refine", "residual" (the FP64 product of the
+ * "solve", "trsm" (variance solves, whole), "trsm_update" (their update
+ * products only), "mean", "refine", "residual" (the FP64 product of the
  * variance refinement), "inverse", "gradient", "batch_fit". */
 int gprat_context_kernel_stats(gprat_context *ctx, const char *family, long *launches, double *total_ms);
 
