@@ -138,6 +138,11 @@ struct BlasTable
         gemv;
     std::function<double(long, const double *, long, const double *, long)> ddot;
     std::function<void(long, double, const double *, long, double *, long)> daxpy;
+    // single precision (core/src/cpu/adapter_cblas_fp32.cpp:14-100)
+    std::function<void(int, char, long, float *, long)> spotrf2;
+    std::function<void(int, int, int, int, int, long, long, float, const float *, long, float *, long)> strsm;
+    std::function<void(int, int, int, long, long, float, const float *, long, float, float *, long)> ssyrk;
+    std::function<void(int, int, int, long, long, long, float, const float *, long, const float *, long, float, float *, long)> sgemm;
 };
 
 template <class I>
@@ -181,6 +186,18 @@ BlasTable bind_blas(void *h, const std::string &name, const char *pre, const cha
     auto daxpy = reinterpret_cast<void (*)(I, double, const double *, I, double *, I)>(sym("cblas_daxpy"));
     t.daxpy = [daxpy](long n, double a, const double *x, long ix, double *y, long iy)
     { daxpy(I(n), a, x, I(ix), y, I(iy)); };
+    auto spotrf = reinterpret_cast<I (*)(int, char, I, float *, I)>(sym("LAPACKE_spotrf2"));
+    t.spotrf2 = [spotrf](int lay, char up, long n, float *a, long lda) { spotrf(lay, up, I(n), a, I(lda)); };
+    auto strsm = reinterpret_cast<void (*)(int, int, int, int, int, I, I, float, const float *, I, float *, I)>(sym("cblas_strsm"));
+    t.strsm = [strsm](int o, int s, int u, int tr, int d, long m, long n, float al, const float *a, long lda, float *b, long ldb)
+    { strsm(o, s, u, tr, d, I(m), I(n), al, a, I(lda), b, I(ldb)); };
+    auto ssyrk = reinterpret_cast<void (*)(int, int, int, I, I, float, const float *, I, float, float *, I)>(sym("cblas_ssyrk"));
+    t.ssyrk = [ssyrk](int o, int u, int tr, long n, long k, float al, const float *a, long lda, float be, float *c, long ldc)
+    { ssyrk(o, u, tr, I(n), I(k), al, a, I(lda), be, c, I(ldc)); };
+    auto sgemm = reinterpret_cast<void (*)(int, int, int, I, I, I, float, const float *, I, const float *, I, float, float *, I)>(
+        sym("cblas_sgemm"));
+    t.sgemm = [sgemm](int o, int ta, int tb, long m, long n, long k, float al, const float *a, long lda, const float *b, long ldb,
+                      float be, float *c, long ldc) { sgemm(o, ta, tb, I(m), I(n), I(k), al, a, I(lda), b, I(ldb), be, c, I(ldc)); };
     return t;
 }
 
@@ -716,6 +733,73 @@ Tile op_axpy(const Tile &y, const Tile &x, int N)
 // :190-194
 double op_dot(const Vec &a, const Vec &b, int N) { return blas().ddot(N, a.data(), 1, b.data(), 1); }
 
+// ---------------------------------------------------------------------------
+// FP32 tile ops: core/src/cpu/adapter_cblas_fp32.cpp:14-100, the same calls
+// as the FP64 adapter on float tiles (LAPACKE_spotrf2, cblas_strsm/ssyrk/
+// sgemm). The reference keeps this adapter beside the FP64 one without
+// wiring it into a GP function; here it is the CPU comparator of the mixed-
+// precision path. Tiles stay FP64 vectors between ops: each op rounds its
+// inputs to float, calls the float BLAS and widens the result.
+
+std::vector<float> to32(const Vec &a) { return std::vector<float>(a.begin(), a.end()); }
+Vec from32(const std::vector<float> &a) { return Vec(a.begin(), a.end()); }
+
+// :14-22
+Tile op32_potrf(const Tile &A, int N)
+{
+    auto a = to32(*A);
+    blas().spotrf2(kRowMajor, 'L', N, a.data(), N);
+    return make_tile(from32(a));
+}
+
+// :24-52
+Tile op32_trsm(const Tile &L, const Tile &A, int N, int M, int trans, int side)
+{
+    auto l = to32(*L), a = to32(*A);
+    blas().strsm(kRowMajor, side, kLower, trans, kNonUnit, N, M, 1.0f, l.data(), N, a.data(), M);
+    return make_tile(from32(a));
+}
+
+// :54-65: A = A - B B^T (lower), all in float
+Tile op32_syrk(const Tile &A, const Tile &B, int N)
+{
+    auto a = to32(*A), b = to32(*B);
+    blas().ssyrk(kRowMajor, kLower, kNoTrans, N, N, -1.0f, b.data(), N, 1.0f, a.data(), N);
+    return make_tile(from32(a));
+}
+
+// :67-100: C = C - op(A) op(B), all in float
+Tile op32_gemm(const Tile &A, const Tile &B, const Tile &C, int N, int M, int K, int tA, int tB)
+{
+    auto a = to32(*A), b = to32(*B), c = to32(*C);
+    blas().sgemm(kRowMajor, tA, tB, K, M, N, -1.0f, a.data(), K, b.data(), M, 1.0f, c.data(), M);
+    return make_tile(from32(c));
+}
+
+// The mixed-precision trailing update: the FP32 adapter's product of float
+// copies (FP32 sums) subtracted from the FP64 tile.
+Tile op_syrk_f32prod(const Tile &A, const Tile &B, int N)
+{
+    auto b = to32(*B);
+    std::vector<float> c(static_cast<std::size_t>(N) * N, 0.0f);
+    blas().ssyrk(kRowMajor, kLower, kNoTrans, N, N, 1.0f, b.data(), N, 0.0f, c.data(), N);
+    Vec a = *A;
+    for (std::size_t i = 0; i < a.size(); i++)
+        a[i] -= static_cast<double>(c[i]);
+    return make_tile(std::move(a));
+}
+
+Tile op_gemm_f32prod(const Tile &A, const Tile &B, const Tile &C, int N, int M, int K, int tA, int tB)
+{
+    auto a = to32(*A), b = to32(*B);
+    std::vector<float> p(static_cast<std::size_t>(K) * M, 0.0f);
+    blas().sgemm(kRowMajor, tA, tB, K, M, N, 1.0f, a.data(), K, b.data(), M, 0.0f, p.data(), M);
+    Vec c = *C;
+    for (std::size_t i = 0; i < c.size(); i++)
+        c[i] -= static_cast<double>(p[i]);
+    return make_tile(std::move(c));
+}
+
 // core/src/cpu/gp_uncertainty.cpp:6-19
 Tile op_diagonal(const Tile &A, std::size_t M)
 {
@@ -772,6 +856,37 @@ void cholesky_tiled(Tiles &A, int N, std::size_t T)
                                      A[m * T + k],
                                      A[n * T + k],
                                      A[m * T + n]);
+        }
+    }
+}
+
+// The DAG of cholesky_tiled with reduced-precision tile ops: level 1 (the
+// mixed-precision comparator) computes the SYRK/GEMM trailing products in
+// FP32 and subtracts them from the FP64 tiles; level 2 runs all four ops
+// through the FP32 adapter.
+void cholesky_tiled_reduced(Tiles &A, int N, std::size_t T, int level)
+{
+    const bool all32 = level >= 2;
+    for (std::size_t k = 0; k < T; k++)
+    {
+        A[k * T + k] = after([N, all32](const Tile &a) { return all32 ? op32_potrf(a, N) : op_potrf(a, N); }, A[k * T + k]);
+        for (std::size_t m = k + 1; m < T; m++)
+            A[m * T + k] = after([N, all32](const Tile &l, const Tile &a)
+                                 { return all32 ? op32_trsm(l, a, N, N, kTrans, kRight) : op_trsm(l, a, N, N, kTrans, kRight); },
+                                 A[k * T + k],
+                                 A[m * T + k]);
+        for (std::size_t m = k + 1; m < T; m++)
+        {
+            A[m * T + m] = after([N, all32](const Tile &a, const Tile &b) { return all32 ? op32_syrk(a, b, N) : op_syrk_f32prod(a, b, N); },
+                                 A[m * T + m],
+                                 A[m * T + k]);
+            for (std::size_t n = k + 1; n < m; n++)
+                A[m * T + n] = after(
+                    [N, all32](const Tile &a, const Tile &b, const Tile &c)
+                    { return all32 ? op32_gemm(a, b, c, N, N, N, kNoTrans, kTrans) : op_gemm_f32prod(a, b, c, N, N, N, kNoTrans, kTrans); },
+                    A[m * T + k],
+                    A[n * T + k],
+                    A[m * T + n]);
         }
     }
 }
@@ -1215,6 +1330,98 @@ double compute_loss(const Vec &x, const Vec &y, const SEKParams &sek, int T_, in
     forward_solve(K, alpha, N, T);
     backward_solve(K, alpha, N, T);
     return loss_tiled(K, alpha, yt, N, T).get();
+}
+
+// alpha = K~^-1 y from a reduced-precision factor, refined in FP64: r = y -
+// K alpha with the FP64 covariance tiles (lower tiles and their transposes),
+// delta from the factor, alpha += delta while the corrections shrink, until
+// a correction is at most 1e-12 max|alpha|. Returns the steps taken.
+int refine_alpha_reduced(Tiles &L, Tiles &K, Tiles &alpha, const Tiles &y, int N, std::size_t T)
+{
+    double prev = 1e300;
+    int steps = 0;
+    for (int it = 0; it < 30; it++)
+    {
+        Tiles r;
+        for (std::size_t i = 0; i < T; i++)
+            r.push_back(y[i]);
+        for (std::size_t i = 0; i < T; i++)
+            for (std::size_t j = 0; j < T; j++)
+            {
+                if (j <= i)
+                    r[i] = after([N](const Tile &a, const Tile &x, const Tile &b) { return op_gemv(a, x, b, N, N, -1.0, kNoTrans); },
+                                 K[i * T + j], alpha[j], r[i]);
+                else
+                    r[i] = after([N](const Tile &a, const Tile &x, const Tile &b) { return op_gemv(a, x, b, N, N, -1.0, kTrans); },
+                                 K[j * T + i], alpha[j], r[i]);
+            }
+        forward_solve(L, r, N, T);
+        backward_solve(L, r, N, T);
+        double dmax = 0.0, amax = 0.0;
+        for (std::size_t i = 0; i < T; i++)
+        {
+            const Vec &d = *r[i].get();
+            const Vec &a = *alpha[i].get();
+            for (std::size_t e = 0; e < d.size(); e++)
+            {
+                dmax = std::max(dmax, std::fabs(d[e]));
+                amax = std::max(amax, std::fabs(a[e]));
+            }
+        }
+        if (it > 0 && dmax >= prev)
+            break;
+        for (std::size_t i = 0; i < T; i++)
+        {
+            Vec a = *alpha[i].get();
+            const Vec &d = *r[i].get();
+            for (std::size_t e = 0; e < a.size(); e++)
+                a[e] += d[e];
+            alpha[i] = Flow<Tile>::now(make_tile(std::move(a)));
+        }
+        steps = it + 1;
+        if (dmax <= 1e-12 * amax || !(dmax < 0.5 * prev))
+            break;
+        prev = dmax;
+    }
+    return steps;
+}
+
+Reduced predict_with_uncertainty_reduced(const Vec &x, const Vec &y, const Vec &z, const SEKParams &sek, int T_, int N, int MT_, int M,
+                                         int r, int level)
+{
+    const std::size_t T = static_cast<std::size_t>(T_), MT = static_cast<std::size_t>(MT_);
+    check_input(x, T * static_cast<std::size_t>(N), r, "training input");
+    check_input(z, MT * static_cast<std::size_t>(M), r, "test input");
+    Tiles K = assemble_K(x, sek, T, N, r);
+    Tiles K64 = K;  // tiles are immutable: the factorisation replaces K's entries
+    Tiles yt = assemble_y(y, T, N);
+    Tiles alpha = assemble_y(y, T, N);
+    Tiles cross = assemble_cross(z, x, sek, MT, T, M, N, r);
+    Tiles pred = zeros(MT, M);
+    Tiles prior;
+    for (std::size_t i = 0; i < MT; i++)
+        prior.push_back(spawn([=, &z] { return tile_prior_diag(i, i, M, r, sek, z); }));
+    Tiles crossT;
+    for (std::size_t j = 0; j < T; j++)
+        for (std::size_t i = 0; i < MT; i++)
+            crossT.push_back(after([M, N](const Tile &c) { return make_tile(tile_transpose(M, N, *c)); }, cross[i * T + j]));
+    Tiles unc = zeros(MT, M);
+    if (level <= 0)
+        cholesky_tiled(K, N, T);
+    else
+        cholesky_tiled_reduced(K, N, T, level);
+    forward_solve(K, alpha, N, T);
+    backward_solve(K, alpha, N, T);
+    Reduced out;
+    out.refine_steps = refine_alpha_reduced(K, K64, alpha, yt, N, T);
+    matrix_vector(cross, alpha, pred, M, N, T, MT);
+    forward_solve_matrix(K, crossT, N, M, T, MT);
+    sym_matrix_matrix_diag(crossT, unc, N, M, T, MT);
+    vector_difference(prior, unc, M, MT);
+    out.mean = flatten(pred);
+    out.var = flatten(unc);
+    out.loss = loss_tiled(K, alpha, yt, N, T).get();
+    return out;
 }
 
 std::vector<double> optimize(const Vec &x,

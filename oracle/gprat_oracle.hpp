@@ -128,6 +128,31 @@ double optimize_step(const std::vector<double> &training_input,
                      const std::vector<bool> &trainable,
                      int iter);
 
+// FP32 comparator of the mixed-precision path: predict_with_uncertainty's
+// DAG with a reduced-precision Cholesky, level 1 = SYRK/GEMM trailing
+// products through the FP32 adapter (core/src/cpu/adapter_cblas_fp32.cpp:54-100)
+// subtracted from FP64 tiles, level 2 = every factor op through the FP32
+// adapter (:14-100); alpha refined in FP64 against K until the correction
+// is at most 1e-12 max|alpha|; the variance solve, against the reduced
+// factor, is not refined; the loss uses the reduced factor's diagonal.
+// Level 0 is the FP64 path plus the refinement.
+struct Reduced
+{
+    std::vector<double> mean, var;
+    double loss = 0.0;
+    int refine_steps = 0;
+};
+Reduced predict_with_uncertainty_reduced(const std::vector<double> &training_input,
+                                         const std::vector<double> &training_output,
+                                         const std::vector<double> &test_input,
+                                         const SEKParams &sek,
+                                         int n_tiles,
+                                         int n_tile_size,
+                                         int m_tiles,
+                                         int m_tile_size,
+                                         int n_regressors,
+                                         int level);
+
 // Scalar pieces exposed for closed-form tests (core/src/cpu/gp_optimizer.cpp:12-36).
 double to_constrained(double parameter, bool noise);
 double to_unconstrained(double parameter, bool noise);

@@ -67,6 +67,10 @@ class GP
         return oracle::optimize_step(in_, out_, n_tiles_, n_tile_size_, n_reg, a, sek, trainable_, iter);
     }
     double calculate_loss() { return oracle::compute_loss(in_, out_, sek, n_tiles_, n_tile_size_, n_reg); }
+    oracle::Reduced predict_reduced(const std::vector<double> &z, int mt, int ms, int level)
+    {
+        return oracle::predict_with_uncertainty_reduced(in_, out_, z, sek, n_tiles_, n_tile_size_, mt, ms, n_reg, level);
+    }
     oracle::Matrix cholesky() { return oracle::cholesky(in_, sek, n_tiles_, n_tile_size_, n_reg); }
 
     std::vector<double> in_, out_;
@@ -122,6 +126,17 @@ PYBIND11_MODULE(gprat_oracle, m)
         .def("optimize", &GP::optimize, py::arg("AdamParams"))
         .def("optimize_step", &GP::optimize_step, py::arg("AdamParams"), py::arg("iter"))
         .def("compute_loss", &GP::calculate_loss)
+        .def(
+            "predict_with_uncertainty_reduced",
+            [](GP &g, const std::vector<double> &z, int mt, int ms, int level)
+            {
+                oracle::Reduced r = g.predict_reduced(z, mt, ms, level);
+                return py::make_tuple(r.mean, r.var, r.loss, r.refine_steps);
+            },
+            py::arg("test_data"),
+            py::arg("m_tiles"),
+            py::arg("m_tile_size"),
+            py::arg("level"))
         .def("cholesky", &GP::cholesky)
         .def_property_readonly("hyperparameters",
                                [](const GP &g)

@@ -14,6 +14,12 @@ needs minutes of host BLAS):
     factor, unrefined; the loss is normalised per training point and sits
     near zero, so its relative error is inflated by cancellation);
   * gradient: 1e-5 relative (K^-1 from the mixed-precision factor).
+Where the oracle runs live, the oracle's FP32 comparator (the same precision
+scheme on the CPU: FP32 trailing products, alpha refined in FP64, variance
+unrefined) separates precision error from GPU error: the GPU's refined
+variance must beat the comparator's unrefined one, and the GPU's loss error
+(the mixed-precision factor's log-determinant) must be of the comparator's
+size (within 10x, or below 1e-9 absolute).
 At n = 65536 the oracle is out of reach on the host (17 GB of tiles and
 minutes per factorisation); there the FP64 GPU path, itself pinned to the
 oracle, is the reference for mean, variance and loss.
@@ -75,9 +81,15 @@ def test_mixed_vs_oracle(ctx, oracle, T, b, r, m, noise):
     om, ov = go.predict_with_uncertainty(list(z), mt, mb)
     ol = go.compute_loss()
     og = _oracle_grad(oracle, x, y, T, b, r, theta)
+    cm, cv, cl, csteps = go.predict_with_uncertainty_reduced(list(z), mt, mb, 1)
+    ecv, ecl = norm_rel(cv, ov), abs(cl - ol)
+    print(f"n={n} r={r} CPU FP32 comparator: mean {norm_rel(cm, om):.2e} var {ecv:.2e} loss abs {ecl:.2e} ({csteps} steps)")
     mix = capi.Model(ctx, x, y, n, r, theta, capi.PRECISION_MIXED)
     try:
         _check(mix, z, m, om, ov, ol, og, f"n={n} r={r}")
+        _, vmx = mix.predict(z, m)
+        assert norm_rel(vmx, ov) < ecv
+        assert abs(mix.loss() - ol) <= max(10 * ecl, 1e-9)
     finally:
         mix.release()
 
