@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-I", CSRC, "-I", INCLUDE,
 ]
 KERNEL_SOURCES = ["k_covariance.hip", "k_gemm.hip", "k_gemm32.hip", "k_triangular.hip", "k_batch.hip", "k_calibrate.hip", "k_refine.hip", "k_noalloc.hip"]
-HOST_SOURCES = ["engine.cpp", "panel_plan.cpp", "server.cpp", "capi.cpp"]
+HOST_SOURCES = ["engine.cpp", "batch.cpp", "panel_plan.cpp", "server.cpp", "capi.cpp"]
 
 
 def _run(cmd, verbose):

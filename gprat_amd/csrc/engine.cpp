@@ -135,6 +135,7 @@ void DevBuf<T>::reset()
 template class DevBuf<double>;
 template class DevBuf<float>;
 template class DevBuf<int>;
+template class DevBuf<int2>;
 
 // ---------------------------------------------------------------------------
 // Context
@@ -165,6 +166,7 @@ Context::~Context()
     for (auto e : spare_events)
         hipEventDestroy(e);
     panel_plans.clear();
+    batch_graphs.clear();
     info.reset();
     scalars.reset();
     if (trail && trail != main)
@@ -239,26 +241,7 @@ void Context::sync()
 }
 
 // Optional timing bracket around a launch sequence.
-class Timed
-{
-  public:
-    Timed(Context &c, hipStream_t s, const char *family) : c_(c), s_(s), on_(c.profiling)
-    {
-        if (on_)
-            c_.prof_begin(s_, family, p_);
-    }
-    ~Timed()
-    {
-        if (on_)
-            c_.prof_end(s_, p_);
-    }
 
-  private:
-    Context &c_;
-    hipStream_t s_;
-    bool on_;
-    Context::Pending p_;
-};
 
 // ---------------------------------------------------------------------------
 // Layout
@@ -1335,46 +1318,6 @@ void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean,
 // ---------------------------------------------------------------------------
 // Batched fit of many GPs
 
-// One persistent launch over GPs whose device buffers are already filled
-// (series and zero-padded targets); losses[g] receives GP g's loss and the
-// result's entry g the first non-positive pivot column of GP g (INT_MAX if
-// its matrix was positive definite).
-std::vector<int> batch_run(Context &c, const std::vector<k::BatchGP> &gps, double *losses_host, bool mixed, int *refine_host)
-{
-    const int count = static_cast<int>(gps.size());
-    require(count >= 1, "batch: need at least one GP");
-    c.make_current();
-    // largest GPs first, so the persistent workgroups finish together
-    std::vector<int> order(count);
-    for (int g = 0; g < count; g++)
-        order[g] = g;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int z) { return gps[a].n > gps[z].n; });
-    DevBuf<double> desc(c, (sizeof(k::BatchGP) * count + 7) / 8);
-    DevBuf<int> dorder(c, count), info(c, count), counter(c, 1);
-    DevBuf<double> losses(c, count);
-    DevBuf<int> steps(c, count);
-    if (mixed)
-        for (const auto &q : gps)
-            require(q.work != nullptr, "batch: mixed precision needs a work vector per GP");
-    GP_HIP(hipMemcpyAsync(desc.get(), gps.data(), sizeof(k::BatchGP) * count, hipMemcpyDefault, c.main));
-    GP_HIP(hipMemcpyAsync(dorder.get(), order.data(), sizeof(int) * count, hipMemcpyDefault, c.main));
-    hipDeviceProp_t prop;
-    GP_HIP(hipGetDeviceProperties(&prop, c.device));
-    const int workgroups = std::max(1, std::min(count, prop.multiProcessorCount));
-    {
-        Timed t(c, c.main, "batch_fit");
-        k::batch_fit(c.main, reinterpret_cast<const k::BatchGP *>(desc.get()), dorder.get(), count, counter.get(), losses.get(),
-                     info.get(), workgroups, mixed, mixed ? steps.get() : nullptr);
-    }
-    if (mixed && refine_host)
-        GP_HIP(hipMemcpyAsync(refine_host, steps.get(), sizeof(int) * count, hipMemcpyDefault, c.main));
-    std::vector<int> hinfo(count);
-    GP_HIP(hipMemcpyAsync(hinfo.data(), info.get(), sizeof(int) * count, hipMemcpyDefault, c.main));
-    GP_HIP(hipMemcpyAsync(losses_host, losses.get(), sizeof(double) * count, hipMemcpyDefault, c.main));
-    c.sync();
-    return hinfo;
-}
-
 static void check_desc(const gprat_gp_desc &d, int g)
 {
     require(d.n >= 1 && d.n_reg >= 1, "batch: GP " + std::to_string(g) + ": n and n_reg must be >= 1");
@@ -1451,7 +1394,7 @@ void batch_fit(Context &c, Batch &b, int count, const gprat_gp_desc *gps, int pr
     GP_HIP(hipMemcpyAsync(base, staging.data(), staging.size() * sizeof(double), hipMemcpyDefault, c.main));
     b.losses.assign(count, 0.0);
     b.refine_steps.assign(count, 0);
-    const std::vector<int> info = batch_run(c, b.gps, b.losses.data(), mixed, b.refine_steps.data());
+    const std::vector<int> info = batch_run(c, b.gps, b.losses.data(), mixed, b.refine_steps.data(), &b.gp_mixed);
     for (int g = 0; g < count; g++)
         if (info[g] != INT_MAX)
             throw Error(GPRAT_ERR_NOT_SPD, "batch: GP " + std::to_string(g)
@@ -1465,8 +1408,9 @@ void batch_predict(Batch &b, int index, const double *z, std::size_t z_len, int 
 {
     require(index >= 0 && index < b.count, "batch: GP index out of range");
     const k::BatchGP &q = b.gps[index];
+    const bool mixed = b.precision == GPRAT_PRECISION_MIXED && (b.gp_mixed.empty() || b.gp_mixed[index]);
     predict_view(FactorView{ b.ctx, q.x, q.n, q.n_pad, q.n_reg, q.n_pad, SEK{ q.lengthscale, q.vertical, q.noise }, q.L, q.W, q.alpha,
-                             b.precision == GPRAT_PRECISION_MIXED },
+                             mixed },
                  z, z_len, mt, mean, var);
 }
 

@@ -5,6 +5,7 @@
 
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -46,6 +47,7 @@ class DevicePool
 };
 
 struct Context;
+struct BatchGraph;  // batch.cpp
 
 // RAII handle on a pooled device block.
 template <class T>
@@ -130,9 +132,39 @@ struct Context
     const PanelPlanDev &head_plan(int nbh, int nb, int wgs);
     const PanelPlanDev &cached_plan(long key, const std::function<PanelPlan()> &build);
 
+    // captured level-path launch sequences of the batched fit, by batch
+    // composition (batch.cpp), and how often each composition was seen
+    std::map<std::string, std::shared_ptr<BatchGraph>> batch_graphs;
+    std::map<std::string, int> batch_seen;
+    unsigned long long batch_clock = 0;
+
     void make_current() const;
     void sync();
 };
+
+// Brackets the launches of one kernel family with HIP events on their
+// stream when the context profiles.
+class Timed
+{
+  public:
+    Timed(Context &c, hipStream_t s, const char *family) : c_(c), s_(s), on_(c.profiling)
+    {
+        if (on_)
+            c_.prof_begin(s_, family, p_);
+    }
+    ~Timed()
+    {
+        if (on_)
+            c_.prof_end(s_, p_);
+    }
+
+  private:
+    Context &c_;
+    hipStream_t s_;
+    bool on_;
+    Context::Pending p_;
+};
+
 
 // Resident fitted model.
 struct Model
@@ -194,6 +226,7 @@ struct Batch
     std::vector<double> losses;
     int precision = GPRAT_PRECISION_FP64;
     std::vector<int> refine_steps;  // mixed precision: refinement steps per GP
+    std::vector<char> gp_mixed;     // per GP: fitted in mixed precision (a mixed batch may run some size classes in FP64)
     std::size_t bytes() const { return arena.bytes(); }
 };
 
@@ -235,8 +268,14 @@ void predict_full_cov(Model &m, const double *z, std::size_t z_len, int mt, doub
 void predict_full_cov_view(const FactorView &m, const double *z, std::size_t z_len, int mt, double *mean, double *cov);
 // mixed: FP32 block products and FP64 refinement of alpha (each GP needs
 // its `work` vector); refine_host (count ints) then gets the steps per GP.
+// Routes each GP to the persistent kernel or the level-synchronous path and
+// to FP64 or mixed precision (mixed only if requested and measured faster
+// for its size class), per the autotuned table; mixed_out receives the
+// per-GP precision decision.
 std::vector<int> batch_run(Context &c, const std::vector<k::BatchGP> &gps, double *losses_host, bool mixed = false,
-                           int *refine_host = nullptr);
+                           int *refine_host = nullptr, std::vector<char> *mixed_out = nullptr);
+// Where the batched fit's tuning table came from (a file or the defaults).
+std::string batch_tuning_source(int device);
 void batch_fit(Context &c, Batch &b, int count, const gprat_gp_desc *gps, int precision);
 void batch_losses(Batch &b, double *losses);
 void batch_predict(Batch &b, int index, const double *z, std::size_t z_len, int mt, double *mean, double *var);

@@ -348,6 +348,284 @@ __device__ __forceinline__ void residual_rows(const BatchGP &g, double scale, co
     }
 }
 
+// Rows [r0, r1) (whole 64-row blocks) of one GP's lower band: K on and
+// below the diagonal blocks, identity padding; wavefronts take rows, lanes
+// take columns. For window lengths R = 4, 8, 16 a lane keeps its column's
+// window in registers and the row's window (the row is wavefront-uniform)
+// arrives through scalar loads; the same operations in the same order as
+// window_sqdist (no contraction), so every entry matches the generic loop.
+template <int R>
+__device__ __forceinline__ void assemble_rows_w(const BatchGP &g, long r0, long r1)
+{
+#pragma clang fp contract(off)
+    const double scale = -0.5 / (g.lengthscale * g.lengthscale);
+    const long np = g.n_pad;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const double *x = g.x;
+    for (long j0 = 0; j0 < r1; j0 += 64)
+    {
+        const long j = j0 + lane;
+        const long jc = j < g.n ? j : g.n - 1;  // lanes past the end read a valid window and store the padding value
+        double xj[R];
+#pragma unroll
+        for (int k = 0; k < R; k++)
+            xj[k] = x[jc + k];
+        // rows of this 64-block column that lie in [r0, r1) and on or below its diagonal block
+        const long rs = r0 > j0 ? r0 : j0;
+        for (long i = rs + wv; i < r1; i += THREADS / 64)
+        {
+            double val;
+            if (i < g.n && j < g.n)
+            {
+                double d = 0.0;
+#pragma unroll
+                for (int k = 0; k < R; k++)
+                {
+                    const double t = x[i + k] - xj[k];
+                    d += t * t;
+                }
+                val = g.vertical * exp(scale * d);
+                if (i == j)
+                    val += g.noise;
+            }
+            else
+                val = (i == j) ? 1.0 : 0.0;
+            g.L[i * np + j] = val;
+        }
+    }
+}
+
+__device__ __forceinline__ void assemble_rows(const BatchGP &g, long r0, long r1)
+{
+    if (g.n_reg == 4)
+        return assemble_rows_w<4>(g, r0, r1);
+    if (g.n_reg == 8)
+        return assemble_rows_w<8>(g, r0, r1);
+    if (g.n_reg == 16)
+        return assemble_rows_w<16>(g, r0, r1);
+    const double scale = -0.5 / (g.lengthscale * g.lengthscale);
+    const long np = g.n_pad;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (long i = r0 + wv; i < r1; i += THREADS / 64)
+        for (long j = lane; j < ((i >> 6) + 1) * 64; j += 64)
+        {
+            double val;
+            if (i < g.n && j < g.n)
+            {
+                val = g.vertical * exp(scale * window_sqdist(g.x, i, g.x, j, g.n_reg));
+                if (i == j)
+                    val += g.noise;
+            }
+            else
+            {
+                val = (i == j) ? 1.0 : 0.0;
+            }
+            g.L[i * np + j] = val;
+        }
+}
+
+// This lane's 16 entries of K_ij - sum_{k<jb} L_ik L_jk^T at the
+// accumulator positions of the product's precision (left-looking step of
+// block (ib, jb); the K_ij loads are in flight while the products run).
+template <bool MIXED>
+__device__ __forceinline__ void reduced_block(const BatchGP &g, int ib, int jb, Shared &sh, double (&kij)[16])
+{
+    const long np = g.n_pad;
+    const double *Cij = g.L + static_cast<long>(ib) * NB * np + jb * NB;
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+    {
+        int r, c;
+        acc_position<MIXED>(q, r, c);
+        kij[q] = Cij[static_cast<long>(r) * np + c];
+    }
+    if constexpr (MIXED)
+    {
+        v4f acc[2][2];
+        accumulate_row_products32(g.L, np, ib, jb, jb, sh.st32, acc);
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+            kij[q] -= static_cast<double>(acc[q >> 3][(q >> 2) & 1][q & 3]);
+    }
+    else
+    {
+        v4d acc[2][2];
+        accumulate_row_products(g.L, np, ib, jb, jb, sh.st, acc);
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+            kij[q] -= acc[q >> 3][(q >> 2) & 1][q & 3];
+    }
+}
+
+// Diagonal block jb: reduced, then factored with its inverse in LDS; L_jj
+// goes back to the factor, W_jj (row-major, zero above the diagonal) to
+// the GP's inverses and, when wjj is given, to LDS (stride XS).
+template <bool MIXED>
+__device__ __forceinline__ void diag_block(const BatchGP &g, int gi, int jb, Shared &sh, double *wjj, int *info)
+{
+    const long np = g.n_pad;
+    double kij[16];
+    reduced_block<MIXED>(g, jb, jb, sh, kij);
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+    {
+        int r, c;
+        acc_position<MIXED>(q, r, c);
+        sh.fa.D[r][c] = (c <= r) ? kij[q] : 0.0;
+    }
+    __syncthreads();
+    factor_block64(sh.fa.D, sh.fa.W, sh.fa.R, info + gi, jb * NB);
+    double *Cjj = g.L + static_cast<long>(jb) * NB * np + jb * NB;
+    double *Wjj = g.W + static_cast<long>(jb) * NB * NB;
+    for (int e = threadIdx.x; e < NB * NB; e += THREADS)
+    {
+        const int r = e / NB, c = e % NB;
+        const double w = (c <= r) ? sh.fa.W[r][c] : 0.0;
+        if (c <= r)
+            Cjj[static_cast<long>(r) * np + c] = sh.fa.D[r][c];
+        Wjj[e] = w;
+        if (wjj)
+            wjj[r * XS + c] = w;
+    }
+    __syncthreads();
+}
+
+// Off-diagonal block (ib, jb), ib > jb: reduced, then L_ij = (K_ij - acc)
+// W_jj^T with W_jj from LDS (wjj) or, when wjj is null, staged from the
+// GP's inverses into the second half of the stage.
+template <bool MIXED>
+__device__ __forceinline__ void off_block(const BatchGP &g, int ib, int jb, Shared &sh, const double *wjj)
+{
+    const long np = g.n_pad;
+    double kij[16];
+    reduced_block<MIXED>(g, ib, jb, sh, kij);
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+    {
+        int r, c;
+        acc_position<MIXED>(q, r, c);
+        sh.st.a[r * XS + c] = kij[q];
+    }
+    if (!wjj)
+        blk::stage_block(sh.st.b, g.W + static_cast<long>(jb) * NB * NB, NB);
+    __syncthreads();
+    v4d xb[2][2];
+    blk::mfma_nt(sh.st.a, wjj ? wjj : sh.st.b, xb);
+    blk::store_acc(g.L + static_cast<long>(ib) * NB * np + jb * NB, np, xb, true, false);
+    __syncthreads();
+}
+
+// alpha (refined in FP64 for MIXED) and the loss of one factored GP.
+template <bool MIXED>
+__device__ __forceinline__ void finish_gp(const BatchGP &g, int gi, double (*red)[THREADS], double *losses, int *refine_steps)
+{
+    const int tid = threadIdx.x;
+    const double *x = g.x;
+    const double *y = g.y;
+    const double *L = g.L;
+    const double *Wg = g.W;
+    double *alpha = g.alpha;
+    const long np = g.n_pad;
+    const int nblk = static_cast<int>(np / NB);
+    const double scale = -0.5 / (g.lengthscale * g.lengthscale);
+
+    // alpha: forward then backward block substitution with W
+    double *xs = red[0];
+    double *part = red[1];
+    for (long i = tid; i < np; i += THREADS)
+        alpha[i] = (i < g.n) ? y[i] : 0.0;
+    __syncthreads();
+    block_solves(L, Wg, np, nblk, alpha, xs, part);
+    if constexpr (MIXED)
+    {
+        // FP64 refinement: r = y - K alpha with K regenerated from the
+        // series, delta = K^-1 r from the factor, alpha += delta, until
+        // the correction (or the next one, as the contraction so far
+        // predicts it) is at most 1e-12 max|alpha|, or stops shrinking
+        double *r = g.work;
+        double prev = 1e300;
+        int steps = 0;
+        for (int it = 0; it < 10; it++)
+        {
+            if (g.n_reg == 4)
+                residual_rows<4>(g, scale, alpha, r);
+            else if (g.n_reg == 8)
+                residual_rows<8>(g, scale, alpha, r);
+            else if (g.n_reg == 16)
+                residual_rows<16>(g, scale, alpha, r);
+            else
+                for (long i = tid; i < np; i += THREADS)
+                {
+                    double s = 0.0;
+                    if (i < g.n)
+                    {
+                        for (long j = 0; j < g.n; j++)
+                            s += g.vertical * exp(scale * window_sqdist(x, i, x, j, g.n_reg)) * alpha[j];
+                        s += g.noise * alpha[i];
+                        s = y[i] - s;
+                    }
+                    r[i] = s;
+                }
+            __syncthreads();
+            block_solves(L, Wg, np, nblk, r, xs, part);
+            double dmax = 0.0, amax = 0.0;
+            for (long i = tid; i < np; i += THREADS)
+            {
+                dmax = fmax(dmax, fabs(r[i]));
+                amax = fmax(amax, fabs(alpha[i]));
+            }
+            dmax = block_max(dmax, part);
+            amax = block_max(amax, part);
+            // a correction that did not shrink is not applied
+            if (it > 0 && dmax >= prev)
+                break;
+            for (long i = tid; i < np; i += THREADS)
+                alpha[i] += r[i];
+            __syncthreads();
+            steps = it + 1;
+            const bool converged = dmax <= 1e-12 * amax || (it > 0 && dmax * (dmax / prev) <= 1e-12 * amax);
+            if (converged || !(dmax < 0.5 * prev))
+                break;
+            prev = dmax;
+        }
+        if (refine_steps && tid == 0)
+            refine_steps[gi] = steps;
+    }
+
+    // loss = 0.5 (y^T alpha + sum log L_ii^2 + n log 2 pi) / n
+    double s1 = 0.0, s2 = 0.0;
+    for (long i = tid; i < g.n; i += THREADS)
+    {
+        s1 += y[i] * alpha[i];
+        const double d = L[i * np + i];
+        s2 += log(d * d);
+    }
+    red[0][tid] = s1;
+    red[1][tid] = s2;
+    __syncthreads();
+    for (int w = THREADS / 2; w > 0; w >>= 1)
+    {
+        if (tid < w)
+        {
+            red[0][tid] += red[0][tid + w];
+            red[1][tid] += red[1][tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0)
+    {
+        const double Nn = static_cast<double>(g.n);
+        double l = red[0][0] + red[1][0];
+        l += Nn * log(2.0 * 3.14159265358979323846);
+        losses[gi] = 0.5 * l / Nn;
+    }
+    __syncthreads();
+}
+
+// Persistent path: each workgroup takes whole GPs from a device counter
+// (largest first) and runs assembly, the left-looking factorisation and
+// finish_gp without leaving the workgroup; W_jj stays in LDS for its column.
 template <bool MIXED>
 __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict__ gps,
                                                         const int *__restrict__ order,
@@ -373,200 +651,57 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
             break;
         const int gi = order[slot];
         const BatchGP g = gps[gi];
-        const double *x = g.x;
-        const double *y = g.y;
-        double *L = g.L;
-        double *Wg = g.W;
-        double *alpha = g.alpha;
-        const long np = g.n_pad;
-        const int nblk = static_cast<int>(np / NB);
-        const double scale = -0.5 / (g.lengthscale * g.lengthscale);
-
-        // 1. lower band of K (whole 64-blocks on and below the diagonal),
-        // identity padding; wavefronts take rows, lanes take columns
-        const int lane = tid & 63, wv = tid >> 6;
-        for (long i = wv; i < np; i += THREADS / 64)
-            for (long j = lane; j < ((i >> 6) + 1) * 64; j += 64)
-            {
-                double val;
-                if (i < g.n && j < g.n)
-                {
-                    val = g.vertical * exp(scale * window_sqdist(x, i, x, j, g.n_reg));
-                    if (i == j)
-                        val += g.noise;
-                }
-                else
-                {
-                    val = (i == j) ? 1.0 : 0.0;
-                }
-                L[i * np + j] = val;
-            }
+        const int nblk = static_cast<int>(g.n_pad / NB);
+        assemble_rows(g, 0, g.n_pad);
         __syncthreads();
-
-        // 2. left-looking blocked Cholesky
         for (int jb = 0; jb < nblk; jb++)
         {
-            for (int ib = jb; ib < nblk; ib++)
-            {
-                double *Cij = L + static_cast<long>(ib) * NB * np + jb * NB;
-                // this lane's 16 entries of K_ij - sum_k L_ik L_jk^T, at the
-                // accumulator positions of the product's precision
-                double kij[16];
-#pragma unroll
-                for (int q = 0; q < 16; q++)
-                {
-                    int r, c;
-                    acc_position<MIXED>(q, r, c);
-                    kij[q] = Cij[static_cast<long>(r) * np + c];  // in flight while the products run
-                }
-                if constexpr (MIXED)
-                {
-                    v4f acc[2][2];
-                    accumulate_row_products32(L, np, ib, jb, jb, sh.st32, acc);
-#pragma unroll
-                    for (int q = 0; q < 16; q++)
-                        kij[q] -= static_cast<double>(acc[q >> 3][(q >> 2) & 1][q & 3]);
-                }
-                else
-                {
-                    v4d acc[2][2];
-                    accumulate_row_products(L, np, ib, jb, jb, sh.st, acc);
-#pragma unroll
-                    for (int q = 0; q < 16; q++)
-                        kij[q] -= acc[q >> 3][(q >> 2) & 1][q & 3];
-                }
-                if (ib == jb)
-                {
-                    // D = K_jj - acc (lower), factor + inverse in LDS
-#pragma unroll
-                    for (int q = 0; q < 16; q++)
-                    {
-                        int r, c;
-                        acc_position<MIXED>(q, r, c);
-                        sh.fa.D[r][c] = (c <= r) ? kij[q] : 0.0;
-                    }
-                    __syncthreads();
-                    factor_block64(sh.fa.D, sh.fa.W, sh.fa.R, info + gi, jb * NB);
-                    double *Wjj = Wg + static_cast<long>(jb) * NB * NB;
-                    for (int e = tid; e < NB * NB; e += THREADS)
-                    {
-                        const int r = e / NB, c = e % NB;
-                        const double w = (c <= r) ? sh.fa.W[r][c] : 0.0;
-                        if (c <= r)
-                            Cij[static_cast<long>(r) * np + c] = sh.fa.D[r][c];
-                        Wjj[e] = w;
-                        wjj[r * XS + c] = w;
-                    }
-                    __syncthreads();
-                }
-                else
-                {
-                    // L_ij = (K_ij - acc) W_jj^T
-#pragma unroll
-                    for (int q = 0; q < 16; q++)
-                    {
-                        int r, c;
-                        acc_position<MIXED>(q, r, c);
-                        sh.st.a[r * XS + c] = kij[q];
-                    }
-                    __syncthreads();
-                    v4d xb[2][2];
-                    blk::mfma_nt(sh.st.a, wjj, xb);
-                    blk::store_acc(Cij, np, xb, true, false);
-                    __syncthreads();
-                }
-            }
+            diag_block<MIXED>(g, gi, jb, sh, wjj, info);
+            for (int ib = jb + 1; ib < nblk; ib++)
+                off_block<MIXED>(g, ib, jb, sh, wjj);
         }
-
-        // 3. alpha: forward then backward block substitution with W
-        double *xs = red[0];
-        double *part = red[1];
-        for (long i = tid; i < np; i += THREADS)
-            alpha[i] = (i < g.n) ? y[i] : 0.0;
-        __syncthreads();
-        block_solves(L, Wg, np, nblk, alpha, xs, part);
-        if constexpr (MIXED)
-        {
-            // FP64 refinement: r = y - K alpha with K regenerated from the
-            // series, delta = K^-1 r from the factor, alpha += delta, until
-            // the correction (or the next one, as the contraction so far
-            // predicts it) is at most 1e-12 max|alpha|, or stops shrinking
-            double *r = g.work;
-            double prev = 1e300;
-            int steps = 0;
-            for (int it = 0; it < 10; it++)
-            {
-                if (g.n_reg == 4)
-                    residual_rows<4>(g, scale, alpha, r);
-                else if (g.n_reg == 8)
-                    residual_rows<8>(g, scale, alpha, r);
-                else if (g.n_reg == 16)
-                    residual_rows<16>(g, scale, alpha, r);
-                else
-                    for (long i = tid; i < np; i += THREADS)
-                    {
-                        double s = 0.0;
-                        if (i < g.n)
-                        {
-                            for (long j = 0; j < g.n; j++)
-                                s += g.vertical * exp(scale * window_sqdist(x, i, x, j, g.n_reg)) * alpha[j];
-                            s += g.noise * alpha[i];
-                            s = y[i] - s;
-                        }
-                        r[i] = s;
-                    }
-                __syncthreads();
-                block_solves(L, Wg, np, nblk, r, xs, part);
-                double dmax = 0.0, amax = 0.0;
-                for (long i = tid; i < np; i += THREADS)
-                {
-                    const double a = alpha[i] + r[i];
-                    alpha[i] = a;
-                    dmax = fmax(dmax, fabs(r[i]));
-                    amax = fmax(amax, fabs(a));
-                }
-                __syncthreads();
-                dmax = block_max(dmax, part);
-                amax = block_max(amax, part);
-                steps = it + 1;
-                const bool converged = dmax <= 1e-12 * amax || (it > 0 && dmax < prev && dmax * (dmax / prev) <= 1e-12 * amax);
-                if (converged || !(dmax < 0.5 * prev))
-                    break;
-                prev = dmax;
-            }
-            if (refine_steps && tid == 0)
-                refine_steps[gi] = steps;
-        }
-
-        // 4. loss = 0.5 (y^T alpha + sum log L_ii^2 + n log 2 pi) / n
-        double s1 = 0.0, s2 = 0.0;
-        for (long i = tid; i < g.n; i += THREADS)
-        {
-            s1 += y[i] * alpha[i];
-            const double d = L[i * np + i];
-            s2 += log(d * d);
-        }
-        red[0][tid] = s1;
-        red[1][tid] = s2;
-        __syncthreads();
-        for (int w = THREADS / 2; w > 0; w >>= 1)
-        {
-            if (tid < w)
-            {
-                red[0][tid] += red[0][tid + w];
-                red[1][tid] += red[1][tid + w];
-            }
-            __syncthreads();
-        }
-        if (tid == 0)
-        {
-            const double Nn = static_cast<double>(g.n);
-            double l = red[0][0] + red[1][0];
-            l += Nn * log(2.0 * 3.14159265358979323846);
-            losses[gi] = 0.5 * l / Nn;
-        }
-        __syncthreads();
+        finish_gp<MIXED>(g, gi, red, losses, refine_steps);
     }
+}
+
+// Level-synchronous path for large GPs, one launch per step over all of
+// them: workgroup w takes item w of its launch's list (GP index and block
+// row). The same block functions run in the same order per block as in the
+// persistent path, so a GP gets bit-identical results on either path.
+__global__ __launch_bounds__(THREADS) void k_batch_assemble_level(const BatchGP *__restrict__ gps, const int2 *__restrict__ items)
+{
+    const int2 it = items[blockIdx.x];
+    const BatchGP g = gps[it.x];
+    assemble_rows(g, static_cast<long>(it.y) * NB, static_cast<long>(it.y + 1) * NB);
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(THREADS) void k_batch_diag_level(const BatchGP *__restrict__ gps, const int2 *__restrict__ items, int jb,
+                                                               int *__restrict__ info)
+{
+    __shared__ Shared sh;
+    const int2 it = items[blockIdx.x];
+    const BatchGP g = gps[it.x];
+    diag_block<MIXED>(g, it.x, jb, sh, nullptr, info);
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(THREADS) void k_batch_off_level(const BatchGP *__restrict__ gps, const int2 *__restrict__ items, int jb)
+{
+    __shared__ Shared sh;
+    const int2 it = items[blockIdx.x];
+    const BatchGP g = gps[it.x];
+    off_block<MIXED>(g, it.y, jb, sh, nullptr);
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(THREADS) void k_batch_finish_level(const BatchGP *__restrict__ gps, const int2 *__restrict__ items,
+                                                                 double *__restrict__ losses, int *__restrict__ refine_steps)
+{
+    __shared__ double red[2][THREADS];
+    const int2 it = items[blockIdx.x];
+    const BatchGP g = gps[it.x];
+    finish_gp<MIXED>(g, it.x, red, losses, refine_steps);
 }
 
 __global__ void k_set_ints(int *p, int count, int value)
@@ -578,19 +713,57 @@ __global__ void k_set_ints(int *p, int count, int value)
 
 }  // namespace
 
+void batch_info_reset(hipStream_t s, int *info, int count)
+{
+    require(count >= 1, "batch: bad count");
+    hipLaunchKernelGGL(k_set_ints, dim3(ceil_div(count, 256)), dim3(256), 0, s, info, count, INT_MAX);
+    GP_CHECK_LAUNCH();
+}
+
 void batch_fit(hipStream_t s, const BatchGP *gps, const int *order, int count, int *counter, double *losses, int *info,
                int workgroups, bool mixed, int *refine_steps)
 {
     require(count >= 1 && workgroups >= 1, "batch_fit: bad shape");
     hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(1), 0, s, counter, 1, 0);
     GP_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_set_ints, dim3(ceil_div(count, 256)), dim3(256), 0, s, info, count, INT_MAX);
-    GP_CHECK_LAUNCH();
     if (mixed)
         hipLaunchKernelGGL(k_batch_fit<true>, dim3(workgroups), dim3(THREADS), 0, s, gps, order, count, counter, losses, info, refine_steps);
     else
         hipLaunchKernelGGL(k_batch_fit<false>, dim3(workgroups), dim3(THREADS), 0, s, gps, order, count, counter, losses, info,
                            refine_steps);
+    GP_CHECK_LAUNCH();
+}
+
+void batch_level_step(hipStream_t s, BatchLevelStep step, const BatchGP *gps, const int2 *items, int n_items, int jb, bool mixed,
+                      double *losses, int *info, int *refine_steps)
+{
+    if (n_items <= 0)
+        return;
+    const dim3 grid(static_cast<unsigned>(n_items));
+    switch (step)
+    {
+    case BatchLevelStep::Assemble:
+        hipLaunchKernelGGL(k_batch_assemble_level, grid, dim3(THREADS), 0, s, gps, items);
+        break;
+    case BatchLevelStep::Diagonal:
+        if (mixed)
+            hipLaunchKernelGGL(k_batch_diag_level<true>, grid, dim3(THREADS), 0, s, gps, items, jb, info);
+        else
+            hipLaunchKernelGGL(k_batch_diag_level<false>, grid, dim3(THREADS), 0, s, gps, items, jb, info);
+        break;
+    case BatchLevelStep::OffDiagonal:
+        if (mixed)
+            hipLaunchKernelGGL(k_batch_off_level<true>, grid, dim3(THREADS), 0, s, gps, items, jb);
+        else
+            hipLaunchKernelGGL(k_batch_off_level<false>, grid, dim3(THREADS), 0, s, gps, items, jb);
+        break;
+    case BatchLevelStep::Finish:
+        if (mixed)
+            hipLaunchKernelGGL(k_batch_finish_level<true>, grid, dim3(THREADS), 0, s, gps, items, losses, refine_steps);
+        else
+            hipLaunchKernelGGL(k_batch_finish_level<false>, grid, dim3(THREADS), 0, s, gps, items, losses, refine_steps);
+        break;
+    }
     GP_CHECK_LAUNCH();
 }
 

@@ -180,8 +180,27 @@ struct BatchGP
 // of the factorisation on FP32 matrix cores (FP64 diagonal blocks, FP64
 // storage) and alpha refined in FP64 against K regenerated from the
 // series; refine_steps[g] (when given) gets GP g's refinement steps.
+// info must be reset (batch_info_reset) before the first launch of a batch.
 void batch_fit(hipStream_t s, const BatchGP *gps, const int *order, int count, int *counter, double *losses, int *info,
                int workgroups, bool mixed = false, int *refine_steps = nullptr);
+// info[g] = INT_MAX for g < count (a kernel, so it may be captured in a graph).
+void batch_info_reset(hipStream_t s, int *info, int count);
+// Level-synchronous path for large GPs: one launch per step over all of
+// them, one workgroup per item (int2 {GP index, block row}); the same block
+// functions as the persistent path, so results are bit-identical. Order:
+// Assemble (items: every 64-row block of every GP), then for jb = 0, 1, ...:
+// Diagonal (items: GPs with more than jb blocks) and OffDiagonal (items:
+// blocks ib > jb of those GPs), then Finish (items: every GP; alpha, its
+// FP64 refinement for mixed, the loss).
+enum class BatchLevelStep
+{
+    Assemble,
+    Diagonal,
+    OffDiagonal,
+    Finish
+};
+void batch_level_step(hipStream_t s, BatchLevelStep step, const BatchGP *gps, const int2 *items, int n_items, int jb, bool mixed,
+                      double *losses, int *info, int *refine_steps);
 
 // ---- triangular family (k_triangular.hip) ---------------------------------
 

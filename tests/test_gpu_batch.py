@@ -1,5 +1,10 @@
-"""Batched many-GP fit (one persistent launch) against single-GP fits and
-the CPU oracle: every GP's loss, mean and variance agree to 1e-10."""
+"""Batched many-GP fit against single-GP fits and the CPU oracle: every GP's
+loss, mean and variance agree to 1e-10. The two execution paths (a
+persistent kernel running whole GPs, and level-synchronous launches over
+the 64x64 blocks of all large GPs, replayed from a captured HIP graph when
+a batch composition repeats) give bit-identical results."""
+import os
+
 import numpy as np
 import pytest
 
@@ -81,17 +86,85 @@ def test_batch_reports_not_spd(ctx):
     assert e.value.status == capi.ERR_NOT_SPD and "GP 1" in str(e.value)
 
 
-def test_mixed_batch_vs_fp64_batch(ctx):
-    """Mixed precision batch (FP32 block products, FP64 refinement of alpha)
-    against the FP64 batch on the same 48 GPs: mean 1e-8 normwise (the
-    refined alpha; the bar of the mixed-precision configuration). The
-    variance (solved with the mixed factor, not refined) and the loss (its
+@pytest.fixture
+def force(monkeypatch):
+    """Set the batched fit's routing overrides for one test."""
+    def set_(path=None, mixed=None):
+        for k, v in (("GPRAT_BATCH_PATH", path), ("GPRAT_BATCH_MIXED", mixed)):
+            if v is None:
+                monkeypatch.delenv(k, raising=False)
+            else:
+                monkeypatch.setenv(k, v)
+    return set_
+
+
+def test_paths_and_graph_replays_are_bit_identical(ctx, force):
+    """The same batch through the persistent kernel and the level path
+    (first direct launches, then the capture on the composition's second
+    fit, then replays), with other device work and synchronisations between
+    the fits: every replay equals the eager result bit for bit."""
+    from gprat_amd import capi
+    probs = _problems(24, seed=21, n_lo=64, n_hi=1500)
+    fits = [(x, y, n, r, t) for x, y, n, r, t, _ in probs]
+    picks = (0, 7, 23)
+
+    def run():
+        b = capi.Batch(ctx, fits)
+        try:
+            return b.losses().copy(), [b.predict(g, probs[g][5], 50) for g in picks]
+        finally:
+            b.release()
+
+    force(path="persistent")
+    ref_loss, ref_pred = run()
+    force(path="level")
+    x, y, n, r, t, z = probs[3]
+    for rep in range(4):
+        loss, pred = run()
+        assert np.array_equal(loss, ref_loss), rep
+        for (m1, v1), (m2, v2) in zip(pred, ref_pred):
+            assert np.array_equal(m1, m2) and np.array_equal(v1, v2), rep
+        # eager work of the program between the fits
+        mdl = capi.Model(ctx, x, y, n, r, t)
+        mdl.predict(z, 50)
+        mdl.release()
+        ctx.synchronize()
+
+
+def test_tuned_mixed_batch_runs_fp64_where_mixed_is_slower(ctx, force):
+    """With the committed tuning table (mixed precision measured slower than
+    FP64 for every size class of the batched fit), a batch that asks for
+    mixed precision runs every GP in FP64: no refinement steps, results equal
+    to the FP64 batch."""
+    from gprat_amd import capi
+    force()
+    probs = _problems(12, seed=31, n_lo=64, n_hi=2048)
+    fits = [(x, y, n, r, t) for x, y, n, r, t, _ in probs]
+    b64 = capi.Batch(ctx, fits)
+    bmx = capi.Batch(ctx, fits, precision=capi.PRECISION_MIXED)
+    try:
+        assert np.all(bmx.refinement() == 0)
+        assert np.array_equal(bmx.losses(), b64.losses())
+        m1, v1 = bmx.predict(5, probs[5][5], 50)
+        m2, v2 = b64.predict(5, probs[5][5], 50)
+        assert np.array_equal(m1, m2) and np.array_equal(v1, v2)
+    finally:
+        b64.release()
+        bmx.release()
+
+
+def test_mixed_batch_vs_fp64_batch(ctx, force):
+    """Mixed precision batch (FP32 block products, FP64 refinement of alpha
+    and of the variance) against the FP64 batch on the same 48 GPs: mean
+    1e-8 normwise (the refined alpha; the bar of the mixed-precision
+    configuration). The variance and the loss (its
     log-determinant comes from the mixed factor) carry the factor's own
     accuracy: every off-diagonal block product rounds its operands to FP32,
     and these GPs reach condition numbers of ~4e5 (noise down to 0.01,
     lengthscale up to 3), so their bars are 1e-3 and 1e-4 (measured
     1.4e-4 and 2.0e-5). Every GP took at least one refinement step."""
     from gprat_amd import capi
+    force(mixed="requested")  # mixed precision for every GP, whatever the tuning table says
     probs = _problems(48, seed=5, n_lo=64, n_hi=2048)
     fits = [(x, y, n, r, t) for x, y, n, r, t, _ in probs]
     b64 = capi.Batch(ctx, fits)
