@@ -213,6 +213,8 @@ PYBIND11_MODULE(gprat, m)
           py::arg("separator") = " ");
     m.def("start_hpx", &start_hpx_wrapper, py::arg("args"), py::arg("n_cores"));
     m.def("resume_hpx", &utils::resume_hpx_runtime);
+    m.def("runtime_state", &utils::runtime_state);
+    m.def("host_threads", &utils::host_threads);
     m.def("suspend_hpx", &utils::suspend_hpx_runtime);
     m.def("stop_hpx", &utils::stop_hpx_runtime);
     m.def("compiled_with_cuda", &utils::compiled_with_cuda, "True: the GPU backend is compiled in");

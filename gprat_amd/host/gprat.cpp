@@ -3,6 +3,9 @@
 // reference; every O(n^2) or O(n^3) step runs on the GPU.
 #include "gprat.hpp"
 
+#include <mutex>
+#include <thread>
+
 #include <cmath>
 #include <cstdio>
 #include <iomanip>
@@ -67,6 +70,15 @@ void adam_update(gprat_hyper::SEKParams &p,
 }
 
 int g_host_threads = 0;
+enum class RuntimeState
+{
+    NotStarted,
+    Running,
+    Suspended,
+    Stopped
+};
+RuntimeState g_runtime_state = RuntimeState::NotStarted;
+std::mutex g_runtime_mu;
 
 }  // namespace
 
@@ -261,6 +273,7 @@ GP::GP(std::vector<double> input,
 
 gprat_context *GP::gpu_context()
 {
+    utils::require_runnable();
     auto gpu = std::dynamic_pointer_cast<HIP_GPU>(target_);
     if (!gpu)
         throw std::runtime_error("gprat_amd executes on the GPU only; construct the GP with a GPU target "
@@ -474,8 +487,17 @@ void print_vector(const std::vector<double> &vec, int start, int end, const std:
     std::cout << std::endl;
 }
 
+// The host runtime's life cycle as core/src/utils_c.cpp:123-133 drives HPX:
+// start (with "--hpx:threads=N"), suspend, resume, stop, start again. This
+// backend needs no host runtime to run the GP path on the GPU, so calls made
+// before start or after stop run; a suspended runtime runs no work, so the
+// GP API refuses calls until it is resumed.
 void start_hpx_runtime(int argc, char **argv)
 {
+    std::lock_guard<std::mutex> g(g_runtime_mu);
+    if (g_runtime_state == RuntimeState::Running || g_runtime_state == RuntimeState::Suspended)
+        throw std::runtime_error("the runtime is already started (stop_hpx_runtime first)");
+    g_host_threads = static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
     // honour "--hpx:threads=N" so callers can query the requested core count
     for (int i = 0; i < argc && argv && argv[i]; i++)
     {
@@ -484,11 +506,51 @@ void start_hpx_runtime(int argc, char **argv)
         if (a.rfind(key, 0) == 0)
             g_host_threads = std::atoi(a.c_str() + key.size());
     }
+    g_runtime_state = RuntimeState::Running;
 }
-void resume_hpx_runtime() { }
-void suspend_hpx_runtime() { }
-void stop_hpx_runtime() { }
+void suspend_hpx_runtime()
+{
+    std::lock_guard<std::mutex> g(g_runtime_mu);
+    if (g_runtime_state != RuntimeState::Running)
+        throw std::runtime_error("suspend_hpx_runtime: the runtime is not running");
+    g_runtime_state = RuntimeState::Suspended;
+}
+void resume_hpx_runtime()
+{
+    std::lock_guard<std::mutex> g(g_runtime_mu);
+    if (g_runtime_state != RuntimeState::Suspended)
+        throw std::runtime_error("resume_hpx_runtime: the runtime is not suspended");
+    g_runtime_state = RuntimeState::Running;
+}
+void stop_hpx_runtime()
+{
+    std::lock_guard<std::mutex> g(g_runtime_mu);
+    if (g_runtime_state != RuntimeState::Running && g_runtime_state != RuntimeState::Suspended)
+        throw std::runtime_error("stop_hpx_runtime: the runtime is not started");
+    g_runtime_state = RuntimeState::Stopped;
+}
 int host_threads() { return g_host_threads; }
+std::string runtime_state()
+{
+    std::lock_guard<std::mutex> g(g_runtime_mu);
+    switch (g_runtime_state)
+    {
+    case RuntimeState::Running:
+        return "running";
+    case RuntimeState::Suspended:
+        return "suspended";
+    case RuntimeState::Stopped:
+        return "stopped";
+    default:
+        return "not started";
+    }
+}
+void require_runnable()
+{
+    std::lock_guard<std::mutex> g(g_runtime_mu);
+    if (g_runtime_state == RuntimeState::Suspended)
+        throw std::runtime_error("the runtime is suspended: resume_hpx_runtime() before calling the GP API");
+}
 
 bool compiled_with_cuda() { return true; }
 bool compiled_with_hip() { return true; }

@@ -183,13 +183,20 @@ int compute_train_tile_size(int n_samples, int n_tiles);
 std::pair<int, int> compute_test_tiles(int n_test, int n_tiles, int n_tile_size);
 std::vector<double> load_data(const std::string &file_path, int n_samples, int offset);
 void print_vector(const std::vector<double> &vec, int start, int end, const std::string &separator);
-// No HPX runtime exists in this backend; these keep the reference's call
-// sequence valid (start records the requested host thread count).
+// The reference's runtime life cycle (core/src/utils_c.cpp:123-133): start
+// (records "--hpx:threads=N"), suspend, resume, stop, and start again; out
+// of order calls throw std::runtime_error as HPX refuses them. The GPU path
+// needs no host runtime: the GP API runs before start and after stop, and
+// refuses calls while the runtime is suspended.
 void start_hpx_runtime(int argc, char **argv);
 void resume_hpx_runtime();
 void suspend_hpx_runtime();
 void stop_hpx_runtime();
 int host_threads();
+// "not started", "running", "suspended" or "stopped"
+std::string runtime_state();
+// Throws if the runtime is suspended.
+void require_runnable();
 // True when the GPU backend is compiled in (always, for this library).
 bool compiled_with_cuda();
 bool compiled_with_hip();

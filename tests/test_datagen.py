@@ -70,3 +70,51 @@ def test_written_dataset_reads_back_through_gp_data(tmp_path):
     x, _, z = datagen.gp_problem(n_train, n_test, n_reg)
     assert np.array_equal(np.asarray(gprat.GP_data(os.path.join(str(tmp_path), "training_input.txt"), n_train, n_reg).data), x)
     assert np.array_equal(np.asarray(gprat.GP_data(os.path.join(str(tmp_path), "test_input.txt"), n_test, n_reg).data), z)
+
+
+def _uniform_scalar(seed, count):
+    """[0, 1) doubles from the top 53 bits of SplitMix64, scalar."""
+    return [(w >> 11) * (1.0 / 9007199254740992.0) for w in splitmix64_scalar(seed, count)]
+
+
+def test_mass_spring_damper_against_state_space_recurrence():
+    """The semi-implicit Euler step written as a linear state-space map,
+    s_t = A s_(t-1) + B u_t with s = (x, v):
+        v' = (1 - dt c/m) v - (dt k/m) x + (dt/m) u
+        x' = x + dt v' = (1 - dt^2 k/m) x + dt (1 - dt c/m) v + (dt^2/m) u,
+    evaluated from scalar SplitMix64 inputs, gives the generator's noise-free
+    output to rounding (different operation order), and the noise is the
+    Box-Muller draw from the same stream: y - x = 0.01 sqrt(-2 ln(1 - u1))
+    cos(2 pi u2) with u1, u2 the stream's second and third thirds."""
+    import math
+    from gprat_amd import datagen
+    n, seed = 400, 0
+    m, k, c, dt = 1.0, 1.0, 0.5, 0.05
+    A = [[1 - dt * dt * k / m, dt * (1 - dt * c / m)], [-dt * k / m, 1 - dt * c / m]]
+    B = [dt * dt / m, dt / m]
+    r = _uniform_scalar(seed, 3 * n)
+    u = [2.0 * v - 1.0 for v in r[:n]]
+    xs, x, v = [], 0.0, 0.0
+    for t in range(n):
+        x, v = A[0][0] * x + A[0][1] * v + B[0] * u[t], A[1][0] * x + A[1][1] * v + B[1] * u[t]
+        xs.append(x)
+    noise = [0.01 * math.sqrt(-2.0 * math.log1p(-r[n + t])) * math.cos(2.0 * math.pi * r[2 * n + t]) for t in range(n)]
+    gu, gy = datagen.msd_series(n, seed)
+    assert np.array_equal(gu, np.array(u))
+    assert np.max(np.abs((gy - np.array(noise)) - np.array(xs))) < 1e-12
+    assert np.max(np.abs(gy - np.array(xs) - np.array(noise))) < 1e-12
+
+
+def test_mass_spring_damper_step_response():
+    """The oscillator the generator integrates (m = 1, k = 1, c = 0.5,
+    dt = 0.05, semi-implicit Euler), driven by a constant u = 1, settles at
+    the static deflection u / k = 1 with a bounded overshoot: the
+    parameters give a stable, underdamped system."""
+    m, k, c, dt = 1.0, 1.0, 0.5, 0.05
+    x = v = 0.0
+    peak = 0.0
+    for _ in range(4000):
+        v = v + dt * (1.0 - k * x - c * v) / m
+        x = x + dt * v
+        peak = max(peak, x)
+    assert abs(x - 1.0 / k) < 1e-6 and 1.0 < peak < 2.0
