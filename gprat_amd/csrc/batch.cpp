@@ -26,9 +26,11 @@ namespace
 struct TuneRow
 {
     int n_pad_max;
-    bool level;
+    int path;  // 0 persistent, 1 level (64-wide), 2 level128
     bool mixed_faster;
 };
+
+int path_code(const std::string &p) { return p == "level128" ? 2 : (p == "level" ? 1 : 0); }
 
 struct Tuning
 {
@@ -85,7 +87,7 @@ Tuning load_tuning(int device)
         int mf = 0;
         if (!(ls >> r.n_pad_max >> p >> mf))
             continue;
-        r.level = p == "level";
+        r.path = path_code(p);
         r.mixed_faster = mf != 0;
         t.rows.push_back(r);
     }
@@ -106,18 +108,19 @@ const Tuning &tuning(int device)
     return it->second;
 }
 
-// Path and precision of one GP. GPRAT_BATCH_PATH (persistent | level) and
-// GPRAT_BATCH_MIXED (requested: mixed precision for every GP of a mixed
-// batch) override the table, for measurement.
+// Path and precision of one GP. GPRAT_BATCH_PATH (persistent | level |
+// level128) and GPRAT_BATCH_MIXED (requested: mixed precision for every GP
+// of a mixed batch) override the table, for measurement. The 128-wide level
+// path is FP64: a mixed GP routed there takes the 64-wide one.
 struct Route
 {
-    bool level;
+    int path;
     bool mixed;
 };
 
 Route route(const Tuning &t, int n_pad, bool mixed_requested)
 {
-    Route r{ n_pad > 1024, mixed_requested };
+    Route r{ n_pad > 1024 ? 1 : 0, mixed_requested };
     const TuneRow *row = nullptr;
     for (const auto &x : t.rows)
         if (n_pad <= x.n_pad_max)
@@ -129,19 +132,16 @@ Route route(const Tuning &t, int n_pad, bool mixed_requested)
         row = &t.rows.back();
     if (row)
     {
-        r.level = row->level;
+        r.path = row->path;
         r.mixed = mixed_requested && row->mixed_faster;
     }
     if (const char *e = std::getenv("GPRAT_BATCH_PATH"))
-    {
-        if (std::string(e) == "level")
-            r.level = true;
-        else if (std::string(e) == "persistent")
-            r.level = false;
-    }
+        r.path = path_code(e);
     if (const char *e = std::getenv("GPRAT_BATCH_MIXED"))
         if (std::string(e) == "requested")
             r.mixed = mixed_requested;
+    if (r.mixed && r.path == 2)
+        r.path = 1;
     return r;
 }
 
@@ -155,15 +155,20 @@ struct LevelPlan
     std::vector<long> diag_off, diag_n, off_off, off_n;
 };
 
-LevelPlan level_plan(const std::vector<int> &ids, const std::vector<int> &nb)
+// nb: 64-row blocks per GP; width: the column width of the factorisation
+// steps (64 or 128; blocks per step column = ceil(nb * 64 / width)).
+LevelPlan level_plan(const std::vector<int> &ids, const std::vector<int> &nb64, int width)
 {
     LevelPlan p;
+    std::vector<int> nb(nb64.size());
+    for (std::size_t g = 0; g < nb64.size(); g++)
+        nb[g] = static_cast<int>(ceil_div(static_cast<long>(nb64[g]) * 64, width));
     int maxb = 0;
     for (int g : ids)
         maxb = std::max(maxb, nb[g]);
     p.assemble_off = 0;
     for (int g : ids)
-        for (int i = 0; i < nb[g]; i++)
+        for (int i = 0; i < nb64[g]; i++)
             p.items.push_back(make_int2(g, i));
     p.assemble_n = static_cast<long>(p.items.size());
     for (int jb = 0; jb < maxb; jb++)
@@ -187,17 +192,19 @@ LevelPlan level_plan(const std::vector<int> &ids, const std::vector<int> &nb)
     return p;
 }
 
-void launch_level(hipStream_t s, const LevelPlan &p, const int2 *items, const k::BatchGP *gps, bool mixed, double *losses, int *info,
-                  int *steps)
+void launch_level(hipStream_t s, const LevelPlan &p, const int2 *items, const k::BatchGP *gps, bool mixed, bool wide, double *losses,
+                  int *info, int *steps)
 {
     k::batch_level_step(s, k::BatchLevelStep::Assemble, gps, items + p.assemble_off, static_cast<int>(p.assemble_n), 0, mixed, losses,
                         info, steps);
+    const auto diag = wide ? k::BatchLevelStep::Diagonal128 : k::BatchLevelStep::Diagonal;
+    const auto off = wide ? k::BatchLevelStep::OffDiagonal128 : k::BatchLevelStep::OffDiagonal;
     for (std::size_t jb = 0; jb < p.diag_off.size(); jb++)
     {
-        k::batch_level_step(s, k::BatchLevelStep::Diagonal, gps, items + p.diag_off[jb], static_cast<int>(p.diag_n[jb]),
-                            static_cast<int>(jb), mixed, losses, info, steps);
-        k::batch_level_step(s, k::BatchLevelStep::OffDiagonal, gps, items + p.off_off[jb], static_cast<int>(p.off_n[jb]),
-                            static_cast<int>(jb), mixed, losses, info, steps);
+        k::batch_level_step(s, diag, gps, items + p.diag_off[jb], static_cast<int>(p.diag_n[jb]), static_cast<int>(jb), mixed, losses, info,
+                            steps);
+        k::batch_level_step(s, off, gps, items + p.off_off[jb], static_cast<int>(p.off_n[jb]), static_cast<int>(jb), mixed, losses, info,
+                            steps);
     }
     k::batch_level_step(s, k::BatchLevelStep::Finish, gps, items + p.finish_off, static_cast<int>(p.finish_n), 0, mixed, losses, info,
                         steps);
@@ -255,15 +262,45 @@ std::vector<int> batch_run(Context &c, const std::vector<k::BatchGP> &gps, doubl
     for (int g = 0; g < count; g++)
         order[g] = g;
     std::stable_sort(order.begin(), order.end(), [&](int a, int z) { return gps[a].n > gps[z].n; });
-    std::vector<int> pers[2], lev[2];  // [mixed]
+    // persistent kernel: [mixed]; level path: FP64 64-wide, mixed 64-wide, FP64 128-wide
+    std::vector<int> pers[2];
+    struct LevelGroup
+    {
+        std::vector<int> ids;
+        bool mixed, wide;
+        LevelPlan plan;
+        long base = 0;
+    };
+    LevelGroup groups[3] = { { {}, false, false, {}, 0 }, { {}, true, false, {}, 0 }, { {}, false, true, {}, 0 } };
     for (int g : order)
-        (routes[g].level ? lev : pers)[routes[g].mixed ? 1 : 0].push_back(g);
+    {
+        if (routes[g].path == 0)
+            pers[routes[g].mixed ? 1 : 0].push_back(g);
+        else
+            groups[routes[g].path == 2 ? 2 : (routes[g].mixed ? 1 : 0)].ids.push_back(g);
+    }
+    // the 128-wide steps keep each block column's 128x128 inverse
+    std::vector<k::BatchGP> dgps = gps;
+    long w128_total = 0;
+    for (int g : groups[2].ids)
+        w128_total += ceil_div(gps[g].n_pad, 128) * 128 * 128;
+    DevBuf<double> w128;
+    if (w128_total > 0)
+    {
+        w128 = DevBuf<double>(c, static_cast<std::size_t>(w128_total));
+        long off = 0;
+        for (int g : groups[2].ids)
+        {
+            dgps[g].W128 = w128.get() + off;
+            off += ceil_div(gps[g].n_pad, 128) * 128 * 128;
+        }
+    }
 
     DevBuf<double> desc(c, (sizeof(k::BatchGP) * count + 7) / 8);
     DevBuf<int> dorder(c, count), info(c, count), counter(c, 2);
     DevBuf<double> losses(c, count);
     DevBuf<int> steps(c, count);
-    GP_HIP(hipMemcpyAsync(desc.get(), gps.data(), sizeof(k::BatchGP) * count, hipMemcpyDefault, c.main));
+    GP_HIP(hipMemcpyAsync(desc.get(), dgps.data(), sizeof(k::BatchGP) * count, hipMemcpyDefault, c.main));
     std::vector<int> porder;
     for (int m = 0; m < 2; m++)
         porder.insert(porder.end(), pers[m].begin(), pers[m].end());
@@ -280,25 +317,24 @@ std::vector<int> batch_run(Context &c, const std::vector<k::BatchGP> &gps, doubl
     GP_HIP(hipStreamWaitEvent(ls, ready, 0));
     std::vector<std::pair<std::shared_ptr<BatchGraph>, std::vector<int>>> replayed;  // graph and its GPs
     DevBuf<int2> items;
-    std::vector<LevelPlan> plans(2);
     long item_total = 0;
-    std::vector<long> item_base(2, 0);
-    for (int m = 0; m < 2; m++)
-        if (!lev[m].empty())
+    for (auto &gr : groups)
+        if (!gr.ids.empty())
         {
-            plans[m] = level_plan(lev[m], nb);
-            item_base[m] = item_total;
-            item_total += static_cast<long>(plans[m].items.size());
+            gr.plan = level_plan(gr.ids, nb, gr.wide ? 128 : 64);
+            gr.base = item_total;
+            item_total += static_cast<long>(gr.plan.items.size());
         }
     {
         Timed t(c, ls, "batch_fit_level");
-        for (int m = 0; m < 2; m++)
+        for (int q = 0; q < 3; q++)
         {
-            if (lev[m].empty())
+            LevelGroup &gr = groups[q];
+            if (gr.ids.empty())
                 continue;
-            // the composition: precision, then each GP's size in item order
-            std::string key = std::to_string(m) + ":" + std::to_string(count);
-            for (int g : lev[m])
+            // the composition: step kind, then each GP's size in item order
+            std::string key = std::to_string(q) + ":" + std::to_string(count);
+            for (int g : gr.ids)
                 key += "," + std::to_string(g) + "/" + std::to_string(nb[g]);
             if (c.batch_seen.size() > 64)
                 c.batch_seen.clear();
@@ -309,16 +345,16 @@ std::vector<int> batch_run(Context &c, const std::vector<k::BatchGP> &gps, doubl
                 // capture this composition's sequence into a graph of its own buffers
                 auto bg = std::make_shared<BatchGraph>();
                 bg->desc = DevBuf<double>(c, (sizeof(k::BatchGP) * count + 7) / 8);
-                bg->items = DevBuf<int2>(c, plans[m].items.size());
+                bg->items = DevBuf<int2>(c, gr.plan.items.size());
                 bg->info = DevBuf<int>(c, count);
                 bg->steps = DevBuf<int>(c, count);
                 bg->losses = DevBuf<double>(c, count);
-                GP_HIP(hipMemcpyAsync(bg->items.get(), plans[m].items.data(), sizeof(int2) * plans[m].items.size(), hipMemcpyDefault, ls));
+                GP_HIP(hipMemcpyAsync(bg->items.get(), gr.plan.items.data(), sizeof(int2) * gr.plan.items.size(), hipMemcpyDefault, ls));
                 GP_HIP(hipStreamSynchronize(ls));
                 GP_HIP(hipStreamBeginCapture(ls, hipStreamCaptureModeThreadLocal));
                 k::batch_info_reset(ls, bg->info.get(), count);
-                launch_level(ls, plans[m], bg->items.get(), reinterpret_cast<const k::BatchGP *>(bg->desc.get()), m == 1, bg->losses.get(),
-                             bg->info.get(), bg->steps.get());
+                launch_level(ls, gr.plan, bg->items.get(), reinterpret_cast<const k::BatchGP *>(bg->desc.get()), gr.mixed, gr.wide,
+                             bg->losses.get(), bg->info.get(), bg->steps.get());
                 GP_HIP(hipStreamEndCapture(ls, &bg->graph));
                 GP_HIP(hipGraphInstantiate(&bg->exec, bg->graph, nullptr, nullptr, 0));
                 // at most eight graphs per context, least recently used out
@@ -337,17 +373,17 @@ std::vector<int> batch_run(Context &c, const std::vector<k::BatchGP> &gps, doubl
             {
                 BatchGraph &bg = *git->second;
                 bg.last_use = ++c.batch_clock;
-                GP_HIP(hipMemcpyAsync(bg.desc.get(), gps.data(), sizeof(k::BatchGP) * count, hipMemcpyDefault, ls));
+                GP_HIP(hipMemcpyAsync(bg.desc.get(), dgps.data(), sizeof(k::BatchGP) * count, hipMemcpyDefault, ls));
                 GP_HIP(hipGraphLaunch(bg.exec, ls));
-                replayed.emplace_back(git->second, lev[m]);
+                replayed.emplace_back(git->second, gr.ids);
             }
             else
             {
                 if (!items.get())
                     items = DevBuf<int2>(c, static_cast<std::size_t>(item_total));
-                GP_HIP(hipMemcpyAsync(items.get() + item_base[m], plans[m].items.data(), sizeof(int2) * plans[m].items.size(),
-                                      hipMemcpyDefault, ls));
-                launch_level(ls, plans[m], items.get() + item_base[m], gdesc, m == 1, losses.get(), info.get(), steps.get());
+                GP_HIP(hipMemcpyAsync(items.get() + gr.base, gr.plan.items.data(), sizeof(int2) * gr.plan.items.size(), hipMemcpyDefault,
+                                      ls));
+                launch_level(ls, gr.plan, items.get() + gr.base, gdesc, gr.mixed, gr.wide, losses.get(), info.get(), steps.get());
             }
         }
     }

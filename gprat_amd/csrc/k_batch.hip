@@ -14,6 +14,7 @@
 // the diagonal block's inverse, which stays in LDS for the whole column.
 #include "block64.hpp"
 #include "block_mfma.hpp"
+#include "gemm_tile.hpp"
 #include "kernels.hpp"
 
 #include <climits>
@@ -704,6 +705,181 @@ __global__ __launch_bounds__(THREADS) void k_batch_finish_level(const BatchGP *_
     finish_gp<MIXED>(g, it.x, red, losses, refine_steps);
 }
 
+// ---------------------------------------------------------------------------
+// Level-synchronous path on 128-wide block columns: the accumulation of a
+// 128x128 block, C -= L_i,0:j0 L_j,0:j0^T, is one 128x128 matrix-core tile of
+// k_gemm.hip (K = j0 in 16-deep slabs), where the 64-wide path runs 64x64
+// block pairs. The diagonal step factors the 128x128 block as two 64x64
+// blocks (L00, L10 = D10 W00^T, D11 -= L10 L10^T, L11) and forms its inverse
+// [[W00, 0], [-W11 L10 W00, W11]] into the GP's W128 (row-major 128x128 per
+// block column); the 64x64 inverses also go to W, where the solves and
+// predictions read them. The off-diagonal step then computes L_ij = C_ij
+// W128^T in place (a second tile product with K = the block's width).
+
+union Shared128
+{
+    struct
+    {
+        double As[2][gt::BK * gt::LDS_STRIDE];
+        double Bs[2][gt::BK * gt::LDS_STRIDE];
+    } g;
+    Factor fa;
+    Stage st;
+};
+
+// C(rows x cols block at (i0, j0) of the GP's factor) -= sum over the first
+// j0 columns of L_i L_j^T (lower only when i0 == j0).
+__device__ __forceinline__ void tile_update(const BatchGP &g, long i0, long j0, int rows, int cols, Shared128 &sh)
+{
+    if (j0 == 0)
+        return;
+    const long np = g.n_pad;
+    gt::Params p{};
+    p.A = g.L + i0 * np;  // A(i, k) = L[i0 + i][k]
+    p.lda = np;
+    p.B = g.L + j0 * np;  // B(k, j) = L[j0 + j][k]
+    p.ldb = np;
+    p.C = g.L + i0 * np + j0;
+    p.ldc = np;
+    p.M = rows;
+    p.N = cols;
+    p.K = static_cast<int>(j0);
+    p.lower = i0 == j0 ? 1 : 0;
+    p.diag_off = 0;
+    p.set = 0;
+    gt::gemm_tile<false, true>(p, 0, 0, sh.g.As, sh.g.Bs);
+    __syncthreads();
+}
+
+// Diagonal step of 128-wide block column jb of one GP (the block may be 64
+// wide at the end of the matrix).
+__device__ __forceinline__ void diag_block128(const BatchGP &g, int gi, int jb, Shared128 &sh, int *info)
+{
+    const long np = g.n_pad;
+    const long j0 = static_cast<long>(jb) * 128;
+    const int w = static_cast<int>(min(static_cast<long>(128), np - j0));
+    tile_update(g, j0, j0, w, w, sh);
+    double *C = g.L + j0 * np + j0;
+    double *W00 = g.W + (j0 / NB) * NB * NB;
+    double *Wt = g.W128 + static_cast<long>(jb) * 128 * 128;
+    const int tid = threadIdx.x;
+    // D00 -> L00, W00
+    for (int e = tid; e < NB * NB; e += THREADS)
+    {
+        const int r = e / NB, c = e % NB;
+        sh.fa.D[r][c] = (c <= r) ? C[static_cast<long>(r) * np + c] : 0.0;
+    }
+    __syncthreads();
+    factor_block64(sh.fa.D, sh.fa.W, sh.fa.R, info + gi, static_cast<int>(j0));
+    for (int e = tid; e < NB * NB; e += THREADS)
+    {
+        const int r = e / NB, c = e % NB;
+        const double wv = (c <= r) ? sh.fa.W[r][c] : 0.0;
+        if (c <= r)
+            C[static_cast<long>(r) * np + c] = sh.fa.D[r][c];
+        W00[e] = wv;
+        Wt[r * 128 + c] = wv;
+        if (w == 128)
+            Wt[r * 128 + NB + c] = 0.0;
+    }
+    __syncthreads();
+    if (w < 128)
+        return;
+    double *C10 = C + NB * np, *C11 = C + NB * np + NB;
+    double *W11 = W00 + NB * NB;
+    v4d acc[2][2];
+    // L10 = D10 W00^T
+    blk::stage_block(sh.st.a, C10, np);
+    blk::stage_block(sh.st.b, W00, NB);
+    __syncthreads();
+    blk::mfma_nt(sh.st.a, sh.st.b, acc);
+    __syncthreads();
+    blk::store_acc(C10, np, acc, true, false);
+    __syncthreads();
+    // D11 = C11 - L10 L10^T -> L11, W11
+    blk::stage_block(sh.st.a, C10, np);
+    blk::stage_block(sh.st.b, C10, np);
+    __syncthreads();
+    blk::mfma_nt(sh.st.a, sh.st.b, acc);
+    double c11[16];
+    blk::load_at_acc(C11, np, c11);
+    __syncthreads();
+    int q = 0;
+    blk::for_each(acc,
+                  [&](int r, int c, double v)
+                  {
+                      sh.fa.D[r][c] = (c <= r) ? c11[q] - v : 0.0;
+                      q++;
+                  });
+    __syncthreads();
+    factor_block64(sh.fa.D, sh.fa.W, sh.fa.R, info + gi, static_cast<int>(j0 + NB));
+    for (int e = tid; e < NB * NB; e += THREADS)
+    {
+        const int r = e / NB, c = e % NB;
+        const double wv = (c <= r) ? sh.fa.W[r][c] : 0.0;
+        if (c <= r)
+            C11[static_cast<long>(r) * np + c] = sh.fa.D[r][c];
+        W11[e] = wv;
+        Wt[(NB + r) * 128 + NB + c] = wv;
+    }
+    __syncthreads();
+    // W10 = -W11 (L10 W00): T = L10 W00 first (B = W00^T staged transposed)
+    blk::stage_block(sh.st.a, C10, np);
+    blk::stage_block_t(sh.st.b, W00, NB);
+    __syncthreads();
+    blk::mfma_nt(sh.st.a, sh.st.b, acc);
+    __syncthreads();
+    // T^T into the second stage half, W11 into the first
+    blk::for_each(acc, [&](int r, int c, double v) { sh.st.b[c * XS + r] = v; });
+    blk::stage_block(sh.st.a, W11, NB);
+    __syncthreads();
+    blk::mfma_nt(sh.st.a, sh.st.b, acc);
+    blk::for_each(acc, [&](int r, int c, double v) { Wt[(NB + r) * 128 + c] = -v; });
+    __syncthreads();
+}
+
+// Off-diagonal step: block row i of block column jb, L_ij = (C_ij - acc) W128_jj^T.
+__device__ __forceinline__ void off_block128(const BatchGP &g, int ib, int jb, Shared128 &sh)
+{
+    const long np = g.n_pad;
+    const long i0 = static_cast<long>(ib) * 128, j0 = static_cast<long>(jb) * 128;
+    const int rows = static_cast<int>(min(static_cast<long>(128), np - i0));
+    const int w = static_cast<int>(min(static_cast<long>(128), np - j0));
+    tile_update(g, i0, j0, rows, w, sh);
+    gt::Params p{};
+    p.A = g.L + i0 * np + j0;  // the updated block, read whole before the tile is written
+    p.lda = np;
+    p.B = g.W128 + static_cast<long>(jb) * 128 * 128;  // B(k, c) = W(c, k)
+    p.ldb = 128;
+    p.C = g.L + i0 * np + j0;
+    p.ldc = np;
+    p.M = rows;
+    p.N = w;
+    p.K = w;
+    p.set = 1;
+    gt::gemm_tile<false, true>(p, 0, 0, sh.g.As, sh.g.Bs);
+}
+
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_batch_diag_level128(const BatchGP *__restrict__ gps,
+                                                                                                           const int2 *__restrict__ items,
+                                                                                                           int jb, int *__restrict__ info)
+{
+    __shared__ Shared128 sh;
+    const int2 it = items[blockIdx.x];
+    const BatchGP g = gps[it.x];
+    diag_block128(g, it.x, jb, sh, info);
+}
+
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_batch_off_level128(const BatchGP *__restrict__ gps,
+                                                                                                          const int2 *__restrict__ items,
+                                                                                                          int jb)
+{
+    __shared__ Shared128 sh;
+    const int2 it = items[blockIdx.x];
+    const BatchGP g = gps[it.x];
+    off_block128(g, it.y, jb, sh);
+}
+
 __global__ void k_set_ints(int *p, int count, int value)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -756,6 +932,14 @@ void batch_level_step(hipStream_t s, BatchLevelStep step, const BatchGP *gps, co
             hipLaunchKernelGGL(k_batch_off_level<true>, grid, dim3(THREADS), 0, s, gps, items, jb);
         else
             hipLaunchKernelGGL(k_batch_off_level<false>, grid, dim3(THREADS), 0, s, gps, items, jb);
+        break;
+    case BatchLevelStep::Diagonal128:
+        require(!mixed, "batch: the 128-wide level path is FP64");
+        hipLaunchKernelGGL(k_batch_diag_level128, grid, dim3(THREADS), 0, s, gps, items, jb, info);
+        break;
+    case BatchLevelStep::OffDiagonal128:
+        require(!mixed, "batch: the 128-wide level path is FP64");
+        hipLaunchKernelGGL(k_batch_off_level128, grid, dim3(THREADS), 0, s, gps, items, jb);
         break;
     case BatchLevelStep::Finish:
         if (mixed)

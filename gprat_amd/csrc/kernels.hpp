@@ -173,6 +173,7 @@ struct BatchGP
     double *work;  // n_pad doubles, mixed precision only (refinement residual and correction)
     int n, n_pad, n_reg, pad_;
     double lengthscale, vertical, noise;
+    double *W128;  // 128-wide level path only: ceil(n_pad / 128) row-major 128x128 block-column inverses
 };
 // Persistent fit of `count` GPs (visited in `order`, largest first) by
 // `workgroups` workgroups; losses[g] gets GP g's loss, info[g] its first
@@ -192,11 +193,16 @@ void batch_info_reset(hipStream_t s, int *info, int count);
 // Diagonal (items: GPs with more than jb blocks) and OffDiagonal (items:
 // blocks ib > jb of those GPs), then Finish (items: every GP; alpha, its
 // FP64 refinement for mixed, the loss).
+// Diagonal128 / OffDiagonal128: the same steps on 128-wide block columns
+// (FP64 only; each GP needs W128; its rounding differs from the 64-wide
+// steps').
 enum class BatchLevelStep
 {
     Assemble,
     Diagonal,
     OffDiagonal,
+    Diagonal128,
+    OffDiagonal128,
     Finish
 };
 void batch_level_step(hipStream_t s, BatchLevelStep step, const BatchGP *gps, const int2 *items, int n_items, int jb, bool mixed,
