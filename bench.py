@@ -541,7 +541,8 @@ def serving_leg(capi, count=192, budget=200 << 30, slots=4, runs=3):
         "req_s_if_all_work_at_fp64_peak": len(reqs) / (work / (FP64_MFMA_PEAK_TFLOPS * 1e12)),
         "req_s_if_all_work_at_fp32_peak": fp32_bound,
         "saturated_fraction_of_fp32_peak_bound": (len(reqs) / sat_total) / fp32_bound,
-        "server_stats": {k: stats[k] for k in ("batched_fits", "batch_launches", "evictions", "refits")},
+        "server_stats": {k: stats[k] for k in ("batched_fits", "batch_launches", "evictions", "refits",
+                                                        "cache_flushes", "borrowed_blocks")},
         "workload": "70% fits n log-uniform [256, 32768] (n_reg 8), 30% predicts m uniform [1, 4096] on earlier models",
     }
 
@@ -575,6 +576,8 @@ def main():
     ap.add_argument("--chol-n", type=int, default=32768, help="size of the Cholesky-rate leg (0 skips it)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched many-GP leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop leg")
+    ap.add_argument("--only", choices=("batched", "serving"), default=None,
+                    help="run one leg alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this backend runs on one GPU; the GP path has no multi-GPU mode in this build "
@@ -583,7 +586,15 @@ def main():
 
     from gprat_amd import capi
 
+    if args.only == "serving":
+        print(json.dumps({"serving": serving_leg(capi)}), flush=True)
+        return
     ctx = capi.Context(0, 1)
+    if args.only == "batched":
+        out = batched_leg(capi, ctx)
+        ctx.close()
+        print(json.dumps({"batched_2048_gps": out}), flush=True)
+        return
     families = ("syrk_trailing", "residual", "trsm", "trsm_update", "panel", "panel_rows", "cholesky", "assemble", "solve", "mean",
                 "refine")
     head, fams, model, (mean, var, *_r) = request_leg(capi, ctx, HN, HT, HR, HM, HTHETA, capi.PRECISION_MIXED, args.steps,

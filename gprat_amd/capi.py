@@ -40,7 +40,8 @@ class ServerStats(C.Structure):
     """gprat_server_stats."""
     _fields_ = [(k, _LL) for k in ("submitted", "completed", "failed", "fits", "predicts", "batched_fits",
                                    "batch_launches", "evictions", "refits", "resident_models")] + \
-               [(k, _SZ) for k in ("resident_bytes", "reserved_bytes", "peak_bytes", "budget")]
+               [(k, _SZ) for k in ("resident_bytes", "reserved_bytes", "peak_bytes", "budget")] + \
+               [(k, _LL) for k in ("cache_flushes", "borrowed_blocks")]
 
 
 # name -> argtypes (restype int unless listed in _RESTYPE)

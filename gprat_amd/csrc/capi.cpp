@@ -825,6 +825,8 @@ extern "C" int gprat_server_get_stats(gprat_server *server, gprat_server_stats *
             out->reserved_bytes = s.reserved_bytes;
             out->peak_bytes = s.peak_bytes;
             out->budget = s.budget;
+            out->cache_flushes = s.cache_flushes;
+            out->borrowed_blocks = s.borrowed_blocks;
         });
 }
 

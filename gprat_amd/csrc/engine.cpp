@@ -33,7 +33,13 @@ DevicePool::~DevicePool()
 {
     hipSetDevice(device_);
     for (auto &kv : free_)
-        hipFree(kv.second);
+    {
+        if (kv.second.returned)
+            hipEventDestroy(kv.second.returned);
+        hipFree(kv.second.p);
+    }
+    for (auto e : spare_)
+        hipEventDestroy(e);
     for (auto &kv : live_)
         hipFree(kv.first);
 }
@@ -44,21 +50,50 @@ static std::size_t size_class(std::size_t bytes)
     return ((bytes + g - 1) / g) * g;
 }
 
+void DevicePool::share_with(const std::vector<DevicePool *> &siblings, hipStream_t stream)
+{
+    std::lock_guard<std::mutex> g(mu_);
+    siblings_.clear();
+    for (DevicePool *s : siblings)
+        if (s != this)
+            siblings_.push_back(s);
+    stream_ = stream;
+}
+
+bool DevicePool::take(std::size_t want, Cached &out, std::size_t &size)
+{
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = free_.lower_bound(want);
+    // reuse a cached block if it is not more than 25% larger than needed
+    if (it == free_.end() || it->first > want + want / 4)
+        return false;
+    out = it->second;
+    size = it->first;
+    cached_ -= it->first;
+    free_.erase(it);
+    return true;
+}
+
 void *DevicePool::get(std::size_t bytes)
 {
     const std::size_t want = size_class(bytes ? bytes : 1);
+    Cached b{};
+    std::size_t size = 0;
+    // own cache: this pool's stream order already covers the block's earlier uses
+    bool found = take(want, b, size);
+    bool foreign = false;
+    for (std::size_t i = 0; !found && i < siblings_.size(); i++)
+        foreign = found = siblings_[i]->take(want, b, size);
+    if (found)
     {
+        if (b.returned && foreign)
+            GP_HIP(hipEventSynchronize(b.returned));
         std::lock_guard<std::mutex> g(mu_);
-        auto it = free_.lower_bound(want);
-        // reuse a cached block if it is not more than 25% larger than needed
-        if (it != free_.end() && it->first <= want + want / 4)
-        {
-            void *p = it->second;
-            live_[p] = it->first;
-            cached_ -= it->first;
-            free_.erase(it);
-            return p;
-        }
+        if (b.returned)
+            spare_.push_back(b.returned);
+        live_[b.p] = size;
+        borrowed_ += foreign ? 1 : 0;
+        return b.p;
     }
     void *p = nullptr;
     if (hipMalloc(&p, want) != hipSuccess)
@@ -82,11 +117,37 @@ void DevicePool::put(void *p)
 {
     if (!p)
         return;
+    hipEvent_t ev = nullptr;
+    hipStream_t stream = nullptr;
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        auto it = live_.find(p);
+        if (it == live_.end())
+            return;
+        stream = stream_;
+        if (stream && !spare_.empty())
+        {
+            ev = spare_.back();
+            spare_.pop_back();
+        }
+    }
+    if (stream)
+    {
+        // the event must belong to the stream's device, whichever device
+        // the releasing thread has current
+        int cur = -1;
+        GP_HIP(hipGetDevice(&cur));
+        if (cur != device_)
+            GP_HIP(hipSetDevice(device_));
+        if (!ev)
+            GP_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        GP_HIP(hipEventRecord(ev, stream));
+        if (cur != device_)
+            GP_HIP(hipSetDevice(cur));
+    }
     std::lock_guard<std::mutex> g(mu_);
     auto it = live_.find(p);
-    if (it == live_.end())
-        return;
-    free_.emplace(it->second, p);
+    free_.emplace(it->second, Cached{ p, ev });
     cached_ += it->second;
     live_.erase(it);
 }
@@ -98,7 +159,7 @@ void DevicePool::trim(std::size_t keep)
     // the blocks leave the cache under the lock (largest first); the wait
     // for kernels that may still use them and the frees happen after it is
     // released, so put() and get() from other threads never wait on the device
-    std::vector<void *> drop;
+    std::vector<Cached> drop;
     {
         std::lock_guard<std::mutex> g(mu_);
         while (cached_ > keep && !free_.empty())
@@ -113,8 +174,12 @@ void DevicePool::trim(std::size_t keep)
         return;
     hipSetDevice(device_);
     hipDeviceSynchronize();
-    for (void *p : drop)
-        hipFree(p);
+    for (const Cached &b : drop)
+    {
+        if (b.returned)
+            hipEventDestroy(b.returned);
+        hipFree(b.p);
+    }
 }
 
 template <class T>

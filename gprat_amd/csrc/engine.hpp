@@ -37,13 +37,40 @@ class DevicePool
         std::lock_guard<std::mutex> g(mu_);
         return cached_;
     }
+    // Pools of one owner (the server's slots) may take each other's cached
+    // blocks when their own cache has none that fits, instead of allocating
+    // (and, when the device is full, freeing every cache, which waits for
+    // the whole device once per freed block). Each pool's context must run
+    // on the one stream `stream`: a block returned to a sharing pool carries
+    // an event recorded there, and a sibling that takes it waits for that
+    // event on the host first, so every use of the block in its former
+    // owner's stream order has completed before the new owner touches it.
+    void share_with(const std::vector<DevicePool *> &siblings, hipStream_t stream);
+    // Blocks taken from siblings' caches so far.
+    long long borrowed() const
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        return borrowed_;
+    }
 
   private:
+    struct Cached
+    {
+        void *p;
+        hipEvent_t returned;  // recorded on the owner's stream at put (sharing pools), else null
+    };
+    // Removes a cached block of size class in [want, want + 25%] (own lock).
+    bool take(std::size_t want, Cached &out, std::size_t &size);
+
     mutable std::mutex mu_;
     int device_;
-    std::multimap<std::size_t, void *> free_;
+    std::multimap<std::size_t, Cached> free_;
     std::map<void *, std::size_t> live_;
     std::size_t cached_ = 0;
+    std::vector<DevicePool *> siblings_;
+    hipStream_t stream_ = nullptr;
+    std::vector<hipEvent_t> spare_;  // events of blocks reused by this pool
+    long long borrowed_ = 0;
 };
 
 struct Context;

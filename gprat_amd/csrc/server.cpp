@@ -42,17 +42,26 @@ Server::Server(const ServerConfig &cfg) : cfg_(cfg)
     if (cfg_.budget == 0)
         cfg_.budget = total_b / 10 * 7;
     // Blocks cached by the slots' pools live outside the budget and are kept
-    // (freeing synchronises the whole device): when an allocation fails,
-    // every slot's cache is freed and the allocation tried again.
+    // (freeing synchronises the whole device). A slot whose own cache has no
+    // block of the right size takes one from another slot's cache (each
+    // slot's context runs on one stream); only when the device is full is
+    // every cache freed and the allocation tried again.
     st_.budget = cfg_.budget;
     for (int s = 0; s < cfg_.slots; s++)
         ctx_.push_back(std::make_unique<Context>(cfg_.device, 1));
+    std::vector<DevicePool *> pools;
     for (auto &c : ctx_)
+        pools.push_back(&c->pool);
+    for (auto &c : ctx_)
+    {
+        c->pool.share_with(pools, c->main);
         c->pool.on_out_of_memory = [this]
         {
+            cache_flushes_++;
             for (auto &o : ctx_)
                 o->pool.trim(0);
         };
+    }
     for (int s = 0; s < cfg_.slots; s++)
         workers_.emplace_back([this, s] { worker(s); });
 }
@@ -171,6 +180,9 @@ ServerStats Server::stats()
 {
     std::lock_guard<std::mutex> g(mu_);
     ServerStats s = st_;
+    s.cache_flushes = cache_flushes_.load();
+    for (const auto &c : ctx_)
+        s.borrowed_blocks += c->pool.borrowed();
     s.resident_bytes = resident_;
     s.reserved_bytes = reserved_;
     s.resident_models = 0;

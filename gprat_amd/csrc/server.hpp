@@ -13,6 +13,7 @@
 // not depend on what else is in flight.
 #pragma once
 
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -60,6 +61,8 @@ struct ServerStats
     long long evictions = 0, refits = 0;
     long long resident_models = 0;
     std::size_t resident_bytes = 0, reserved_bytes = 0, peak_bytes = 0, budget = 0;
+    long long cache_flushes = 0;    // device full: every slot's cached blocks freed
+    long long borrowed_blocks = 0;  // cached blocks one slot took from another's cache
 };
 
 class Server
@@ -154,6 +157,7 @@ class Server
     int heavy_in_flight_ = 0;
     bool stop_ = false;
     ServerStats st_;
+    std::atomic<long long> cache_flushes_{ 0 };
 };
 
 }  // namespace gpamd

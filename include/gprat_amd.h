@@ -371,6 +371,9 @@ typedef struct gprat_server_stats
     long long fits, predicts, batched_fits, batch_launches;
     long long evictions, refits, resident_models;
     size_t resident_bytes, reserved_bytes, peak_bytes, budget;
+    /* device allocations that failed and freed every slot's cached blocks;
+     * cached blocks one slot reused from another slot's cache */
+    long long cache_flushes, borrowed_blocks;
 } gprat_server_stats;
 
 int gprat_server_create(const gprat_server_config *config, gprat_server **out);

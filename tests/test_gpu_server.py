@@ -87,6 +87,36 @@ def test_same_alone_and_in_the_mix(precision):
     print(f"precision {precision}: max difference alone vs mix {worst:.1e}; {stats}")
 
 
+def test_cached_blocks_move_between_slots():
+    """The slots reuse each other's cached blocks (after waiting for the
+    block's last use on its former slot): the mix, run again and again in
+    other orders on one server with every model released in between, gives
+    bit-identical results on the same path and never has to flush the
+    caches."""
+    from gprat_amd import capi
+    orders = [[3, 0, 6, 1, 5, 2, 4], [6, 5, 4, 3, 2, 1, 0], [0, 6, 2, 5, 4, 1, 3], [5, 3, 1, 6, 0, 4, 2]]
+    with capi.Server(0, slots=4, precision=capi.PRECISION_MIXED) as srv:
+        runs = []
+        for order in orders:
+            out = _run_mix(srv, order, waves=True)
+            for i in order:
+                srv.release_model(out[("pred", i)]["model"])
+            runs.append(out)
+        stats = srv.stats()
+    assert stats["failed"] == 0 and stats["cache_flushes"] == 0, stats
+    first = runs[0]
+    for out in runs[1:]:
+        for key, r in first.items():
+            g = out[key]
+            if (first[("fit", key[1])]["batched"] > 0) != (out[("fit", key[1])]["batched"] > 0):
+                continue
+            if key[0] == "fit":
+                assert g["loss"] == r["loss"], key
+            else:
+                assert np.array_equal(g["mean"], r["mean"]) and np.array_equal(g["var"], r["var"]), key
+    print(f"blocks borrowed between slots: {stats['borrowed_blocks']}; {stats}")
+
+
 def test_small_fits_waiting_together_share_a_launch():
     """Many small fits queued at once run as batched launches; their
     results agree with the single-GP path to 1e-10."""
