@@ -166,10 +166,13 @@ LevelPlan level_plan(const std::vector<int> &ids, const std::vector<int> &nb64, 
     int maxb = 0;
     for (int g : ids)
         maxb = std::max(maxb, nb[g]);
+    // the 64-wide steps generate each block of K when they reduce it; the
+    // 128-wide ones update the factor in place, so K is assembled first
     p.assemble_off = 0;
-    for (int g : ids)
-        for (int i = 0; i < nb64[g]; i++)
-            p.items.push_back(make_int2(g, i));
+    if (width == 128)
+        for (int g : ids)
+            for (int i = 0; i < nb64[g]; i++)
+                p.items.push_back(make_int2(g, i));
     p.assemble_n = static_cast<long>(p.items.size());
     for (int jb = 0; jb < maxb; jb++)
     {

@@ -6,7 +6,8 @@
 // size.
 //
 // The factorisation is left-looking over 64-wide block columns: block
-// (i, j) of column j accumulates sum_{k<j} L_ik L_jk^T on FP64 MFMA in
+// (i, j) of column j is generated from the series (no assembly pass: each
+// block of K is needed once), accumulates sum_{k<j} L_ik L_jk^T on FP64 MFMA in
 // registers, with the next pair of blocks already on its way from memory
 // (register prefetch) while the current pair is multiplied; the sum is then
 // subtracted from K_ij once and the block is either factored (i == j:
@@ -426,21 +427,93 @@ __device__ __forceinline__ void assemble_rows(const BatchGP &g, long r0, long r1
         }
 }
 
+// Block (ib, jb) of one GP's covariance, generated from its series into D
+// (64 rows, stride XS): the operations and their order of assemble_rows_w
+// (lane = column, its window in registers; the row wavefront-uniform, its
+// window through scalar loads), so every entry equals the assembled one.
+template <int R>
+__device__ __forceinline__ void gen_block_w(const BatchGP &g, int ib, int jb, double *D)
+{
+#pragma clang fp contract(off)
+    const double scale = -0.5 / (g.lengthscale * g.lengthscale);
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const double *x = g.x;
+    const long j = static_cast<long>(jb) * NB + lane;
+    const long jc = j < g.n ? j : g.n - 1;  // lanes past the end read a valid window and store the padding value
+    double xj[R];
+#pragma unroll
+    for (int k = 0; k < R; k++)
+        xj[k] = x[jc + k];
+    for (int rr = wv; rr < NB; rr += THREADS / 64)
+    {
+        const long i = static_cast<long>(ib) * NB + rr;
+        double val;
+        if (i < g.n && j < g.n)
+        {
+            double d = 0.0;
+#pragma unroll
+            for (int k = 0; k < R; k++)
+            {
+                const double t = x[i + k] - xj[k];
+                d += t * t;
+            }
+            val = g.vertical * exp(scale * d);
+            if (i == j)
+                val += g.noise;
+        }
+        else
+            val = (i == j) ? 1.0 : 0.0;
+        D[rr * XS + lane] = val;
+    }
+}
+
+__device__ __forceinline__ void gen_block(const BatchGP &g, int ib, int jb, double *D)
+{
+    if (g.n_reg == 4)
+        return gen_block_w<4>(g, ib, jb, D);
+    if (g.n_reg == 8)
+        return gen_block_w<8>(g, ib, jb, D);
+    if (g.n_reg == 16)
+        return gen_block_w<16>(g, ib, jb, D);
+    const double scale = -0.5 / (g.lengthscale * g.lengthscale);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const long j = static_cast<long>(jb) * NB + lane;
+    for (int rr = wv; rr < NB; rr += THREADS / 64)
+    {
+        const long i = static_cast<long>(ib) * NB + rr;
+        double val;
+        if (i < g.n && j < g.n)
+        {
+            val = g.vertical * exp(scale * window_sqdist(g.x, i, g.x, j, g.n_reg));
+            if (i == j)
+                val += g.noise;
+        }
+        else
+            val = (i == j) ? 1.0 : 0.0;
+        D[rr * XS + lane] = val;
+    }
+}
+
 // This lane's 16 entries of K_ij - sum_{k<jb} L_ik L_jk^T at the
 // accumulator positions of the product's precision (left-looking step of
-// block (ib, jb); the K_ij loads are in flight while the products run).
+// block (ib, jb)). K_ij is generated from the series here (each block of K
+// is reduced exactly once, so no pass assembles K first), through the
+// stage's first block.
 template <bool MIXED>
 __device__ __forceinline__ void reduced_block(const BatchGP &g, int ib, int jb, Shared &sh, double (&kij)[16])
 {
     const long np = g.n_pad;
-    const double *Cij = g.L + static_cast<long>(ib) * NB * np + jb * NB;
+    gen_block(g, ib, jb, sh.st.a);
+    __syncthreads();
 #pragma unroll
     for (int q = 0; q < 16; q++)
     {
         int r, c;
         acc_position<MIXED>(q, r, c);
-        kij[q] = Cij[static_cast<long>(r) * np + c];
+        kij[q] = sh.st.a[r * XS + c];
     }
+    __syncthreads();  // every entry is read before the stage is reused
     if constexpr (MIXED)
     {
         v4f acc[2][2];
@@ -653,8 +726,6 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
         const int gi = order[slot];
         const BatchGP g = gps[gi];
         const int nblk = static_cast<int>(g.n_pad / NB);
-        assemble_rows(g, 0, g.n_pad);
-        __syncthreads();
         for (int jb = 0; jb < nblk; jb++)
         {
             diag_block<MIXED>(g, gi, jb, sh, wjj, info);
