@@ -61,12 +61,14 @@ def outer_block(n_pad: int) -> int:
     return 256 if n_pad <= 4096 else (512 if n_pad < 12288 else 768)
 
 
-def trailing_update_flops(n_pad: int):
+def trailing_update_flops(n_pad: int, extra: int = 0):
     """Algorithmic flops of each trailing-update launch, in launch order:
     2 * K * (number of elements updated). Step k's launch applies panel k
     (K = its width) to the rows below the next diagonal block: the next
     panel's columns whole, the columns after it lower (the next diagonal
-    block itself is updated on the panel stream)."""
+    block itself is updated on the panel stream); with `extra` rows below
+    the matrix (the one-call prediction's test rows), also every column of
+    those rows right of the panel."""
     bo = outer_block(n_pad)
     out = []
     p0 = 0
@@ -75,10 +77,11 @@ def trailing_update_flops(n_pad: int):
         if pe >= n_pad:
             break
         q1 = min(n_pad, pe + bo)
-        if q1 < n_pad:
-            K = pe - p0
-            S = n_pad - q1
-            out.append(2.0 * K * (S * (q1 - pe) + S * (S + 1) / 2))
+        K = pe - p0
+        S = n_pad - q1
+        f = 2.0 * K * (S * (q1 - pe) + S * (S + 1) / 2 + extra * (n_pad - pe))
+        if q1 < n_pad or extra > 0:
+            out.append(f)
         p0 = pe
     return out
 
@@ -207,18 +210,25 @@ def cholesky_rate(capi, ctx, n=32768, repeats=2):
     return {"n": n, "ms_best": best, "ms_mean": float(np.mean(times)), "tflops": flops / (best * 1e-3) / 1e12}
 
 
-def request_leg(capi, ctx, n, T, r, m, theta, precision, steps, warmup=1, refine_variance=True, families=()):
+def request_leg(capi, ctx, n, T, r, m, theta, precision, steps, warmup=1, refine_variance=True, families=(), oneshot=False):
     """Times `steps` fit+predict requests (inputs resident in HBM): latency
     per request and, for `families`, the kernel-family stats of the timed
-    requests. Returns the timing and the last request's model (released by
-    the caller)."""
+    requests. A request is a resident model's fit and predict, or with
+    `oneshot` the one-call prediction (gprat_predict_with_uncertainty, FP64,
+    the reference's own call). Returns the timing and a model of the same
+    problem (fitted after the timed region; released by the caller)."""
     from gprat_amd import datagen
     x, y, z = datagen.gp_problem(n, m, r)
     dx, dy, dz = ctx.device_array(x), ctx.device_array(y), ctx.device_array(z)
     dmean, dvar = ctx.device_empty(m), ctx.device_empty(m)
 
     def step(keep=False):
+        if oneshot and not keep:
+            ctx.predict(dx, dy, dz, theta, T, n // T, 1, m, r, variance=True, out=(dmean, dvar))
+            return None
         mdl = capi.Model(ctx, dx, dy, n, r, theta, precision)
+        if oneshot:
+            return mdl
         if not refine_variance:
             mdl.set_variance_refinement(False)
         mdl.predict(dz, m, variance=True, out=(dmean, dvar))
@@ -669,19 +679,38 @@ def main():
         result["mixed_unrefined_variance_n32k"] = unr
         result["speedup_mixed_refined_vs_fp64"] = f64["ms_per_request"] / head["ms_per_request"]
         result["speedup_mixed_unrefined_vs_fp64"] = f64["ms_per_request"] / unr["ms_per_request"]
-        c2fams = ("syrk_trailing", "cholesky", "panel", "panel_rows", "assemble", "solve", "trsm", "mean")
-        c2, c2f, mdl, _ = request_leg(capi, ctx, N_TILES * TILE, N_TILES, N_REG, N_TEST, THETA, capi.PRECISION_FP64, 10, warmup=2,
-                                      families=c2fams)
+        # C2 through the reference's own call (gprat_predict_with_uncertainty:
+        # the test rows are factored with K), then through a resident model
+        n16 = N_TILES * TILE
+        c2fams = ("syrk_trailing", "cholesky", "panel", "panel_rows", "assemble", "solve", "variance_rows", "mean")
+        c2, c2f, mdl, (m16, v16, *_r) = request_leg(capi, ctx, n16, N_TILES, N_REG, N_TEST, THETA, capi.PRECISION_FP64, 10, warmup=2,
+                                                    families=c2fams, oneshot=True)
+        c2["api"] = "gprat_predict_with_uncertainty (one call: fit + predict, test rows factored with K)"
         mdl.release()
-        tr16 = trailing_update_flops(N_TILES * TILE)
+        m_pad16 = ((N_TEST + 63) // 64) * 64
+        tr16 = trailing_update_flops(n16, m_pad16)
         c2["roofline_trailing_update"] = roofline_of(
-            "k_gemm_f64<true,false,5> (FP64 trailing update)", sum(tr16) / len(tr16), c2f["syrk_trailing"],
-            FP64_MFMA_PEAK_TFLOPS, "trailing_update_n16384_traffic.md")
-        chol_ms = c2f["cholesky"][1] / max(c2f["cholesky"][0], 1)
-        c2["cholesky_ms"] = chol_ms
-        c2["cholesky_tflops"] = (N_TILES * TILE) ** 3 / 3.0 / (chol_ms * 1e-3) / 1e12
+            "k_gemm_f64<true,false,5> (FP64 trailing update, test rows included)", sum(tr16) / len(tr16), c2f["syrk_trailing"],
+            FP64_MFMA_PEAK_TFLOPS, "trailing_update_oneshot_n16384_traffic.md")
+        fact_ms = c2f["cholesky"][1] / max(c2f["cholesky"][0], 1)
+        c2["factorisation_with_test_rows_ms"] = fact_ms
+        c2["factorisation_with_test_rows_tflops"] = (n16 ** 3 / 3.0 + n16 ** 2 * N_TEST) / (fact_ms * 1e-3) / 1e12
         c2["phase_ms_per_request"] = {k: v[1] / 10 for k, v in c2f.items()}
         result["fp64_n16k"] = c2
+        c2fams_r = ("syrk_trailing", "cholesky", "panel", "panel_rows", "assemble", "solve", "trsm", "mean")
+        c2r, c2rf, mdl, _ = request_leg(capi, ctx, n16, N_TILES, N_REG, N_TEST, THETA, capi.PRECISION_FP64, 10, warmup=2,
+                                        families=c2fams_r)
+        mdl.release()
+        chol_ms = c2rf["cholesky"][1] / max(c2rf["cholesky"][0], 1)
+        c2r["api"] = "resident model: gprat_model_fit + gprat_model_predict"
+        c2r["cholesky_ms"] = chol_ms
+        c2r["cholesky_tflops"] = n16 ** 3 / 3.0 / (chol_ms * 1e-3) / 1e12
+        tr16r = trailing_update_flops(n16)
+        c2r["roofline_trailing_update"] = roofline_of(
+            "k_gemm_f64<true,false,5> (FP64 trailing update)", sum(tr16r) / len(tr16r), c2rf["syrk_trailing"],
+            FP64_MFMA_PEAK_TFLOPS, "trailing_update_n16384_traffic.md")
+        c2r["phase_ms_per_request"] = {k: v[1] / 10 for k, v in c2rf.items()}
+        result["fp64_n16k_resident"] = c2r
         result["optimize_n16k"] = optimize_leg()
         calib = ctx.calibrate()
         result["device_calibration"] = calib

@@ -179,19 +179,26 @@ class Context:
         return out.reshape(-1, n_tile_size, n_tile_size)
 
     def predict(self, train_in, train_out, test_in, theta, n_tiles, n_tile_size, m_tiles, m_tile_size, n_reg,
-                variance=False, full_cov=False):
-        x, y, z, t = _arr(train_in), _arr(train_out), _arr(test_in), _arr(theta)
+                variance=False, full_cov=False, out=None):
+        """Fit and predict in one call. Inputs may be host arrays or
+        DeviceArrays; with `out` = (mean, var) DeviceArrays the results stay
+        on the device (and those are returned)."""
+        (px, nx, _kx), (py, ny, _ky), (pz, nz, _kz) = _ptr_len(train_in), _ptr_len(train_out), _ptr_len(test_in)
+        t = _arr(theta)
         m = m_tiles * m_tile_size
-        mean = np.empty(m)
-        var = np.empty(m)
-        args = (self.handle, _dptr(x), x.size, _dptr(y), y.size, _dptr(z), z.size, _dptr(t), n_tiles, n_tile_size,
-                m_tiles, m_tile_size, n_reg)
-        if full_cov:
-            _check(lib().gprat_predict_with_full_cov(*args, _dptr(mean), _dptr(var)))
-        elif variance:
-            _check(lib().gprat_predict_with_uncertainty(*args, _dptr(mean), _dptr(var)))
+        if out is not None:
+            mean, var = out
+            pm, pv = mean.as_dptr(), var.as_dptr()
         else:
-            _check(lib().gprat_predict(*args, _dptr(mean)))
+            mean, var = np.empty(m), np.empty(m)
+            pm, pv = _dptr(mean), _dptr(var)
+        args = (self.handle, px, nx, py, ny, pz, nz, _dptr(t), n_tiles, n_tile_size, m_tiles, m_tile_size, n_reg)
+        if full_cov:
+            _check(lib().gprat_predict_with_full_cov(*args, pm, pv))
+        elif variance:
+            _check(lib().gprat_predict_with_uncertainty(*args, pm, pv))
+        else:
+            _check(lib().gprat_predict(*args, pm))
             return mean
         return mean, var
 

@@ -454,9 +454,12 @@ void cholesky_fp64(Context &c, double *A, long lda, int n_pad, double *Winv) { c
 
 int panel_width(int n_pad) { return outer_block(n_pad); }
 
-void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mixed, double *Wpanel)
+void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mixed, double *Wpanel, int extra)
 {
     require(n_pad % 128 == 0 && lda >= n_pad && lda % 2 == 0, "cholesky: bad layout");
+    require(extra >= 0 && extra % 64 == 0, "cholesky: extra rows must come in 64s");
+    // rows of L and the rows below it that ride along (right-hand sides)
+    const long rows = static_cast<long>(n_pad) + extra;
     const bool ahead = use_lookahead() && c.panel != c.main;
     hipStream_t T = ahead ? c.trail : c.main, P = ahead ? c.panel : c.main;
     const int cap = ahead ? std::max(1, 2 * c.cus - trailing_reserve()) : 0;
@@ -468,9 +471,9 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
     DevBuf<float> p32;
     DevBuf<double> p64;
     if (mixed)
-        p32 = DevBuf<float>(c, static_cast<std::size_t>(Bo) * n_pad);
+        p32 = DevBuf<float>(c, static_cast<std::size_t>(Bo) * rows);
     else
-        p64 = DevBuf<double>(c, static_cast<std::size_t>(Bo) * n_pad);
+        p64 = DevBuf<double>(c, static_cast<std::size_t>(Bo) * rows);
     // inverses of the panels' diagonal blocks: kept (Wpanel, block kb at
     // kb * Bo * Bo) or two scratch blocks used by alternate panels (the
     // panel stream factors panel k+2 only after the trailing stream's last
@@ -478,7 +481,7 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
     const int chain_wgs = ahead ? std::max(1, trailing_reserve()) : 4096, tail_wgs = ahead ? 64 : 4096;
     const int head_wgs = ahead ? head_workgroups() : 4096;
     DevBuf<double> Wscratch, tmp(c, static_cast<std::size_t>(std::max(1L, c.panel_plan(Bo / 64, chain_wgs, tail_wgs).tmp_doubles)));
-    DevBuf<double> Xs(c, static_cast<std::size_t>(Bo) * n_pad);  // the panel's rows below, before the copy back
+    DevBuf<double> Xs(c, static_cast<std::size_t>(Bo) * rows);  // the panel's rows below, before the copy back
     DevBuf<double> Xh(c, static_cast<std::size_t>(Bo) * Bo);     // the next diagonal block's rows of the panel, solved
     DevBuf<int> queues(c, static_cast<std::size_t>(nblk));          // tile counters of the capped trailing updates, one per step
     if (!Wpanel)
@@ -556,14 +559,14 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
     auto rows_below = [&](int p0, int pe, hipEvent_t solved)
     {
         Timed t(c, T, "panel_rows");
-        const int w = pe - p0, R = n_pad - pe;
+        const int w = pe - p0, R = static_cast<int>(rows - pe);
         double *panel = A + static_cast<long>(pe) * lda + p0;
         k::rows_solve(T, A, lda, pe, p0, R, w, Wblock(p0), Bo, Xs.get(), w);
         wait(T, solved, P);
         if (mixed)
-            k::panel_to_f32t(T, Xs.get(), w, 0, 0, R, w, p32.get(), n_pad, panel, lda);
+            k::panel_to_f32t(T, Xs.get(), w, 0, 0, R, w, p32.get(), rows, panel, lda);
         else
-            k::panel_to_f64t(T, Xs.get(), w, 0, 0, R, w, p64.get(), n_pad, panel, lda);
+            k::panel_to_f64t(T, Xs.get(), w, 0, 0, R, w, p64.get(), rows, panel, lda);
     };
     // Panel [p0, pe)'s update of everything below the next diagonal block:
     // rows q1 .., columns pe .. (the next panel's columns whole, the rest
@@ -572,11 +575,11 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
     {
         Timed t(c, T, "syrk_trailing");
         double *C = A + static_cast<long>(q1) * lda + pe;
-        const int M = n_pad - q1, N = n_pad - pe, K = pe - p0;
+        const int M = static_cast<int>(rows - q1), N = n_pad - pe, K = pe - p0;
         if (mixed)
-            k::gemm_f32_trailing(T, p32.get(), n_pad, q1 - pe, 0, C, lda, M, N, K, q1 - pe, cap, queues.get() + p0 / Bo);
+            k::gemm_f32_trailing(T, p32.get(), rows, q1 - pe, 0, C, lda, M, N, K, q1 - pe, cap, queues.get() + p0 / Bo);
         else
-            trailing_kmajor(T, p64.get(), n_pad, q1 - pe, 0, q1 - pe, C, lda, M, N, K, cap, queues.get() + p0 / Bo);
+            trailing_kmajor(T, p64.get(), rows, q1 - pe, 0, q1 - pe, C, lda, M, N, K, cap, queues.get() + p0 / Bo);
     };
     {
         Timed whole(c, c.main, "cholesky");
@@ -599,11 +602,17 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
             wait(T, factored, P);
             rows_below(p0, pe, solved);
             trailed = nullptr;
-            if (q1 < n_pad)
+            if (q1 < rows)
             {
                 trailing(p0, pe, q1);
                 trailed = record(T);
             }
+        }
+        if (extra > 0)
+        {
+            // the rows below, against the last diagonal block
+            wait(T, record(P), P);
+            rows_below((nblk - 1) * Bo, n_pad, nullptr);
         }
         wait(c.main, record(T), T);
         wait(c.main, record(P), P);
@@ -1358,6 +1367,65 @@ void predict_mean_noalloc(Model &m, const double *z, std::size_t z_len, int mt, 
     k::predict_mean(c.main, m.x.get(), dz, m.alpha.get(), m.r, m.sek, m.n, mt, partial, splits, dmean);
     GP_HIP(hipMemcpyAsync(mean, dmean, sizeof(double) * mt, hipMemcpyDefault, c.main));
     GP_HIP(hipStreamSynchronize(c.main));
+}
+
+// The rows of K* below K in one matrix, factored with it: on return they
+// hold V^T = K* L^-T (the trailing updates and panel-row solves of every
+// step carry the variance's forward solve, and fill the steps where the
+// diagonal-block chain leaves the device idle), and var = v - their sums of
+// squares. The results agree with fit() and predict() to rounding, not bit
+// for bit: the test rows enlarge the late steps' trailing updates past the
+// workgroup cap, whose split K halves then round some of L's elements
+// differently, and the variance sums its squares in another order.
+void predict_with_uncertainty_once(Context &c, const double *x, std::size_t x_len, const double *y, std::size_t y_len, int n, int r,
+                                   const double theta[3], const double *z, std::size_t z_len, int mt, double *mean, double *var)
+{
+    require(n >= 1 && r >= 1 && mt >= 1, "predict: need training and test points");
+    require(x != nullptr && x_len >= static_cast<std::size_t>(n) + r - 1, "training input must hold n + n_reg - 1 values");
+    require(z != nullptr && z_len >= static_cast<std::size_t>(mt) + r - 1, "predict: test input must hold m + n_reg - 1 values");
+    require(theta != nullptr && mean != nullptr && var != nullptr, "predict: null argument");
+    c.make_current();
+    Model m;
+    m.ctx = &c;
+    m.n = n;
+    m.r = r;
+    m.n_pad = static_cast<int>(padded(n));
+    m.lda = leading_dim(m.n_pad);
+    m.sek = SEK{ theta[0], theta[1], theta[2] };
+    m.precision = GPRAT_PRECISION_FP64;
+    const int m_pad = static_cast<int>(round_up(mt, 64));
+    const std::size_t xl = static_cast<std::size_t>(n) + r - 1, zl = static_cast<std::size_t>(mt) + r - 1;
+    m.x = DevBuf<double>(c, xl);
+    m.L = DevBuf<double>(c, (static_cast<std::size_t>(m.n_pad) + m_pad) * m.lda);
+    m.Winv = DevBuf<double>(c, static_cast<std::size_t>(m.n_pad) * 64);
+    m.wblock = panel_width(m.n_pad);
+    m.Wpanel = DevBuf<double>(c, static_cast<std::size_t>(ceil_div(m.n_pad, m.wblock)) * m.wblock * m.wblock);
+    DevBuf<double> dz(c, zl);
+    upload(c, m.x.get(), x, xl);
+    upload(c, dz.get(), z, zl);
+    double *rows = m.L.get() + static_cast<long>(m.n_pad) * m.lda;
+    {
+        Timed t(c, c.main, "assemble");
+        k::assemble_lower(c.main, m.x.get(), r, m.sek, m.L.get(), m.lda, n, m.n_pad);
+        k::assemble_cross_rows(c.main, m.x.get(), dz.get(), r, m.sek, rows, m.lda, n, m.n_pad, mt, m_pad);
+    }
+    cholesky(c, m.L.get(), m.lda, m.n_pad, m.Winv.get(), false, m.Wpanel.get(), m_pad);
+    attach_targets(m, y, y_len);
+    const int splits = k::mean_splits(m.n, mt);
+    DevBuf<double> partial(c, static_cast<std::size_t>(splits) * mt);
+    DevBuf<double> dmean(c, mt), colsq(c, mt), dvar(c, mt);
+    {
+        Timed t(c, c.main, "mean");
+        k::predict_mean(c.main, m.x.get(), dz.get(), m.alpha.get(), m.r, m.sek, m.n, mt, partial.get(), splits, dmean.get());
+    }
+    {
+        Timed t(c, c.main, "variance_rows");
+        k::row_sumsq(c.main, rows, m.lda, mt, m.n_pad, colsq.get());
+        k::variance_finish(c.main, colsq.get(), m.sek.vertical, mt, dvar.get());
+    }
+    GP_HIP(hipMemcpyAsync(mean, dmean.get(), sizeof(double) * mt, hipMemcpyDefault, c.main));
+    GP_HIP(hipMemcpyAsync(var, dvar.get(), sizeof(double) * mt, hipMemcpyDefault, c.main));
+    c.sync();
 }
 
 FactorView view(const Model &m)

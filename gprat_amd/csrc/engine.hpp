@@ -264,7 +264,10 @@ void cholesky_fp64(Context &c, double *A, long lda, int n_pad, double *Winv);
 // mixed: FP32 matrix-core trailing updates (FP64 panels and diagonal blocks).
 // Wpanel (optional): receives the k-major inverses of the panels' diagonal
 // blocks, panel_width(n_pad) wide, block k at Wpanel + k * width^2.
-void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mixed, double *Wpanel = nullptr);
+// extra: rows below the n_pad x n_pad matrix (A holds n_pad + extra rows)
+// that the factorisation treats as rows of L below it: on return they hold
+// B L^-T for the B they held (the forward solve of their transposes).
+void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mixed, double *Wpanel = nullptr, int extra = 0);
 int panel_width(int n_pad);
 void refine_alpha(Model &m, double *work);
 void solve_spd(Context &c, const double *L, long lda, const double *Winv, int n_pad, const double *y, double *work, double *out,
@@ -285,6 +288,9 @@ void fit(Context &c,
          const double theta[3],
          int precision);
 void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean, double *var);
+// Fit and predict in one call (FP64): mean and latent variance at mt test points.
+void predict_with_uncertainty_once(Context &c, const double *x, std::size_t x_len, const double *y, std::size_t y_len, int n, int r,
+                                   const double theta[3], const double *z, std::size_t z_len, int mt, double *mean, double *var);
 void predict_view(const FactorView &m, const double *z, std::size_t z_len, int mt, double *mean, double *var);
 // Predictive mean through the fixed per-GPU scratch: no device allocation.
 void predict_mean_noalloc(Model &m, const double *z, std::size_t z_len, int mt, double *mean);

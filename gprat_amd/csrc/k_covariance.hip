@@ -475,6 +475,111 @@ __global__ __launch_bounds__(THREADS) void k_assemble_upper(const double *__rest
     }
 }
 
+// Rows of K* (test point j by training point i) stored row-major, as the
+// rows below the covariance that the one-call prediction factors with it:
+// lane = training column (its window in registers), the test row
+// wavefront-uniform (its window through scalar loads). Same operations in
+// the same order as the transposed assembly ((test - train), no
+// contraction), so the entries equal it bit for bit.
+template <int R>
+__global__ __launch_bounds__(THREADS) void k_assemble_cross_rows_w(const double *__restrict__ x,
+                                                                    const double *__restrict__ z,
+                                                                    double scale,
+                                                                    double v,
+                                                                    double *__restrict__ S,
+                                                                    long lds,
+                                                                    int n,
+                                                                    int m)
+{
+#pragma clang fp contract(off)
+    const int ti = blockIdx.x, tj = blockIdx.y;
+    const int c = threadIdx.x % TB;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / TB);
+    const int i = ti * TB + c;  // training index
+    const int ic = i < n ? i : n - 1;
+    double xi[R];
+#pragma unroll
+    for (int k = 0; k < R; k++)
+        xi[k] = x[ic + k];
+    for (int rr = w; rr < TB; rr += THREADS / TB)
+    {
+        const int j = tj * TB + rr;  // test index
+        double val = 0.0;
+        if (i < n && j < m)
+        {
+            double d = 0.0;
+#pragma unroll
+            for (int k = 0; k < R; k++)
+            {
+                const double t = z[j + k] - xi[k];
+                d += t * t;
+            }
+            val = se(scale, v, d);
+        }
+        S[static_cast<long>(j) * lds + i] = val;
+    }
+}
+
+__global__ __launch_bounds__(THREADS) void k_assemble_cross_rows(const double *__restrict__ x,
+                                                                  const double *__restrict__ z,
+                                                                  int r,
+                                                                  double scale,
+                                                                  double v,
+                                                                  double *__restrict__ S,
+                                                                  long lds,
+                                                                  int n,
+                                                                  int m)
+{
+    const int ti = blockIdx.x, tj = blockIdx.y;
+    const long i = static_cast<long>(ti) * TB + threadIdx.x % TB;
+    for (int rr = threadIdx.x / TB; rr < TB; rr += THREADS / TB)
+    {
+        const long j = static_cast<long>(tj) * TB + rr;
+        double val = 0.0;
+        if (i < n && j < m)
+            val = se(scale, v, window_sqdist(z, j, x, i, r));
+        S[j * lds + i] = val;
+    }
+}
+
+// out[j] = sum over i < cols of S[j][i]^2, one workgroup per row, in a fixed order.
+__global__ __launch_bounds__(THREADS) void k_row_sumsq(const double *__restrict__ S, long lds, int cols, double *__restrict__ out)
+{
+    __shared__ double red[THREADS];
+    const double *row = S + static_cast<long>(blockIdx.x) * lds;
+    const int tid = threadIdx.x;
+    double s = 0.0;
+    for (int i = 2 * tid; i < cols; i += 2 * THREADS)
+    {
+        const double a = row[i], b = row[i + 1];
+        s = fma(a, a, s);
+        s = fma(b, b, s);
+    }
+    red[tid] = s;
+    __syncthreads();
+    for (int h = THREADS / 2; h > 0; h >>= 1)
+    {
+        if (tid < h)
+            red[tid] += red[tid + h];
+        __syncthreads();
+    }
+    if (tid == 0)
+        out[blockIdx.x] = red[0];
+}
+
+template <int R>
+void launch_cross_rows_w(hipStream_t s, dim3 grid, const double *x, const double *z, double scale, double v, double *S, long lds, int n, int m)
+{
+    hipLaunchKernelGGL((k_assemble_cross_rows_w<R>), grid, dim3(THREADS), 0, s, x, z, scale, v, S, lds, n, m);
+}
+using CrossRowsLaunch = void (*)(hipStream_t, dim3, const double *, const double *, double, double, double *, long, int, int);
+constexpr CrossRowsLaunch cross_rows_w_table[16] = {
+    launch_cross_rows_w<1>,  launch_cross_rows_w<2>,  launch_cross_rows_w<3>,  launch_cross_rows_w<4>,
+    launch_cross_rows_w<5>,  launch_cross_rows_w<6>,  launch_cross_rows_w<7>,  launch_cross_rows_w<8>,
+    launch_cross_rows_w<9>,  launch_cross_rows_w<10>, launch_cross_rows_w<11>, launch_cross_rows_w<12>,
+    launch_cross_rows_w<13>, launch_cross_rows_w<14>, launch_cross_rows_w<15>, launch_cross_rows_w<16>,
+};
+
 template <class T, int R>
 void launch_cross_w(hipStream_t s, dim3 grid, const double *x, const double *z, double scale, double v, T *B, long ldb, int n, int m)
 {
@@ -530,6 +635,27 @@ void assemble_cross_t(hipStream_t s, const double *x, const double *z, int r, SE
 void assemble_cross_t_f32(hipStream_t s, const double *x, const double *z, int r, SEK sek, float *B, long ldb, int n, int n_pad, int m, int m_pad)
 {
     assemble_cross_t_impl<float>(s, x, z, r, sek, B, ldb, n, n_pad, m, m_pad);
+}
+
+void assemble_cross_rows(hipStream_t s, const double *x, const double *z, int r, SEK sek, double *S, long lds, int n, int n_pad, int m,
+                         int m_pad)
+{
+    require(n_pad % TB == 0 && m_pad % TB == 0 && n <= n_pad && m <= m_pad && lds >= n_pad && r >= 1, "assemble_cross_rows: bad shape");
+    dim3 grid(n_pad / TB, m_pad / TB);
+    if (r <= 16 && n >= 1 && m >= 1 && n_pad < (1 << 30) - 16 && m_pad < (1 << 30) - 16)
+        cross_rows_w_table[r - 1](s, grid, x, z, sek.scale(), sek.vertical, S, lds, n, m);
+    else
+        hipLaunchKernelGGL(k_assemble_cross_rows, grid, dim3(THREADS), 0, s, x, z, r, sek.scale(), sek.vertical, S, lds, n, m);
+    GP_CHECK_LAUNCH();
+}
+
+void row_sumsq(hipStream_t s, const double *S, long lds, int rows, int cols, double *out)
+{
+    require(rows >= 0 && cols >= 0 && cols % 2 == 0 && lds % 2 == 0, "row_sumsq: bad shape");
+    if (rows == 0)
+        return;
+    hipLaunchKernelGGL(k_row_sumsq, dim3(rows), dim3(THREADS), 0, s, S, lds, cols, out);
+    GP_CHECK_LAUNCH();
 }
 
 int mean_splits(int n, int m)

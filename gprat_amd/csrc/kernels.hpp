@@ -20,6 +20,13 @@ void assemble_lower(hipStream_t s, const double *x, int r, SEK sek, double *A, l
 void assemble_upper(hipStream_t s, const double *x, int r, SEK sek, double *A, long lda, int n, int n_pad);
 
 // B = K*^T (n_pad x m_pad, ldb): B[i][j] = k(test j, train i); zero outside n x m.
+// Rows of K* (m x n, row-major at S with row stride lds): entry (j, i) for
+// test point j and training point i; zero for i >= n or j >= m up to
+// n_pad x m_pad.
+void assemble_cross_rows(hipStream_t s, const double *x, const double *z, int r, SEK sek, double *S, long lds, int n, int n_pad, int m,
+                         int m_pad);
+// out[j] = sum_{i < cols} S[j][i]^2 for j < rows, in a fixed order.
+void row_sumsq(hipStream_t s, const double *S, long lds, int rows, int cols, double *out);
 void assemble_cross_t(hipStream_t s,
                       const double *x,
                       const double *z,

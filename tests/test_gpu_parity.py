@@ -60,7 +60,10 @@ def test_predict_with_uncertainty(ctx, oracle, case):
     ofm, ofv = gp.predict_with_full_cov(list(z), mt, mb)
     assert norm_rel(fm, ofm) < TOL and norm_rel(fv, ofv) < TOL
     pm = ctx.predict(x, y, z, theta, T, b, mt, mb, r)
-    assert np.array_equal(pm, gm)  # predict and predict_with_uncertainty share the mean kernel
+    # predict factors K alone, predict_with_uncertainty with the test rows
+    # below it (the same kernels; the larger trailing updates may be split
+    # over workgroups differently), so they agree to rounding
+    assert norm_rel(pm, gm) < 1e-11
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"T{c[0]}b{c[1]}m{c[2]}r{c[3]}")

@@ -72,12 +72,16 @@ def test_deterministic_and_resident(ctx):
     assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
     model = capi.Model(ctx, x, y, T * b, r, theta)
     rm, rv = model.predict(z, m)
-    assert np.array_equal(rm, a[0]) and np.array_equal(rv, a[1])
+    # the one-call prediction factors the test rows with K, which changes how
+    # the late trailing updates are split over workgroups (and the variance's
+    # summation order): it agrees with the resident model to rounding, which
+    # alpha's conditioning lifts to ~2e-13 here
+    assert norm_rel(rm, a[0]) < 1e-11 and norm_rel(rv, a[1]) < 1e-11
     # several predictions against the same resident factor (the reduction
     # split depends on m, so smaller batches agree to rounding, not bitwise)
     for mm in (1, 7, 1000):
         pm, pv = model.predict(z[:mm + r - 1], mm)
-        assert norm_rel(pm, a[0][:mm]) < 1e-13 and norm_rel(pv, a[1][:mm]) < 1e-13
+        assert norm_rel(pm, rm[:mm]) < 1e-13 and norm_rel(pv, rv[:mm]) < 1e-13
     assert abs(model.loss() - ctx.compute_loss(x, y, theta, T, b, r)) == 0.0
     model.release()
 

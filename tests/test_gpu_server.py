@@ -192,6 +192,35 @@ def test_eviction_rebuilds_identical_factors():
         assert np.array_equal(a["mean"], b["mean"]) and np.array_equal(a["var"], b["var"])
 
 
+def test_predict_keeps_its_own_least_recently_used_factor():
+    """Two resident factors, then a predict on the least recently used one
+    under a budget one byte short of what the predict needs beside both:
+    admission must evict the other factor, not the one the predict reads
+    (which would cost a refit with bytes never reserved)."""
+    from gprat_amd import capi
+    n, r, m = 4096, 8, 2000
+    theta = (1.0, 1.0, 0.1)
+    (xa, ya, za), (xb, yb, _) = _data(60, n, r, m), _data(61, n, r, m)
+
+    def run(budget):
+        with capi.Server(0, slots=1, hbm_budget=budget, precision=capi.PRECISION_MIXED) as srv:
+            a = srv.submit_fit(xa, ya, n, r, theta)
+            assert srv.wait(a)["status"] == 0
+            b = srv.submit_fit(xb, yb, n, r, theta)
+            assert srv.wait(b)["status"] == 0
+            p = srv.wait(srv.submit_predict(a, za, m))
+            return p, srv.stats()
+
+    p1, s1 = run(0)
+    assert s1["evictions"] == 0
+    p2, s2 = run(int(s1["peak_bytes"]) - 1)
+    print(f"peak with both factors resident {s1['peak_bytes']}; tight budget: {s2}")
+    assert p2["status"] == 0, p2["error"]
+    assert s2["evictions"] == 1 and s2["refits"] == 0 and not p2["refit"], s2
+    assert s2["peak_bytes"] <= s2["budget"]
+    assert np.array_equal(p1["mean"], p2["mean"]) and np.array_equal(p1["var"], p2["var"])
+
+
 def test_failures_are_per_request():
     from gprat_amd import capi
     x, y, z = _data(90, 600, 8, 10)
