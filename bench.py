@@ -268,6 +268,48 @@ def request_leg(capi, ctx, n, T, r, m, theta, precision, steps, warmup=1, refine
     return out, fams, model, (mean, var, x, y, z)
 
 
+def c2_oneshot_leg(capi, ctx, steps, warmup):
+    """The C2 request (n = 16384 in 32 tiles of 512, n_reg 8, m = 4096,
+    FP64) through the reference's own call, gprat_predict_with_uncertainty:
+    fit and predict in one, the test rows of K* factored below K."""
+    n16 = N_TILES * TILE
+    fams = ("syrk_trailing", "cholesky", "panel", "panel_rows", "assemble", "solve", "variance_rows", "mean")
+    c2, f, mdl, _ = request_leg(capi, ctx, n16, N_TILES, N_REG, N_TEST, THETA, capi.PRECISION_FP64, steps, warmup=warmup,
+                                families=fams, oneshot=True)
+    mdl.release()
+    c2["api"] = "gprat_predict_with_uncertainty (one call: fit + predict, test rows factored with K)"
+    m_pad = ((N_TEST + 63) // 64) * 64
+    tr = trailing_update_flops(n16, m_pad)
+    c2["roofline_trailing_update"] = roofline_of(
+        "k_gemm_f64<true,false,5> (FP64 trailing update, test rows included)", sum(tr) / len(tr), f["syrk_trailing"],
+        FP64_MFMA_PEAK_TFLOPS, "trailing_update_oneshot_n16384_traffic.md")
+    fact_ms = f["cholesky"][1] / max(f["cholesky"][0], 1)
+    c2["factorisation_with_test_rows_ms"] = fact_ms
+    c2["factorisation_with_test_rows_tflops"] = (n16 ** 3 / 3.0 + n16 ** 2 * N_TEST) / (fact_ms * 1e-3) / 1e12
+    c2["phase_ms_per_request"] = {k: v[1] / steps for k, v in f.items()}
+    return c2
+
+
+def c2_resident_leg(capi, ctx, steps, warmup):
+    """The same request through a resident model (gprat_model_fit, then
+    gprat_model_predict), whose factorisation is the plain Cholesky."""
+    n16 = N_TILES * TILE
+    fams = ("syrk_trailing", "cholesky", "panel", "panel_rows", "assemble", "solve", "trsm", "mean")
+    c2r, f, mdl, _ = request_leg(capi, ctx, n16, N_TILES, N_REG, N_TEST, THETA, capi.PRECISION_FP64, steps, warmup=warmup,
+                                 families=fams)
+    mdl.release()
+    chol_ms = f["cholesky"][1] / max(f["cholesky"][0], 1)
+    c2r["api"] = "resident model: gprat_model_fit + gprat_model_predict"
+    c2r["cholesky_ms"] = chol_ms
+    c2r["cholesky_tflops"] = n16 ** 3 / 3.0 / (chol_ms * 1e-3) / 1e12
+    tr = trailing_update_flops(n16)
+    c2r["roofline_trailing_update"] = roofline_of(
+        "k_gemm_f64<true,false,5> (FP64 trailing update)", sum(tr) / len(tr), f["syrk_trailing"], FP64_MFMA_PEAK_TFLOPS,
+        "trailing_update_n16384_traffic.md")
+    c2r["phase_ms_per_request"] = {k: v[1] / steps for k, v in f.items()}
+    return c2r
+
+
 def vs_oracle_n32768(mean, var, loss, grad):
     """Errors of the first 2048 test points' mean and variance, the loss and
     the gradient against the oracle at n = 32768 (committed fixture)."""
@@ -595,7 +637,7 @@ def main():
     ap.add_argument("--chol-n", type=int, default=32768, help="size of the Cholesky-rate leg (0 skips it)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched many-GP leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop leg")
-    ap.add_argument("--only", choices=("batched", "serving"), default=None,
+    ap.add_argument("--only", choices=("batched", "serving", "c2", "c2-resident"), default=None,
                     help="run one leg alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -613,6 +655,12 @@ def main():
         out = batched_leg(capi, ctx)
         ctx.close()
         print(json.dumps({"batched_2048_gps": out}), flush=True)
+        return
+    if args.only in ("c2", "c2-resident"):
+        leg = c2_oneshot_leg if args.only == "c2" else c2_resident_leg
+        out = leg(capi, ctx, args.steps, args.warmup)
+        ctx.close()
+        print(json.dumps({args.only: out}), flush=True)
         return
     families = ("syrk_trailing", "residual", "trsm", "trsm_update", "panel", "panel_rows", "cholesky", "assemble", "solve", "mean",
                 "refine")
@@ -679,38 +727,8 @@ def main():
         result["mixed_unrefined_variance_n32k"] = unr
         result["speedup_mixed_refined_vs_fp64"] = f64["ms_per_request"] / head["ms_per_request"]
         result["speedup_mixed_unrefined_vs_fp64"] = f64["ms_per_request"] / unr["ms_per_request"]
-        # C2 through the reference's own call (gprat_predict_with_uncertainty:
-        # the test rows are factored with K), then through a resident model
-        n16 = N_TILES * TILE
-        c2fams = ("syrk_trailing", "cholesky", "panel", "panel_rows", "assemble", "solve", "variance_rows", "mean")
-        c2, c2f, mdl, (m16, v16, *_r) = request_leg(capi, ctx, n16, N_TILES, N_REG, N_TEST, THETA, capi.PRECISION_FP64, 10, warmup=2,
-                                                    families=c2fams, oneshot=True)
-        c2["api"] = "gprat_predict_with_uncertainty (one call: fit + predict, test rows factored with K)"
-        mdl.release()
-        m_pad16 = ((N_TEST + 63) // 64) * 64
-        tr16 = trailing_update_flops(n16, m_pad16)
-        c2["roofline_trailing_update"] = roofline_of(
-            "k_gemm_f64<true,false,5> (FP64 trailing update, test rows included)", sum(tr16) / len(tr16), c2f["syrk_trailing"],
-            FP64_MFMA_PEAK_TFLOPS, "trailing_update_oneshot_n16384_traffic.md")
-        fact_ms = c2f["cholesky"][1] / max(c2f["cholesky"][0], 1)
-        c2["factorisation_with_test_rows_ms"] = fact_ms
-        c2["factorisation_with_test_rows_tflops"] = (n16 ** 3 / 3.0 + n16 ** 2 * N_TEST) / (fact_ms * 1e-3) / 1e12
-        c2["phase_ms_per_request"] = {k: v[1] / 10 for k, v in c2f.items()}
-        result["fp64_n16k"] = c2
-        c2fams_r = ("syrk_trailing", "cholesky", "panel", "panel_rows", "assemble", "solve", "trsm", "mean")
-        c2r, c2rf, mdl, _ = request_leg(capi, ctx, n16, N_TILES, N_REG, N_TEST, THETA, capi.PRECISION_FP64, 10, warmup=2,
-                                        families=c2fams_r)
-        mdl.release()
-        chol_ms = c2rf["cholesky"][1] / max(c2rf["cholesky"][0], 1)
-        c2r["api"] = "resident model: gprat_model_fit + gprat_model_predict"
-        c2r["cholesky_ms"] = chol_ms
-        c2r["cholesky_tflops"] = n16 ** 3 / 3.0 / (chol_ms * 1e-3) / 1e12
-        tr16r = trailing_update_flops(n16)
-        c2r["roofline_trailing_update"] = roofline_of(
-            "k_gemm_f64<true,false,5> (FP64 trailing update)", sum(tr16r) / len(tr16r), c2rf["syrk_trailing"],
-            FP64_MFMA_PEAK_TFLOPS, "trailing_update_n16384_traffic.md")
-        c2r["phase_ms_per_request"] = {k: v[1] / 10 for k, v in c2rf.items()}
-        result["fp64_n16k_resident"] = c2r
+        result["fp64_n16k"] = c2_oneshot_leg(capi, ctx, 10, 2)
+        result["fp64_n16k_resident"] = c2_resident_leg(capi, ctx, 10, 2)
         result["optimize_n16k"] = optimize_leg()
         calib = ctx.calibrate()
         result["device_calibration"] = calib
