@@ -637,7 +637,7 @@ def main():
     ap.add_argument("--chol-n", type=int, default=32768, help="size of the Cholesky-rate leg (0 skips it)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched many-GP leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop leg")
-    ap.add_argument("--only", choices=("batched", "serving", "c2", "c2-resident"), default=None,
+    ap.add_argument("--only", choices=("batched", "serving", "c2", "c2-resident", "chol"), default=None,
                     help="run one leg alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -655,6 +655,11 @@ def main():
         out = batched_leg(capi, ctx)
         ctx.close()
         print(json.dumps({"batched_2048_gps": out}), flush=True)
+        return
+    if args.only == "chol":
+        out = cholesky_rate(capi, ctx, args.chol_n)
+        ctx.close()
+        print(json.dumps({"cholesky": out}), flush=True)
         return
     if args.only in ("c2", "c2-resident"):
         leg = c2_oneshot_leg if args.only == "c2" else c2_resident_leg
