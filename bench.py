@@ -728,7 +728,13 @@ def main():
                                                  refine_variance=False)
         unr["rel_err_vs_oracle"] = vs_oracle_n32768(mu, vu, mdl.loss(), mdl.gradient()[1])
         mdl.release()
+        one, _, mdl, (mo, vo, *_r) = request_leg(capi, ctx, HN, HT, HR, HM, HTHETA, capi.PRECISION_FP64, 3, warmup=1,
+                                                 oneshot=True)
+        one["api"] = "gprat_predict_with_uncertainty (one call, FP64, test rows factored with K)"
+        one["rel_err_vs_oracle"] = vs_oracle_n32768(mo, vo, mdl.loss(), mdl.gradient()[1])
+        mdl.release()
         result["fp64_n32k"] = f64
+        result["fp64_n32k_one_call"] = one
         result["mixed_unrefined_variance_n32k"] = unr
         result["speedup_mixed_refined_vs_fp64"] = f64["ms_per_request"] / head["ms_per_request"]
         result["speedup_mixed_unrefined_vs_fp64"] = f64["ms_per_request"] / unr["ms_per_request"]
