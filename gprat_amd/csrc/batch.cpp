@@ -209,8 +209,8 @@ void launch_level(hipStream_t s, const LevelPlan &p, const int2 *items, const k:
         k::batch_level_step(s, off, gps, items + p.off_off[jb], static_cast<int>(p.off_n[jb]), static_cast<int>(jb), mixed, losses, info,
                             steps);
     }
-    k::batch_level_step(s, k::BatchLevelStep::Finish, gps, items + p.finish_off, static_cast<int>(p.finish_n), 0, mixed, losses, info,
-                        steps);
+    k::batch_level_step(s, wide ? k::BatchLevelStep::Finish128 : k::BatchLevelStep::Finish, gps, items + p.finish_off,
+                        static_cast<int>(p.finish_n), 0, mixed, losses, info, steps);
 }
 
 }  // namespace

@@ -43,6 +43,7 @@ struct Factor
     double D[NB][NB + 1];
     double W[NB][NB + 1];
     double R[NB];
+    double T[NB];  // the diagonal step's L_jb,0:jb z_0:jb (alpha's forward solve)
 };
 // FP32 copies of two 64x64 blocks for the mixed-precision products; row
 // stride XF floats (the 16 rows x 4 k-slots of an MFMA operand read fall in
@@ -141,12 +142,22 @@ __device__ __forceinline__ void mfma_nt_acc32(const float *A, const float *B, v4
 
 // acc = sum_{k < kn} L[i, k] L[j, k]^T over 64-blocks of block rows i and j
 // (block (r, k) at L + r*64*np + k*64), streamed through the LDS stage with
-// the next pair prefetched into registers.
-__device__ __forceinline__ void accumulate_row_products(const double *L, long np, int ib, int jb, int kn, Stage &st, v4d (&acc)[2][2])
+// the next pair prefetched into registers. With zv, also tout = L[i, 0:kn]
+// zv[0:kn] (64 values, fixed order: each block's 64 columns in four
+// 16-column parts per row, the parts summed in a fixed order at the end),
+// from the blocks of row i while they are in LDS.
+__device__ __forceinline__ void accumulate_row_products(const double *L, long np, int ib, int jb, int kn, Stage &st, v4d (&acc)[2][2],
+                                                        const double *zv = nullptr, double *tout = nullptr)
 {
     blk::zero(acc);
+    const int trow = threadIdx.x & 63, tq = threadIdx.x >> 6;
+    double tp = 0.0;
     if (kn == 0)
+    {
+        if (tout && threadIdx.x < NB)
+            tout[threadIdx.x] = 0.0;
         return;
+    }
     const double *ai = L + static_cast<long>(ib) * NB * np;
     const double *bj = L + static_cast<long>(jb) * NB * np;
     BlockRegs ra, rb;
@@ -166,8 +177,23 @@ __device__ __forceinline__ void accumulate_row_products(const double *L, long np
         // keep the prefetch ahead of the products
         __builtin_amdgcn_sched_barrier(0);
         blk::mfma_nt_acc(st.a, st.b, acc);
+        if (zv)
+        {
+            const double *zb = zv + kb * NB + tq * 16;
+#pragma unroll
+            for (int c = 0; c < 16; c++)
+                tp = fma(st.a[trow * XS + tq * 16 + c], zb[c], tp);
+        }
     }
     __syncthreads();
+    if (tout)
+    {
+        st.b[tq * NB + trow] = tp;
+        __syncthreads();
+        if (threadIdx.x < NB)
+            tout[threadIdx.x] = (st.b[threadIdx.x] + st.b[NB + threadIdx.x]) + (st.b[2 * NB + threadIdx.x] + st.b[3 * NB + threadIdx.x]);
+        __syncthreads();
+    }
 }
 
 // The same sum with FP32 operands and FP32 accumulation.
@@ -202,10 +228,10 @@ __device__ __forceinline__ void accumulate_row_products32(const double *L, long 
     __syncthreads();
 }
 
-// vec <- K^-1 vec for one GP from its factor: forward then backward block
-// substitution with the saved 64x64 inverses (W). xs and part are two
-// 256-double LDS buffers.
-__device__ __forceinline__ void block_solves(const double *L, const double *Wg, long np, int nblk, double *vec, double *xs, double *part)
+// vec <- L^-1 vec for one GP from its factor: forward block substitution
+// with the saved 64x64 inverses (W). xs and part are two 256-double LDS
+// buffers.
+__device__ __forceinline__ void forward_solve(const double *L, const double *Wg, long np, int nblk, double *vec, double *xs, double *part)
 {
     const int tid = threadIdx.x;
     const int row = tid & 63, quarter = tid >> 6;
@@ -254,6 +280,13 @@ __device__ __forceinline__ void block_solves(const double *L, const double *Wg, 
         }
         __syncthreads();
     }
+}
+
+// vec <- L^-T vec: the backward block substitution.
+__device__ __forceinline__ void backward_solve(const double *L, const double *Wg, long np, int nblk, double *vec, double *xs, double *part)
+{
+    const int tid = threadIdx.x;
+    const int row = tid & 63, quarter = tid >> 6;
     for (int J = nblk - 1; J >= 0; J--)
     {
         const double *W = Wg + static_cast<long>(J) * NB * NB;
@@ -297,6 +330,13 @@ __device__ __forceinline__ void block_solves(const double *L, const double *Wg, 
         }
         __syncthreads();
     }
+}
+
+// vec <- K^-1 vec: forward, then backward.
+__device__ __forceinline__ void block_solves(const double *L, const double *Wg, long np, int nblk, double *vec, double *xs, double *part)
+{
+    forward_solve(L, Wg, np, nblk, vec, xs, part);
+    backward_solve(L, Wg, np, nblk, vec, xs, part);
 }
 
 // max over the workgroup of v (all threads get it); red holds THREADS doubles
@@ -501,7 +541,7 @@ __device__ __forceinline__ void gen_block(const BatchGP &g, int ib, int jb, doub
 // is reduced exactly once, so no pass assembles K first), through the
 // stage's first block.
 template <bool MIXED>
-__device__ __forceinline__ void reduced_block(const BatchGP &g, int ib, int jb, Shared &sh, double (&kij)[16])
+__device__ __forceinline__ void reduced_block(const BatchGP &g, int ib, int jb, Shared &sh, double (&kij)[16], bool forward = false)
 {
     const long np = g.n_pad;
     gen_block(g, ib, jb, sh.st.a);
@@ -525,7 +565,7 @@ __device__ __forceinline__ void reduced_block(const BatchGP &g, int ib, int jb, 
     else
     {
         v4d acc[2][2];
-        accumulate_row_products(g.L, np, ib, jb, jb, sh.st, acc);
+        accumulate_row_products(g.L, np, ib, jb, jb, sh.st, acc, forward ? g.alpha : nullptr, forward ? sh.fa.T : nullptr);
 #pragma unroll
         for (int q = 0; q < 16; q++)
             kij[q] -= acc[q >> 3][(q >> 2) & 1][q & 3];
@@ -534,13 +574,16 @@ __device__ __forceinline__ void reduced_block(const BatchGP &g, int ib, int jb, 
 
 // Diagonal block jb: reduced, then factored with its inverse in LDS; L_jj
 // goes back to the factor, W_jj (row-major, zero above the diagonal) to
-// the GP's inverses and, when wjj is given, to LDS (stride XS).
+// the GP's inverses and, when wjj is given, to LDS (stride XS). In FP64 the
+// step also takes alpha's forward solve one block further: the products of
+// row block jb with z_0:jb come from the blocks the reduction streams
+// through LDS anyway, and z_jb = W_jj (y_jb - those) goes to alpha.
 template <bool MIXED>
 __device__ __forceinline__ void diag_block(const BatchGP &g, int gi, int jb, Shared &sh, double *wjj, int *info)
 {
     const long np = g.n_pad;
     double kij[16];
-    reduced_block<MIXED>(g, jb, jb, sh, kij);
+    reduced_block<MIXED>(g, jb, jb, sh, kij, !MIXED);
 #pragma unroll
     for (int q = 0; q < 16; q++)
     {
@@ -561,6 +604,18 @@ __device__ __forceinline__ void diag_block(const BatchGP &g, int gi, int jb, Sha
         Wjj[e] = w;
         if (wjj)
             wjj[r * XS + c] = w;
+    }
+    if constexpr (!MIXED)
+    {
+        if (threadIdx.x < NB)
+        {
+            const int r = threadIdx.x;
+            const double *yb = g.y + static_cast<long>(jb) * NB;
+            double s = 0.0;
+            for (int c = 0; c <= r; c++)
+                s = fma(sh.fa.W[r][c], yb[c] - sh.fa.T[c], s);
+            g.alpha[static_cast<long>(jb) * NB + r] = s;
+        }
     }
     __syncthreads();
 }
@@ -590,9 +645,12 @@ __device__ __forceinline__ void off_block(const BatchGP &g, int ib, int jb, Shar
     __syncthreads();
 }
 
-// alpha (refined in FP64 for MIXED) and the loss of one factored GP.
+// alpha (refined in FP64 for MIXED) and the loss of one factored GP. With
+// forward_done, alpha already holds L^-1 y (the 64-wide FP64 diagonal steps
+// compute it) and only the backward substitution is left.
 template <bool MIXED>
-__device__ __forceinline__ void finish_gp(const BatchGP &g, int gi, double (*red)[THREADS], double *losses, int *refine_steps)
+__device__ __forceinline__ void finish_gp(const BatchGP &g, int gi, double (*red)[THREADS], double *losses, int *refine_steps,
+                                          bool forward_done = false)
 {
     const int tid = threadIdx.x;
     const double *x = g.x;
@@ -607,10 +665,14 @@ __device__ __forceinline__ void finish_gp(const BatchGP &g, int gi, double (*red
     // alpha: forward then backward block substitution with W
     double *xs = red[0];
     double *part = red[1];
-    for (long i = tid; i < np; i += THREADS)
-        alpha[i] = (i < g.n) ? y[i] : 0.0;
-    __syncthreads();
-    block_solves(L, Wg, np, nblk, alpha, xs, part);
+    if (!forward_done)
+    {
+        for (long i = tid; i < np; i += THREADS)
+            alpha[i] = (i < g.n) ? y[i] : 0.0;
+        __syncthreads();
+        forward_solve(L, Wg, np, nblk, alpha, xs, part);
+    }
+    backward_solve(L, Wg, np, nblk, alpha, xs, part);
     if constexpr (MIXED)
     {
         // FP64 refinement: r = y - K alpha with K regenerated from the
@@ -732,7 +794,8 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
             for (int ib = jb + 1; ib < nblk; ib++)
                 off_block<MIXED>(g, ib, jb, sh, wjj);
         }
-        finish_gp<MIXED>(g, gi, red, losses, refine_steps);
+        __syncthreads();  // the diagonal steps' alpha entries are visible to every thread
+        finish_gp<MIXED>(g, gi, red, losses, refine_steps, !MIXED);
     }
 }
 
@@ -768,12 +831,12 @@ __global__ __launch_bounds__(THREADS) void k_batch_off_level(const BatchGP *__re
 
 template <bool MIXED>
 __global__ __launch_bounds__(THREADS) void k_batch_finish_level(const BatchGP *__restrict__ gps, const int2 *__restrict__ items,
-                                                                 double *__restrict__ losses, int *__restrict__ refine_steps)
+                                                                 double *__restrict__ losses, int *__restrict__ refine_steps, int forward_done)
 {
     __shared__ double red[2][THREADS];
     const int2 it = items[blockIdx.x];
     const BatchGP g = gps[it.x];
-    finish_gp<MIXED>(g, it.x, red, losses, refine_steps);
+    finish_gp<MIXED>(g, it.x, red, losses, refine_steps, forward_done != 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1013,10 +1076,15 @@ void batch_level_step(hipStream_t s, BatchLevelStep step, const BatchGP *gps, co
         hipLaunchKernelGGL(k_batch_off_level128, grid, dim3(THREADS), 0, s, gps, items, jb);
         break;
     case BatchLevelStep::Finish:
+        // the 64-wide FP64 diagonal steps did alpha's forward solve
         if (mixed)
-            hipLaunchKernelGGL(k_batch_finish_level<true>, grid, dim3(THREADS), 0, s, gps, items, losses, refine_steps);
+            hipLaunchKernelGGL(k_batch_finish_level<true>, grid, dim3(THREADS), 0, s, gps, items, losses, refine_steps, 0);
         else
-            hipLaunchKernelGGL(k_batch_finish_level<false>, grid, dim3(THREADS), 0, s, gps, items, losses, refine_steps);
+            hipLaunchKernelGGL(k_batch_finish_level<false>, grid, dim3(THREADS), 0, s, gps, items, losses, refine_steps, 1);
+        break;
+    case BatchLevelStep::Finish128:
+        require(!mixed, "batch: the 128-wide level path is FP64");
+        hipLaunchKernelGGL(k_batch_finish_level<false>, grid, dim3(THREADS), 0, s, gps, items, losses, refine_steps, 0);
         break;
     }
     GP_CHECK_LAUNCH();

@@ -210,7 +210,8 @@ enum class BatchLevelStep
     OffDiagonal,
     Diagonal128,
     OffDiagonal128,
-    Finish
+    Finish,
+    Finish128  // Finish after 128-wide steps (their diagonal steps leave alpha's forward solve to it)
 };
 void batch_level_step(hipStream_t s, BatchLevelStep step, const BatchGP *gps, const int2 *items, int n_items, int jb, bool mixed,
                       double *losses, int *info, int *refine_steps);
