@@ -799,6 +799,144 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
     }
 }
 
+// ---- 64-wide level path, FP64 off-diagonal blocks on 32-deep stages -----
+// The persistent kernel keeps whole 64x64 blocks in LDS (two staged blocks,
+// the diagonal inverse, the factor's workspace: two workgroups per CU). An
+// off-diagonal level step needs less: staging 32-deep halves of the block
+// pairs takes 35 KB, so four workgroups share a CU and hide each other's
+// staging. Each accumulator sees the same MFMAs in the same order as with
+// whole blocks (k ascending), so the results are bit-identical.
+constexpr int XH = 32 + 2;  // LDS row stride of a staged 64x32 half block
+union HalfShared
+{
+    struct
+    {
+        double a[NB * XH];
+        double b[NB * XH];
+    } h;
+    double kb[NB * XS];  // the generated block of K (stride XS)
+};
+
+// One thread's share of a 64x32 half block: 4 pairs of doubles, row e >> 4,
+// columns (e & 15) * 2, e = tid + 256 s.
+struct HalfRegs
+{
+    d2 v[4];
+
+    __device__ __forceinline__ void load(const double *__restrict__ src, long ld)
+    {
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+        {
+            const int e = threadIdx.x + THREADS * s;
+            v[s] = *reinterpret_cast<const d2 *>(src + static_cast<long>(e >> 4) * ld + (e & 15) * 2);
+        }
+    }
+
+    __device__ __forceinline__ void store(double *dst) const
+    {
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+        {
+            const int e = threadIdx.x + THREADS * s;
+            *reinterpret_cast<d2 *>(dst + (e >> 4) * XH + (e & 15) * 2) = v[s];
+        }
+    }
+};
+
+// acc += A B^T over a 32-deep half (stride XH), in blk::mfma_nt_acc's order.
+__device__ __forceinline__ void mfma_nt_acc_half(const double *A, const double *B, v4d (&acc)[2][2])
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lm = lane & 15, lk = lane >> 4;
+    const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
+#pragma unroll 4
+    for (int k0 = 0; k0 < 32; k0 += 4)
+    {
+        double a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+            a[i] = A[(r0 + i * 16 + lm) * XH + k0 + lk];
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+            b[j] = B[(c0 + j * 16 + lm) * XH + k0 + lk];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+}
+
+// off_block<false> with wjj == nullptr, on half stages.
+__device__ __forceinline__ void off_block_halves(const BatchGP &g, int ib, int jb, HalfShared &sh)
+{
+    const long np = g.n_pad;
+    double kij[16];
+    gen_block(g, ib, jb, sh.kb);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+    {
+        int r, c;
+        acc_position<false>(q, r, c);
+        kij[q] = sh.kb[r * XS + c];
+    }
+    __syncthreads();
+    v4d acc[2][2];
+    blk::zero(acc);
+    const int steps = 2 * jb;
+    if (steps > 0)
+    {
+        const double *ai = g.L + static_cast<long>(ib) * NB * np;
+        const double *bj = g.L + static_cast<long>(jb) * NB * np;
+        HalfRegs ra, rb;
+        ra.load(ai, np);
+        rb.load(bj, np);
+        for (int t = 0; t < steps; t++)
+        {
+            __syncthreads();  // every wavefront is done with the previous pair
+            ra.store(sh.h.a);
+            rb.store(sh.h.b);
+            __syncthreads();
+            if (t + 1 < steps)
+            {
+                ra.load(ai + (t + 1) * 32, np);
+                rb.load(bj + (t + 1) * 32, np);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            mfma_nt_acc_half(sh.h.a, sh.h.b, acc);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+        kij[q] -= acc[q >> 3][(q >> 2) & 1][q & 3];
+    // L_ij = (K_ij - acc) W_jj^T, the 64-deep product as two halves of C's
+    // and W_jj's columns
+    v4d xb[2][2];
+    blk::zero(xb);
+    const double *W = g.W + static_cast<long>(jb) * NB * NB;
+    for (int h = 0; h < 2; h++)
+    {
+        HalfRegs rw;
+        rw.load(W + h * 32, NB);
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+        {
+            int r, c;
+            acc_position<false>(q, r, c);
+            if ((c >> 5) == h)
+                sh.h.a[r * XH + (c & 31)] = kij[q];
+        }
+        rw.store(sh.h.b);
+        __syncthreads();
+        mfma_nt_acc_half(sh.h.a, sh.h.b, xb);
+        __syncthreads();
+    }
+    blk::store_acc(g.L + static_cast<long>(ib) * NB * np + jb * NB, np, xb, true, false);
+}
+
 // Level-synchronous path for large GPs, one launch per step over all of
 // them: workgroup w takes item w of its launch's list (GP index and block
 // row). The same block functions run in the same order per block as in the
@@ -823,10 +961,18 @@ __global__ __launch_bounds__(THREADS) void k_batch_diag_level(const BatchGP *__r
 template <bool MIXED>
 __global__ __launch_bounds__(THREADS) void k_batch_off_level(const BatchGP *__restrict__ gps, const int2 *__restrict__ items, int jb)
 {
-    __shared__ Shared sh;
     const int2 it = items[blockIdx.x];
     const BatchGP g = gps[it.x];
-    off_block<MIXED>(g, it.y, jb, sh, nullptr);
+    if constexpr (MIXED)
+    {
+        __shared__ Shared sh;
+        off_block<true>(g, it.y, jb, sh, nullptr);
+    }
+    else
+    {
+        __shared__ HalfShared sh;
+        off_block_halves(g, it.y, jb, sh);
+    }
 }
 
 template <bool MIXED>
