@@ -255,25 +255,33 @@ double Server::cost(const Request &r, const Entry &e) const
 }
 
 // Cost model of the small-fit routing (milliseconds, FP64, n <= 2048),
-// measured on MI355X (host-timed fits, n = 256 .. 2048): a single-GP fit
-// uses the whole device but is bound by its launches' latency, about linear
-// in n (0.26 ms at n = 256, 1.91 ms at 2048); the batched kernel runs each GP
-// as one workgroup's chain of 64x64 block products on one CU, so a launch
-// lasts about as long as its largest GP's chain (0.37 ms at n = 256, 26.8 ms
-// at 2048 for a launch of one GP).
+// measured on MI355X (host-timed, n = 256 .. 2048):
+// * a single-GP fit uses the whole device but is bound by its launches'
+//   latency, about linear in n (0.27 ms at n = 256, 1.89 ms at 2048);
+// * a batched fit runs its GPs above n_pad 128 as level-synchronous launches
+//   (two per 64-row block of the largest GP), so it lasts as long as that
+//   chain of launches (0.34 ms for one GP of n = 256, 1.55 ms at 1024,
+//   4.67 ms at 2048) plus the GPs' work at the batch's throughput (16 GPs of
+//   n = 2048: 5.51 ms).
 static double single_fit_ms(int n) { return 0.05 + 0.00092 * n; }
-static double chain_ms(int n)
+static double batch_chain_ms(int n_max)
+{
+    const double n = n_max;
+    return 0.14 + 5.37e-4 * n + 8.0e-7 * n * n;
+}
+static double batch_work_ms(int n)
 {
     const double f = static_cast<double>(n) * n * n / 3.0;
-    return 0.3 + f / 9.5e7;
+    return f / 4.0e10;  // 40 TF/s
 }
 
 // Largest n of the small fits that should share one batched launch now,
 // from the small fits waiting at and after `first` (up to batch_max), or 0
-// for none. For each cut n_c, the fits with n <= n_c would take chain_ms(n_c)
-// as one launch against their single-GP times spread over the slots; the cut
-// with the largest saving wins (at least two fits). A lone small fit, or a
-// large small fit beside a few small ones, takes the single-GP path.
+// for none. For each cut n_c, the fits with n <= n_c would take one batched
+// fit (batch_chain_ms(n_c) plus their work) against their single-GP times
+// spread over the slots; the cut with the largest saving wins (at least two
+// fits). A lone small fit, or a few small fits that the slots finish
+// sooner one by one, take the single-GP path.
 int Server::batch_cut(std::deque<Request>::iterator first)
 {
     std::vector<int> ns;
@@ -286,15 +294,16 @@ int Server::batch_cut(std::deque<Request>::iterator first)
             ns.push_back(qm->second->n);
     }
     std::sort(ns.begin(), ns.end());
-    double single = 0.0, best = 0.0;
+    double single = 0.0, work = 0.0, best = 0.0;
     int cut = 0;
     for (std::size_t k = 0; k < ns.size(); k++)
     {
         single += single_fit_ms(ns[k]);
+        work += batch_work_ms(ns[k]);
         const int count = static_cast<int>(k) + 1;
         if (count < 2 || (k + 1 < ns.size() && ns[k + 1] == ns[k]))
             continue;
-        const double saving = single / std::min(count, cfg_.slots) - chain_ms(ns[k]);
+        const double saving = single / std::min(count, cfg_.slots) - (batch_chain_ms(ns[k]) + work);
         if (saving > best)
         {
             best = saving;
