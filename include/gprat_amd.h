@@ -129,7 +129,10 @@ int gprat_predict(gprat_context *ctx,
                   double *mean_out);
 
 /* Replaces gpu::predict_with_uncertainty (gp_functions.cuh:55-65).
- * mean_out and var_out have m values each. */
+ * mean_out and var_out have m values each. Fit and prediction in one call:
+ * the test points' rows of K* are factored together with K (their forward
+ * solve rides in the factorisation); the results agree with a resident
+ * model's to rounding. */
 int gprat_predict_with_uncertainty(gprat_context *ctx,
                                    const double *train_in,
                                    size_t train_in_len,
