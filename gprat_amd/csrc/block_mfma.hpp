@@ -129,6 +129,82 @@ __device__ __forceinline__ void mfma_nt_acc(const double *A, const double *B, v4
     }
 }
 
+// ---- 32-deep half stages -------------------------------------------------
+// A product of two 64x64 blocks staged as two 64x32 halves per operand (LDS
+// stride XH) needs half the LDS of whole blocks, so twice the workgroups fit
+// a CU; mfma_nt_acc_half runs the MFMAs of one half in mfma_nt_acc's order,
+// so the two halves give bit-identical results to one whole-block product.
+constexpr int XH = 32 + 2;
+
+// this thread's share of a 64x32 part of a row-major matrix (rows e >> 4,
+// columns (e & 15) * 2, e = tid + 256 s)
+__device__ __forceinline__ void load_half(const double *__restrict__ src, long ld, d2 (&v)[4])
+{
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+    {
+        const int e = threadIdx.x + THREADS * s;
+        v[s] = *reinterpret_cast<const d2 *>(src + static_cast<long>(e >> 4) * ld + (e & 15) * 2);
+    }
+}
+
+__device__ __forceinline__ void store_half(double *dst, const d2 (&v)[4])
+{
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+    {
+        const int e = threadIdx.x + THREADS * s;
+        *reinterpret_cast<d2 *>(dst + (e >> 4) * XH + (e & 15) * 2) = v[s];
+    }
+}
+
+// this thread's share of a 32x64 part (rows e >> 5, columns (e & 31) * 2),
+// to be stored transposed: dst[c][r] = src[r][c] (stride XH)
+__device__ __forceinline__ void load_half_t(const double *__restrict__ src, long ld, d2 (&v)[4])
+{
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+    {
+        const int e = threadIdx.x + THREADS * s;
+        v[s] = *reinterpret_cast<const d2 *>(src + static_cast<long>(e >> 5) * ld + (e & 31) * 2);
+    }
+}
+
+__device__ __forceinline__ void store_half_t(double *dst, const d2 (&v)[4])
+{
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+    {
+        const int e = threadIdx.x + THREADS * s;
+        const int r = e >> 5, c = (e & 31) * 2;
+        dst[c * XH + r] = v[s][0];
+        dst[(c + 1) * XH + r] = v[s][1];
+    }
+}
+
+__device__ __forceinline__ void mfma_nt_acc_half(const double *A, const double *B, v4d (&acc)[2][2])
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lm = lane & 15, lk = lane >> 4;
+    const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
+#pragma unroll 4
+    for (int k0 = 0; k0 < 32; k0 += 4)
+    {
+        double a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+            a[i] = A[(r0 + i * 16 + lm) * XH + k0 + lk];
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+            b[j] = B[(c0 + j * 16 + lm) * XH + k0 + lk];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+}
+
 __device__ __forceinline__ void mfma_nt(const double *A, const double *B, v4d (&acc)[2][2])
 {
     zero(acc);

@@ -806,7 +806,7 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
 // pairs takes 35 KB, so four workgroups share a CU and hide each other's
 // staging. Each accumulator sees the same MFMAs in the same order as with
 // whole blocks (k ascending), so the results are bit-identical.
-constexpr int XH = 32 + 2;  // LDS row stride of a staged 64x32 half block
+using blk::XH;
 union HalfShared
 {
     struct
@@ -816,57 +816,6 @@ union HalfShared
     } h;
     double kb[NB * XS];  // the generated block of K (stride XS)
 };
-
-// One thread's share of a 64x32 half block: 4 pairs of doubles, row e >> 4,
-// columns (e & 15) * 2, e = tid + 256 s.
-struct HalfRegs
-{
-    d2 v[4];
-
-    __device__ __forceinline__ void load(const double *__restrict__ src, long ld)
-    {
-#pragma unroll
-        for (int s = 0; s < 4; s++)
-        {
-            const int e = threadIdx.x + THREADS * s;
-            v[s] = *reinterpret_cast<const d2 *>(src + static_cast<long>(e >> 4) * ld + (e & 15) * 2);
-        }
-    }
-
-    __device__ __forceinline__ void store(double *dst) const
-    {
-#pragma unroll
-        for (int s = 0; s < 4; s++)
-        {
-            const int e = threadIdx.x + THREADS * s;
-            *reinterpret_cast<d2 *>(dst + (e >> 4) * XH + (e & 15) * 2) = v[s];
-        }
-    }
-};
-
-// acc += A B^T over a 32-deep half (stride XH), in blk::mfma_nt_acc's order.
-__device__ __forceinline__ void mfma_nt_acc_half(const double *A, const double *B, v4d (&acc)[2][2])
-{
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int lm = lane & 15, lk = lane >> 4;
-    const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
-#pragma unroll 4
-    for (int k0 = 0; k0 < 32; k0 += 4)
-    {
-        double a[2], b[2];
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-            a[i] = A[(r0 + i * 16 + lm) * XH + k0 + lk];
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-            b[j] = B[(c0 + j * 16 + lm) * XH + k0 + lk];
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-#pragma unroll
-            for (int j = 0; j < 2; j++)
-                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
-    }
-}
 
 // off_block<false> with wjj == nullptr, on half stages.
 __device__ __forceinline__ void off_block_halves(const BatchGP &g, int ib, int jb, HalfShared &sh)
@@ -890,22 +839,22 @@ __device__ __forceinline__ void off_block_halves(const BatchGP &g, int ib, int j
     {
         const double *ai = g.L + static_cast<long>(ib) * NB * np;
         const double *bj = g.L + static_cast<long>(jb) * NB * np;
-        HalfRegs ra, rb;
-        ra.load(ai, np);
-        rb.load(bj, np);
+        d2 ra[4], rb[4];
+        blk::load_half(ai, np, ra);
+        blk::load_half(bj, np, rb);
         for (int t = 0; t < steps; t++)
         {
             __syncthreads();  // every wavefront is done with the previous pair
-            ra.store(sh.h.a);
-            rb.store(sh.h.b);
+            blk::store_half(sh.h.a, ra);
+            blk::store_half(sh.h.b, rb);
             __syncthreads();
             if (t + 1 < steps)
             {
-                ra.load(ai + (t + 1) * 32, np);
-                rb.load(bj + (t + 1) * 32, np);
+                blk::load_half(ai + (t + 1) * 32, np, ra);
+                blk::load_half(bj + (t + 1) * 32, np, rb);
             }
             __builtin_amdgcn_sched_barrier(0);
-            mfma_nt_acc_half(sh.h.a, sh.h.b, acc);
+            blk::mfma_nt_acc_half(sh.h.a, sh.h.b, acc);
         }
         __syncthreads();
     }
@@ -919,8 +868,8 @@ __device__ __forceinline__ void off_block_halves(const BatchGP &g, int ib, int j
     const double *W = g.W + static_cast<long>(jb) * NB * NB;
     for (int h = 0; h < 2; h++)
     {
-        HalfRegs rw;
-        rw.load(W + h * 32, NB);
+        d2 rw[4];
+        blk::load_half(W + h * 32, NB, rw);
 #pragma unroll
         for (int q = 0; q < 16; q++)
         {
@@ -929,9 +878,9 @@ __device__ __forceinline__ void off_block_halves(const BatchGP &g, int ib, int j
             if ((c >> 5) == h)
                 sh.h.a[r * XH + (c & 31)] = kij[q];
         }
-        rw.store(sh.h.b);
+        blk::store_half(sh.h.b, rw);
         __syncthreads();
-        mfma_nt_acc_half(sh.h.a, sh.h.b, xb);
+        blk::mfma_nt_acc_half(sh.h.a, sh.h.b, xb);
         __syncthreads();
     }
     blk::store_acc(g.L + static_cast<long>(ib) * NB * np + jb * NB, np, xb, true, false);
