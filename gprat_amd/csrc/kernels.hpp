@@ -157,13 +157,13 @@ void panel_to_f32t(hipStream_t s, const double *A, long lda, long r0, long p0, i
 void panel_to_f64t(hipStream_t s, const double *A, long lda, long r0, long p0, int R, int W, double *P, long ldp, double *back = nullptr,
                    long ldback = 0);
 // C[i][j] -= sum_k P[k][a_off + i] P[k][b_off + j] (FP32 products and sums),
-// for i < M, j < N, i + diag_off >= j; K % 32 == 0.
+// for i < M, j < N, i + diag_off >= j; K % 16 == 0.
 // max_wgs > 0 caps the launch (workgroups loop over tiles; with a queue, a
 // device int holding 0 at launch, they take tiles in order from it).
 void gemm_f32_trailing(hipStream_t s, const float *P, long ldp, long a_off, long b_off, double *C, long ldc, int M, int N, int K, long diag_off,
                        int max_wgs = 0, int *queue = nullptr);
 // C (= or -=) A B over FP32 operands, A(i,k) = A[k*lda + i], B(k,j) = B[k*ldb + j],
-// FP32 C; K % 32 == 0, M, N, lda, ldb multiples of 4, A and B 16-byte aligned.
+// FP32 C; K % 16 == 0, M, N, lda, ldb multiples of 4, A and B 16-byte aligned.
 // a_lower: A(i, k) is known zero for k > i (each row tile stops K there)
 void gemm_f32_tn(hipStream_t s, const float *A, long lda, const float *B, long ldb, float *C, long ldc, int M, int N, int K, bool set,
                  int max_wgs = 0, bool a_lower = false);
