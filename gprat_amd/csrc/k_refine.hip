@@ -59,30 +59,35 @@ __global__ __launch_bounds__(THREADS) void k_to_f64(const float *__restrict__ sr
 
 // Z[i][c] += sum_{k = r0}^{i-1} U[k * ldu + i] X[k][c] for the 128-row tile
 // [r0, r0 + 128) of workgroup row blockIdx.y and the 64 columns of
-// blockIdx.x. K and X are staged 16 k-rows at a time through LDS; thread
-// (ty, tx) accumulates rows ty + 16 p (p < 8) and columns tx + 16 q (q < 4)
-// in FP64 FMAs, in k order.
+// blockIdx.x, on the FP64 matrix cores. K (its strictly lower part, zeros
+// elsewhere) and X are staged 16 k-rows at a time through LDS, k-major;
+// wavefront w accumulates rows 32 w .. 32 w + 31 of the tile over all 64
+// columns (two 16-row by four 16-column v_mfma_f64_16x16x4f64 tiles) and
+// skips the k-rows at or below its last row, whose K entries are zero.
 __global__ __launch_bounds__(THREADS) void k_diag_strict_product(const double *__restrict__ U, long ldu, const double *__restrict__ X,
                                                                   long ldx, double *__restrict__ Z, long ldz, int rows)
 {
-    __shared__ double Us[16][129];
+    typedef double v4d __attribute__((ext_vector_type(4)));
+    __shared__ double Us[16][128];
     __shared__ double Xs[16][64];
     const long r0 = static_cast<long>(blockIdx.y) * 128, c0 = static_cast<long>(blockIdx.x) * 64;
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lm = lane & 15, lk = lane >> 4;
     const int span = static_cast<int>(min(static_cast<long>(128), rows - r0));
-    double acc[8][4];
+    const int wend = 32 * w + 32;  // k-rows of this wavefront's products: k < wend
+    v4d acc[2][4];
 #pragma unroll
-    for (int p = 0; p < 8; p++)
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-            acc[p][q] = 0.0;
+        for (int j = 0; j < 4; j++)
+            acc[i][j] = v4d{ 0.0, 0.0, 0.0, 0.0 };
     for (int kk0 = 0; kk0 < span; kk0 += 16)
     {
-        // U rows r0 + kk0 .. +16, columns r0 .. r0 + 128 (the diagonal block, both triangles)
+        // U rows r0 + kk0 .. +16, columns r0 .. r0 + 128: K(i, k) where k < i
         for (int e = threadIdx.x; e < 16 * 128; e += THREADS)
         {
-            const int kr = e >> 7, ic = e & 127;
-            Us[kr][ic] = (kk0 + kr < span && ic < span) ? U[(r0 + kk0 + kr) * ldu + r0 + ic] : 0.0;
+            const int kr = e >> 7, ic = e & 127, k = kk0 + kr;
+            Us[kr][ic] = (k < ic && ic < span) ? U[(r0 + k) * ldu + r0 + ic] : 0.0;
         }
         for (int e = threadIdx.x; e < 16 * 64; e += THREADS)
         {
@@ -90,35 +95,46 @@ __global__ __launch_bounds__(THREADS) void k_diag_strict_product(const double *_
             Xs[kr][cc] = (kk0 + kr < span) ? X[(r0 + kk0 + kr) * ldx + c0 + cc] : 0.0;
         }
         __syncthreads();
-#pragma unroll 4
-        for (int kr = 0; kr < 16; kr++)
+        if (kk0 < wend)
         {
-            const int k = kk0 + kr;
-            double xv[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++)
-                xv[q] = Xs[kr][tx + 16 * q];
-#pragma unroll
-            for (int p = 0; p < 8; p++)
+            for (int k4 = 0; k4 < 16; k4 += 4)
             {
-                const int i = ty + 16 * p;
-                const double a = k < i ? Us[kr][i] : 0.0;  // K(i, k) for k < i, stored at U[k][i]
+                double a[2], b[4];
 #pragma unroll
-                for (int q = 0; q < 4; q++)
-                    acc[p][q] = fma(a, xv[q], acc[p][q]);
+                for (int i = 0; i < 2; i++)
+                    a[i] = Us[k4 + lk][32 * w + 16 * i + lm];
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    b[j] = Xs[k4 + lk][16 * j + lm];
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
             }
         }
         __syncthreads();
     }
+    // accumulator layout: lane holds column lane % 16, rows lane / 16 + 4 e
 #pragma unroll
-    for (int p = 0; p < 8; p++)
-    {
-        const int i = ty + 16 * p;
-        if (i < span)
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-            for (int q = 0; q < 4; q++)
-                Z[(r0 + i) * ldz + c0 + tx + 16 * q] += acc[p][q];
-    }
+        for (int e = 0; e < 4; e++)
+        {
+            const int r = 32 * w + 16 * i + lk + 4 * e;
+            if (r < span)
+            {
+                double *zr = Z + (r0 + r) * ldz + c0 + lm;
+                double cur[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    cur[j] = zr[16 * j];
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    zr[16 * j] = cur[j] + acc[i][j][e];
+            }
+        }
 }
 
 // Partial column sums over the row range of split blockIdx.y:
