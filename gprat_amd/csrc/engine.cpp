@@ -463,8 +463,9 @@ void cholesky(Context &c, double *A, long lda, int n_pad, double *Winv, bool mix
     const bool ahead = use_lookahead() && c.panel != c.main;
     hipStream_t T = ahead ? c.trail : c.main, P = ahead ? c.panel : c.main;
     // Two trailing workgroups per CU, minus the reserve. The FP32 product
-    // would fit three per CU, but a cap of 3 x CUs - 24 slowed the panel
-    // stream 2.5x and the n = 32768 mixed request by 7 ms on MI355X.
+    // would fit three per CU, but any cap above two per CU (552 to 744 on
+    // 256 CUs) slowed the panel stream 2x and the n = 32768 mixed request
+    // by 6 ms on MI355X.
     const int cap = ahead ? std::max(1, 2 * c.cus - trailing_reserve()) : 0;
     const int Bo = outer_block(n_pad);
     const int nblk = static_cast<int>(ceil_div(n_pad, Bo));
