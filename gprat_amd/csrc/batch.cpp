@@ -147,7 +147,9 @@ Route route(const Tuning &t, int n_pad, bool mixed_requested)
 
 // Items of the level path's launches for the GPs `ids` (64-blocks per GP
 // in nb): Assemble (every block row), per column jb Diagonal (GPs with more
-// than jb blocks) and OffDiagonal (their blocks below jb), then Finish.
+// than jb blocks) and OffDiagonal (their blocks below jb, and with prefix,
+// for jb >= 1, {GP, -1} for the GPs with more than jb + 1 blocks: their
+// next diagonal block's first sums), then Finish.
 struct LevelPlan
 {
     std::vector<int2> items;
@@ -157,7 +159,7 @@ struct LevelPlan
 
 // nb: 64-row blocks per GP; width: the column width of the factorisation
 // steps (64 or 128; blocks per step column = ceil(nb * 64 / width)).
-LevelPlan level_plan(const std::vector<int> &ids, const std::vector<int> &nb64, int width)
+LevelPlan level_plan(const std::vector<int> &ids, const std::vector<int> &nb64, int width, bool prefix)
 {
     LevelPlan p;
     std::vector<int> nb(nb64.size());
@@ -182,6 +184,11 @@ LevelPlan level_plan(const std::vector<int> &ids, const std::vector<int> &nb64, 
                 p.items.push_back(make_int2(g, jb));
         p.diag_n.push_back(static_cast<long>(p.items.size()) - p.diag_off.back());
         p.off_off.push_back(static_cast<long>(p.items.size()));
+        // the next diagonal blocks' sums are as long as this column's blocks
+        if (prefix && jb >= 1)
+            for (int g : ids)
+                if (nb[g] > jb + 1)
+                    p.items.push_back(make_int2(g, -1));
         // rows first, so the workgroups of one GP's column run side by side
         for (int g : ids)
             for (int i = jb + 1; i < nb[g]; i++)
@@ -299,6 +306,21 @@ std::vector<int> batch_run(Context &c, const std::vector<k::BatchGP> &gps, doubl
         }
     }
 
+    // the FP64 64-wide level steps carry each next diagonal block's first
+    // sums from one launch to the next
+    long pre_count = 0;
+    for (int g : groups[0].ids)
+        pre_count += nb[g] >= 3 ? 1 : 0;
+    DevBuf<double> pre;
+    if (pre_count > 0)
+    {
+        pre = DevBuf<double>(c, static_cast<std::size_t>(pre_count) * k::BATCH_CARRY_DOUBLES);
+        long slot = 0;
+        for (int g : groups[0].ids)
+            if (nb[g] >= 3)
+                dgps[g].pre = pre.get() + (slot++) * k::BATCH_CARRY_DOUBLES;
+    }
+
     DevBuf<double> desc(c, (sizeof(k::BatchGP) * count + 7) / 8);
     DevBuf<int> dorder(c, count), info(c, count), counter(c, 2);
     DevBuf<double> losses(c, count);
@@ -324,7 +346,7 @@ std::vector<int> batch_run(Context &c, const std::vector<k::BatchGP> &gps, doubl
     for (auto &gr : groups)
         if (!gr.ids.empty())
         {
-            gr.plan = level_plan(gr.ids, nb, gr.wide ? 128 : 64);
+            gr.plan = level_plan(gr.ids, nb, gr.wide ? 128 : 64, !gr.mixed && !gr.wide);
             gr.base = item_total;
             item_total += static_cast<long>(gr.plan.items.size());
         }

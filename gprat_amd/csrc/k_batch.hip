@@ -140,18 +140,50 @@ __device__ __forceinline__ void mfma_nt_acc32(const float *A, const float *B, v4
     }
 }
 
+// Carried state of a partial accumulate_row_products (a diagonal block's
+// sum over its first k-blocks, computed one launch early): each thread's 16
+// accumulator entries, then its forward-solve partial, at
+// carry[q * THREADS + tid] (q < 16) and carry[16 * THREADS + tid].
+static_assert(BATCH_CARRY_DOUBLES == 17 * THREADS, "carry layout");
+
+__device__ __forceinline__ void save_carry(double *carry, const v4d (&acc)[2][2], double tp)
+{
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+        carry[q * THREADS + threadIdx.x] = acc[q >> 3][(q >> 2) & 1][q & 3];
+    carry[16 * THREADS + threadIdx.x] = tp;
+}
+
+__device__ __forceinline__ void load_carry(const double *carry, v4d (&acc)[2][2], double &tp)
+{
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+        acc[q >> 3][(q >> 2) & 1][q & 3] = carry[q * THREADS + threadIdx.x];
+    tp = carry[16 * THREADS + threadIdx.x];
+}
+
 // acc = sum_{k < kn} L[i, k] L[j, k]^T over 64-blocks of block rows i and j
 // (block (r, k) at L + r*64*np + k*64), streamed through the LDS stage with
 // the next pair prefetched into registers. With zv, also tout = L[i, 0:kn]
 // zv[0:kn] (64 values, fixed order: each block's 64 columns in four
 // 16-column parts per row, the parts summed in a fixed order at the end),
-// from the blocks of row i while they are in LDS.
+// from the blocks of row i while they are in LDS. With carry, the sums over
+// k < k0 come from there (save_carry of the same sums) and the loop starts
+// at block k0: every accumulator then sees the same operations in the same
+// order as without.
 __device__ __forceinline__ void accumulate_row_products(const double *L, long np, int ib, int jb, int kn, Stage &st, v4d (&acc)[2][2],
-                                                        const double *zv = nullptr, double *tout = nullptr)
+                                                        const double *zv = nullptr, double *tout = nullptr, int k0 = 0,
+                                                        const double *carry = nullptr)
 {
-    blk::zero(acc);
     const int trow = threadIdx.x & 63, tq = threadIdx.x >> 6;
     double tp = 0.0;
+    if (carry)
+        load_carry(carry, acc, tp);
+    else
+    {
+        blk::zero(acc);
+        k0 = 0;
+    }
     if (kn == 0)
     {
         if (tout && threadIdx.x < NB)
@@ -161,9 +193,9 @@ __device__ __forceinline__ void accumulate_row_products(const double *L, long np
     const double *ai = L + static_cast<long>(ib) * NB * np;
     const double *bj = L + static_cast<long>(jb) * NB * np;
     BlockRegs ra, rb;
-    ra.load(ai, np);
-    rb.load(bj, np);
-    for (int kb = 0; kb < kn; kb++)
+    ra.load(ai + k0 * NB, np);
+    rb.load(bj + k0 * NB, np);
+    for (int kb = k0; kb < kn; kb++)
     {
         __syncthreads();  // every wavefront is done with the previous pair
         ra.store(st.a);
@@ -541,7 +573,8 @@ __device__ __forceinline__ void gen_block(const BatchGP &g, int ib, int jb, doub
 // is reduced exactly once, so no pass assembles K first), through the
 // stage's first block.
 template <bool MIXED>
-__device__ __forceinline__ void reduced_block(const BatchGP &g, int ib, int jb, Shared &sh, double (&kij)[16], bool forward = false)
+__device__ __forceinline__ void reduced_block(const BatchGP &g, int ib, int jb, Shared &sh, double (&kij)[16], bool forward = false,
+                                              const double *carry = nullptr)
 {
     const long np = g.n_pad;
     gen_block(g, ib, jb, sh.st.a);
@@ -565,7 +598,7 @@ __device__ __forceinline__ void reduced_block(const BatchGP &g, int ib, int jb, 
     else
     {
         v4d acc[2][2];
-        accumulate_row_products(g.L, np, ib, jb, jb, sh.st, acc, forward ? g.alpha : nullptr, forward ? sh.fa.T : nullptr);
+        accumulate_row_products(g.L, np, ib, jb, jb, sh.st, acc, forward ? g.alpha : nullptr, forward ? sh.fa.T : nullptr, jb - 1, carry);
 #pragma unroll
         for (int q = 0; q < 16; q++)
             kij[q] -= acc[q >> 3][(q >> 2) & 1][q & 3];
@@ -578,12 +611,14 @@ __device__ __forceinline__ void reduced_block(const BatchGP &g, int ib, int jb, 
 // step also takes alpha's forward solve one block further: the products of
 // row block jb with z_0:jb come from the blocks the reduction streams
 // through LDS anyway, and z_jb = W_jj (y_jb - those) goes to alpha.
+// carry: the block's sums over its first jb - 1 k-blocks (FP64 only), from
+// the previous level launch (diag_prefix_halves).
 template <bool MIXED>
-__device__ __forceinline__ void diag_block(const BatchGP &g, int gi, int jb, Shared &sh, double *wjj, int *info)
+__device__ __forceinline__ void diag_block(const BatchGP &g, int gi, int jb, Shared &sh, double *wjj, int *info, const double *carry = nullptr)
 {
     const long np = g.n_pad;
     double kij[16];
-    reduced_block<MIXED>(g, jb, jb, sh, kij, !MIXED);
+    reduced_block<MIXED>(g, jb, jb, sh, kij, !MIXED, carry);
 #pragma unroll
     for (int q = 0; q < 16; q++)
     {
@@ -886,6 +921,45 @@ __device__ __forceinline__ void off_block_halves(const BatchGP &g, int ib, int j
     blk::store_acc(g.L + static_cast<long>(ib) * NB * np + jb * NB, np, xb, true, false);
 }
 
+// The first jn - 1 k-blocks of diagonal block jn's reduction (FP64), run in
+// the off-diagonal launch of column jn - 1 beside that column's blocks, so
+// the diagonal launch of column jn only adds block jn - 1 before it factors.
+// The same MFMAs in the same order as accumulate_row_products (row block
+// jn is both operands, staged once per half), and the same per-thread
+// forward-solve sums: thread (row, part) takes its 16 columns of a block
+// from the half that holds them. The state goes to g.pre (save_carry).
+__device__ __forceinline__ void diag_prefix_halves(const BatchGP &g, int jn, HalfShared &sh)
+{
+    const long np = g.n_pad;
+    const int trow = threadIdx.x & 63, tq = threadIdx.x >> 6;
+    v4d acc[2][2];
+    blk::zero(acc);
+    double tp = 0.0;
+    const int steps = 2 * (jn - 1);
+    const double *aj = g.L + static_cast<long>(jn) * NB * np;
+    d2 ra[4];
+    blk::load_half(aj, np, ra);
+    for (int t = 0; t < steps; t++)
+    {
+        __syncthreads();  // every wavefront is done with the previous half
+        blk::store_half(sh.h.a, ra);
+        __syncthreads();
+        if (t + 1 < steps)
+            blk::load_half(aj + (t + 1) * 32, np, ra);
+        __builtin_amdgcn_sched_barrier(0);
+        blk::mfma_nt_acc_half(sh.h.a, sh.h.a, acc);
+        if ((tq >> 1) == (t & 1))
+        {
+            const double *zb = g.alpha + (t >> 1) * NB + tq * 16;
+            const double *arow = sh.h.a + trow * XH + (tq & 1) * 16;
+#pragma unroll 4
+            for (int c = 0; c < 16; c++)
+                tp = fma(arow[c], zb[c], tp);
+        }
+    }
+    save_carry(g.pre, acc, tp);
+}
+
 // Level-synchronous path for large GPs, one launch per step over all of
 // them: workgroup w takes item w of its launch's list (GP index and block
 // row). The same block functions run in the same order per block as in the
@@ -904,7 +978,8 @@ __global__ __launch_bounds__(THREADS) void k_batch_diag_level(const BatchGP *__r
     __shared__ Shared sh;
     const int2 it = items[blockIdx.x];
     const BatchGP g = gps[it.x];
-    diag_block<MIXED>(g, it.x, jb, sh, nullptr, info);
+    // FP64 steps from column 2 on continue the sums the previous launch began
+    diag_block<MIXED>(g, it.x, jb, sh, nullptr, info, !MIXED && jb >= 2 ? g.pre : nullptr);
 }
 
 template <bool MIXED>
@@ -920,7 +995,10 @@ __global__ __launch_bounds__(THREADS) void k_batch_off_level(const BatchGP *__re
     else
     {
         __shared__ HalfShared sh;
-        off_block_halves(g, it.y, jb, sh);
+        if (it.y < 0)
+            diag_prefix_halves(g, jb + 1, sh);  // item {GP, -1}: the next diagonal block's first sums
+        else
+            off_block_halves(g, it.y, jb, sh);
     }
 }
 

@@ -181,7 +181,11 @@ struct BatchGP
     int n, n_pad, n_reg, pad_;
     double lengthscale, vertical, noise;
     double *W128;  // 128-wide level path only: ceil(n_pad / 128) row-major 128x128 block-column inverses
+    double *pre;   // 64-wide FP64 level path, n_pad >= 192: BATCH_CARRY_DOUBLES, the next diagonal block's partial sums
 };
+// Doubles of a BatchGP's `pre` (one thread's 16 accumulator entries and one
+// forward-solve partial per thread of a 256-thread workgroup).
+constexpr int BATCH_CARRY_DOUBLES = 17 * 256;
 // Persistent fit of `count` GPs (visited in `order`, largest first) by
 // `workgroups` workgroups; losses[g] gets GP g's loss, info[g] its first
 // non-positive pivot column (INT_MAX if none). mixed: the block products
@@ -200,6 +204,10 @@ void batch_info_reset(hipStream_t s, int *info, int count);
 // Diagonal (items: GPs with more than jb blocks) and OffDiagonal (items:
 // blocks ib > jb of those GPs), then Finish (items: every GP; alpha, its
 // FP64 refinement for mixed, the loss).
+// FP64 OffDiagonal launches also take items {GP, -1}: the first jb sums of
+// the GP's diagonal block jb + 1 (for jb >= 1), which the Diagonal launch of
+// column jb + 1 then continues (the GP needs `pre`); the rounding is that
+// of the whole sum in one launch.
 // Diagonal128 / OffDiagonal128: the same steps on 128-wide block columns
 // (FP64 only; each GP needs W128; its rounding differs from the 64-wide
 // steps').
