@@ -277,9 +277,9 @@ def c2_oneshot_leg(capi, ctx, steps, warmup):
     c2, f, mdl, _ = request_leg(capi, ctx, n16, N_TILES, N_REG, N_TEST, THETA, capi.PRECISION_FP64, steps, warmup=warmup,
                                 families=fams, oneshot=True)
     mdl.release()
-    c2["api"] = "gprat_predict_with_uncertainty (one call: fit + predict, test rows factored with K)"
+    c2["api"] = "gprat_predict_with_uncertainty (one call: fit + predict, test rows and targets factored with K)"
     m_pad = ((N_TEST + 63) // 64) * 64
-    tr = trailing_update_flops(n16, m_pad)
+    tr = trailing_update_flops(n16, m_pad + 64)  # the test rows, then the targets' 64-row block
     c2["roofline_trailing_update"] = roofline_of(
         "k_gemm_f64<true,false,5> (FP64 trailing update, test rows included)", sum(tr) / len(tr), f["syrk_trailing"],
         FP64_MFMA_PEAK_TFLOPS, "trailing_update_oneshot_n16384_traffic.md")
@@ -730,7 +730,7 @@ def main():
         mdl.release()
         one, _, mdl, (mo, vo, *_r) = request_leg(capi, ctx, HN, HT, HR, HM, HTHETA, capi.PRECISION_FP64, 3, warmup=1,
                                                  oneshot=True)
-        one["api"] = "gprat_predict_with_uncertainty (one call, FP64, test rows factored with K)"
+        one["api"] = "gprat_predict_with_uncertainty (one call, FP64, test rows and targets factored with K)"
         one["rel_err_vs_oracle"] = vs_oracle_n32768(mo, vo, mdl.loss(), mdl.gradient()[1])
         mdl.release()
         result["fp64_n32k"] = f64

@@ -305,7 +305,7 @@ static int predict_common(gprat_context *ctx,
             std::lock_guard<std::recursive_mutex> g(c.mu);
             if (want_var && t.m > 0)
             {
-                // the test rows ride along in the factorisation
+                // the test rows and the targets ride along in the factorisation
                 gpamd::predict_with_uncertainty_once(c, train_in, train_in_len, train_out, train_out_len, t.n, n_reg, theta, test_in,
                                                      test_in_len, t.m, mean_out, var_out);
                 return;

@@ -1388,6 +1388,7 @@ void predict_with_uncertainty_once(Context &c, const double *x, std::size_t x_le
     require(x != nullptr && x_len >= static_cast<std::size_t>(n) + r - 1, "training input must hold n + n_reg - 1 values");
     require(z != nullptr && z_len >= static_cast<std::size_t>(mt) + r - 1, "predict: test input must hold m + n_reg - 1 values");
     require(theta != nullptr && mean != nullptr && var != nullptr, "predict: null argument");
+    require(y != nullptr && y_len >= static_cast<std::size_t>(n), "training output must hold n values");
     c.make_current();
     Model m;
     m.ctx = &c;
@@ -1398,9 +1399,11 @@ void predict_with_uncertainty_once(Context &c, const double *x, std::size_t x_le
     m.sek = SEK{ theta[0], theta[1], theta[2] };
     m.precision = GPRAT_PRECISION_FP64;
     const int m_pad = static_cast<int>(round_up(mt, 64));
+    // below K: the m test rows of K*, then a 64-row block whose first row is y
+    const int extra = m_pad + 64;
     const std::size_t xl = static_cast<std::size_t>(n) + r - 1, zl = static_cast<std::size_t>(mt) + r - 1;
     m.x = DevBuf<double>(c, xl);
-    m.L = DevBuf<double>(c, (static_cast<std::size_t>(m.n_pad) + m_pad) * m.lda);
+    m.L = DevBuf<double>(c, (static_cast<std::size_t>(m.n_pad) + extra) * m.lda);
     m.Winv = DevBuf<double>(c, static_cast<std::size_t>(m.n_pad) * 64);
     m.wblock = panel_width(m.n_pad);
     m.Wpanel = DevBuf<double>(c, static_cast<std::size_t>(ceil_div(m.n_pad, m.wblock)) * m.wblock * m.wblock);
@@ -1408,19 +1411,22 @@ void predict_with_uncertainty_once(Context &c, const double *x, std::size_t x_le
     upload(c, m.x.get(), x, xl);
     upload(c, dz.get(), z, zl);
     double *rows = m.L.get() + static_cast<long>(m.n_pad) * m.lda;
+    double *yrow = rows + static_cast<long>(m_pad) * m.lda;
     {
         Timed t(c, c.main, "assemble");
         k::assemble_lower(c.main, m.x.get(), r, m.sek, m.L.get(), m.lda, n, m.n_pad);
         k::assemble_cross_rows(c.main, m.x.get(), dz.get(), r, m.sek, rows, m.lda, n, m.n_pad, mt, m_pad);
+        k::fill(c.main, yrow, 64L * m.lda, 0.0);
+        GP_HIP(hipMemcpyAsync(yrow, y, sizeof(double) * n, hipMemcpyDefault, c.main));
     }
-    cholesky(c, m.L.get(), m.lda, m.n_pad, m.Winv.get(), false, m.Wpanel.get(), m_pad);
-    attach_targets(m, y, y_len);
-    const int splits = k::mean_splits(m.n, mt);
-    DevBuf<double> partial(c, static_cast<std::size_t>(splits) * mt);
+    // the rows below K come out as K* L^-T (test row t: v_t = L^-1 k_t) and
+    // (L^-1 y)^T, so the mean is K* K^-1 y = V (L^-1 y) and the variance
+    // v - |v_t|^2: no alpha and no separate solve
+    cholesky(c, m.L.get(), m.lda, m.n_pad, m.Winv.get(), false, m.Wpanel.get(), extra);
     DevBuf<double> dmean(c, mt), colsq(c, mt), dvar(c, mt);
     {
         Timed t(c, c.main, "mean");
-        k::predict_mean(c.main, m.x.get(), dz.get(), m.alpha.get(), m.r, m.sek, m.n, mt, partial.get(), splits, dmean.get());
+        k::rowdot(c.main, rows, m.lda, m.n_pad, mt, yrow, dmean.get(), false);
     }
     {
         Timed t(c, c.main, "variance_rows");

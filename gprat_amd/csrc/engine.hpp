@@ -288,7 +288,8 @@ void fit(Context &c,
          const double theta[3],
          int precision);
 void predict(Model &m, const double *z, std::size_t z_len, int mt, double *mean, double *var);
-// Fit and predict in one call (FP64): mean and latent variance at mt test points.
+// Fit and predict in one call (FP64): mean and latent variance at mt test
+// points, with the rows of K* and the targets factored below K.
 void predict_with_uncertainty_once(Context &c, const double *x, std::size_t x_len, const double *y, std::size_t y_len, int n, int r,
                                    const double theta[3], const double *z, std::size_t z_len, int mt, double *mean, double *var);
 void predict_view(const FactorView &m, const double *z, std::size_t z_len, int mt, double *mean, double *var);

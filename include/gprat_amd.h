@@ -130,9 +130,11 @@ int gprat_predict(gprat_context *ctx,
 
 /* Replaces gpu::predict_with_uncertainty (gp_functions.cuh:55-65).
  * mean_out and var_out have m values each. Fit and prediction in one call:
- * the test points' rows of K* are factored together with K (their forward
- * solve rides in the factorisation); the results agree with a resident
- * model's to rounding. */
+ * the test points' rows of K* and a row holding the training targets are
+ * factored together with K, so they come out as K* L^-T and (L^-1 y)^T
+ * (the forward solves ride in the factorisation); the mean is their
+ * product and the variance v minus each test row's sum of squares. The
+ * results agree with a resident model's to rounding. */
 int gprat_predict_with_uncertainty(gprat_context *ctx,
                                    const double *train_in,
                                    size_t train_in_len,

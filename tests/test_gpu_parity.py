@@ -60,9 +60,10 @@ def test_predict_with_uncertainty(ctx, oracle, case):
     ofm, ofv = gp.predict_with_full_cov(list(z), mt, mb)
     assert norm_rel(fm, ofm) < TOL and norm_rel(fv, ofv) < TOL
     pm = ctx.predict(x, y, z, theta, T, b, mt, mb, r)
-    # predict factors K alone, predict_with_uncertainty with the test rows
-    # below it (the same kernels; the larger trailing updates may be split
-    # over workgroups differently), so they agree to rounding
+    # predict factors K alone and forms K* alpha; predict_with_uncertainty
+    # factors the test rows and the targets below K (the same kernels; the
+    # larger trailing updates may be split over workgroups differently) and
+    # forms the mean as (K* L^-T)(L^-1 y), so they agree to rounding
     assert norm_rel(pm, gm) < 1e-11
 
 
