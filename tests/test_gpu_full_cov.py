@@ -77,7 +77,10 @@ def test_stateless_full_posterior_diagonal_matches_full_cov(ctx):
     theta = (1.0, 1.0, 0.1)
     mean, cov = ctx.predict_full_posterior(x, y, z, theta, T, b, m_tiles, m_b, r)
     mean2, var = ctx.predict(x, y, z, theta, T, b, m_tiles, m_b, r, full_cov=True)
-    assert np.array_equal(mean, mean2)
+    # the full posterior comes from a resident model (mean K* alpha); the
+    # diagonal entry point factors the test rows and the targets with K
+    # (mean (K* L^-T)(L^-1 y)): equal to rounding
+    assert norm_rel(mean, mean2) < 1e-12
     assert norm_rel(np.diag(cov), var) < 1e-12
     assert norm_rel(cov, _dense_sigma(x, z, T * b, m_tiles * m_b, r, theta)) < TOL
 
@@ -96,7 +99,8 @@ def test_facade_full_posterior(golden_dir):
     mean, cov = gp.predict_full_posterior(zi.data, *mt)
     fmean, fvar = gp.predict_with_full_cov(zi.data, *mt)
     cov = np.asarray(cov).reshape(n_test, n_test)
-    assert np.array_equal(np.asarray(mean), np.asarray(fmean))
+    # resident-model mean against the one call's (see above): equal to rounding
+    assert norm_rel(np.asarray(mean), np.asarray(fmean)) < 1e-12
     assert norm_rel(np.diag(cov), fvar) < 1e-12
     sigma = _dense_sigma(np.asarray(xi.data), np.asarray(zi.data), n_train, n_test, n_reg, (1.0, 1.0, 0.1))
     assert norm_rel(cov, sigma) < TOL
