@@ -153,7 +153,8 @@ int gprat_predict_with_uncertainty(gprat_context *ctx,
 
 /* Replaces gpu::predict_with_full_cov (gp_functions.cuh:83-93). Like the
  * reference it returns the mean and the diagonal of the posterior
- * covariance (core/src/cpu/gp_functions.cpp:663-695). */
+ * covariance (core/src/cpu/gp_functions.cpp:663-695), computed as
+ * gprat_predict_with_uncertainty computes them. */
 int gprat_predict_with_full_cov(gprat_context *ctx,
                                 const double *train_in,
                                 size_t train_in_len,
