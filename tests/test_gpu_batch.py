@@ -58,6 +58,33 @@ def test_batch_matches_single_fits(ctx):
         batch.release()
 
 
+def test_largest_batched_gps_match_single_fits(ctx):
+    """The top of the batched workload's size range (n up to 2048: the
+    64-wide level path with its carried diagonal sums, up to 32 block
+    columns) against single-GP fits: mean, variance and loss to 1e-10."""
+    from gprat_amd import capi, datagen
+    sizes = [(1537, 4, (0.4, 1.7, 0.02)), (1800, 16, (2.5, 0.6, 0.25)), (1999, 8, (1.0, 1.0, 0.1)), (2048, 8, (3.0, 2.0, 0.01))]
+    probs = []
+    for g, (n, r, t) in enumerate(sizes):
+        x, y, z = datagen.gp_problem(n, 64, r, seed_train=500 + 2 * g, seed_test=501 + 2 * g)
+        probs.append((x, y, n, r, t, z))
+    batch = capi.Batch(ctx, [(x, y, n, r, t) for x, y, n, r, t, _ in probs])
+    try:
+        losses = batch.losses()
+        for g, (x, y, n, r, t, z) in enumerate(probs):
+            single = capi.Model(ctx, x, y, n, r, t)
+            sm, sv = single.predict(z, 64)
+            sl = single.loss()
+            single.release()
+            bm, bv = batch.predict(g, z, 64)
+            print(f"n={n}: mean {norm_rel(bm, sm):.2e} var {norm_rel(bv, sv):.2e} loss {abs(losses[g] - sl) / abs(sl):.2e}")
+            assert norm_rel(bm, sm) < 1e-10, g
+            assert norm_rel(bv, sv) < 1e-10, g
+            assert abs(losses[g] - sl) <= 1e-10 * abs(sl), g
+    finally:
+        batch.release()
+
+
 def test_batch_matches_oracle(ctx, oracle):
     from gprat_amd import capi
     probs = _problems(6, seed=9, n_lo=100, n_hi=400)
