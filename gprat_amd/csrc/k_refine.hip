@@ -42,13 +42,22 @@ __global__ __launch_bounds__(THREADS) void k_transpose_blocks_f32(const double *
         WT[base + static_cast<long>(k0 + kk) * nb + i0 + c] = t[c][kk];  // WT[k0 + kk][i0 + c] = W[i0 + c][k0 + kk]
 }
 
-// R[k][i] = (float) A[(r0 + k) * lda + i], k < rows, i < cols.
+// R[k][i] = (float) A[(r0 + k) * lda + i], k < rows, i < cols; four
+// columns per thread (two 16-byte loads, one 16-byte store).
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(THREADS) void k_rows_to_f32(const double *__restrict__ A, long lda, long r0, int rows, int cols,
                                                           float *__restrict__ R, long ldr)
 {
     const int k = blockIdx.y;
-    for (long i = static_cast<long>(blockIdx.x) * THREADS + threadIdx.x; i < cols; i += static_cast<long>(gridDim.x) * THREADS)
-        R[k * ldr + i] = static_cast<float>(A[(r0 + k) * lda + i]);
+    const double *src = A + (r0 + k) * lda;
+    float *dst = R + k * ldr;
+    for (long i = (static_cast<long>(blockIdx.x) * THREADS + threadIdx.x) * 4; i < cols; i += static_cast<long>(gridDim.x) * THREADS * 4)
+    {
+        const d2v a = *reinterpret_cast<const d2v *>(src + i);
+        const d2v b = *reinterpret_cast<const d2v *>(src + i + 2);
+        *reinterpret_cast<f4v *>(dst + i) = f4v{ static_cast<float>(a.x), static_cast<float>(a.y), static_cast<float>(b.x), static_cast<float>(b.y) };
+    }
 }
 
 __global__ __launch_bounds__(THREADS) void k_to_f64(const float *__restrict__ src, double *__restrict__ dst, long count)
@@ -203,9 +212,12 @@ void transpose_blocks_f32(hipStream_t s, const double *W, float *WT, int nb, int
 void rows_to_f32(hipStream_t s, const double *A, long lda, long r0, int rows, int cols, float *R, long ldr)
 {
     require(rows >= 0 && cols >= 0 && ldr >= cols && rows <= 65535, "rows_to_f32: bad shape");
+    require(cols % 4 == 0 && lda % 2 == 0 && ldr % 4 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0
+                && reinterpret_cast<uintptr_t>(R) % 16 == 0,
+            "rows_to_f32: rows must come in aligned groups of four");
     if (rows == 0 || cols == 0)
         return;
-    dim3 grid(static_cast<unsigned>(std::min<long>(ceil_div(cols, THREADS), 64)), static_cast<unsigned>(rows));
+    dim3 grid(static_cast<unsigned>(std::min<long>(ceil_div(cols, 4 * THREADS), 64)), static_cast<unsigned>(rows));
     hipLaunchKernelGGL(k_rows_to_f32, grid, dim3(THREADS), 0, s, A, lda, r0, rows, cols, R, ldr);
     GP_CHECK_LAUNCH();
 }
