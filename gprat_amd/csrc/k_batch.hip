@@ -296,18 +296,16 @@ __device__ __forceinline__ void forward_solve(const double *L, const double *Wg,
         for (long i = base + NB + tid; i < np; i += THREADS)
         {
             const double *lr = L + i * np + base;
+            // the row's 64 values all in flight before its products, which
+            // run in k order
+            d2 v[NB / 2];
+#pragma unroll
+            for (int q = 0; q < NB / 2; q++)
+                v[q] = *reinterpret_cast<const d2 *>(lr + 2 * q);
             double s = 0.0;
-            // 16 values of the row in flight at a time
-            for (int k0 = 0; k0 < NB; k0 += 16)
-            {
-                d2 v[8];
 #pragma unroll
-                for (int q = 0; q < 8; q++)
-                    v[q] = *reinterpret_cast<const d2 *>(lr + k0 + 2 * q);
-#pragma unroll
-                for (int q = 0; q < 8; q++)
-                    s += v[q].x * xs[k0 + 2 * q] + v[q].y * xs[k0 + 2 * q + 1];
-            }
+            for (int q = 0; q < NB / 2; q++)
+                s += v[q].x * xs[2 * q] + v[q].y * xs[2 * q + 1];
             vec[i] -= s;
         }
         __syncthreads();
@@ -345,19 +343,18 @@ __device__ __forceinline__ void backward_solve(const double *L, const double *Wg
             vec[base + tid] = xs[tid];
         }
         __syncthreads();
+        // a column's 64 values all in flight before its products (a step
+        // is a chain of memory latencies otherwise); products in k order
         for (long jcol = tid; jcol < base; jcol += THREADS)
         {
+            double v[NB];
+#pragma unroll
+            for (int q = 0; q < NB; q++)
+                v[q] = L[(base + q) * np + jcol];
             double s = 0.0;
-            for (int k0 = 0; k0 < NB; k0 += 16)
-            {
-                double v[16];
 #pragma unroll
-                for (int q = 0; q < 16; q++)
-                    v[q] = L[(base + k0 + q) * np + jcol];
-#pragma unroll
-                for (int q = 0; q < 16; q++)
-                    s += v[q] * xs[k0 + q];
-            }
+            for (int q = 0; q < NB; q++)
+                s += v[q] * xs[q];
             vec[jcol] -= s;
         }
         __syncthreads();
