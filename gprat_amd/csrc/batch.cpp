@@ -149,17 +149,18 @@ Route route(const Tuning &t, int n_pad, bool mixed_requested)
 // in nb): Assemble (every block row), per column jb Diagonal (GPs with more
 // than jb blocks) and OffDiagonal (their blocks below jb, and with prefix,
 // for jb >= 1, {GP, -1} for the GPs with more than jb + 1 blocks: their
-// next diagonal block's first sums), then Finish.
+// next diagonal block's first sums), then Finish (every GP) and, with
+// refine, every 64-row block for the refinement's residual.
 struct LevelPlan
 {
     std::vector<int2> items;
-    long assemble_off = 0, assemble_n = 0, finish_off = 0, finish_n = 0;
+    long assemble_off = 0, assemble_n = 0, finish_off = 0, finish_n = 0, rows_off = 0, rows_n = 0;
     std::vector<long> diag_off, diag_n, off_off, off_n;
 };
 
 // nb: 64-row blocks per GP; width: the column width of the factorisation
 // steps (64 or 128; blocks per step column = ceil(nb * 64 / width)).
-LevelPlan level_plan(const std::vector<int> &ids, const std::vector<int> &nb64, int width, bool prefix)
+LevelPlan level_plan(const std::vector<int> &ids, const std::vector<int> &nb64, int width, bool prefix, bool refine)
 {
     LevelPlan p;
     std::vector<int> nb(nb64.size());
@@ -199,6 +200,13 @@ LevelPlan level_plan(const std::vector<int> &ids, const std::vector<int> &nb64, 
     for (int g : ids)
         p.items.push_back(make_int2(g, 0));
     p.finish_n = static_cast<long>(p.items.size()) - p.finish_off;
+    // the mixed-precision refinement's residual: every 64-row block
+    p.rows_off = static_cast<long>(p.items.size());
+    if (refine)
+        for (int g : ids)
+            for (int i = 0; i < nb64[g]; i++)
+                p.items.push_back(make_int2(g, i));
+    p.rows_n = static_cast<long>(p.items.size()) - p.rows_off;
     return p;
 }
 
@@ -216,8 +224,22 @@ void launch_level(hipStream_t s, const LevelPlan &p, const int2 *items, const k:
         k::batch_level_step(s, off, gps, items + p.off_off[jb], static_cast<int>(p.off_n[jb]), static_cast<int>(jb), mixed, losses, info,
                             steps);
     }
-    k::batch_level_step(s, wide ? k::BatchLevelStep::Finish128 : k::BatchLevelStep::Finish, gps, items + p.finish_off,
-                        static_cast<int>(p.finish_n), 0, mixed, losses, info, steps);
+    const int2 *fin = items + p.finish_off;
+    const int nfin = static_cast<int>(p.finish_n);
+    if (!mixed)
+    {
+        k::batch_level_step(s, wide ? k::BatchLevelStep::Finish128 : k::BatchLevelStep::Finish, gps, fin, nfin, 0, false, losses, info,
+                            steps);
+        return;
+    }
+    k::batch_level_step(s, k::BatchLevelStep::RefineStart, gps, fin, nfin, 0, true, losses, info, steps);
+    for (int it = 0; it < k::BATCH_REFINE_STEPS; it++)
+    {
+        k::batch_level_step(s, k::BatchLevelStep::RefineResidual, gps, items + p.rows_off, static_cast<int>(p.rows_n), 0, true, losses,
+                            info, steps);
+        k::batch_level_step(s, k::BatchLevelStep::RefineCorrect, gps, fin, nfin, it, true, losses, info, steps);
+    }
+    k::batch_level_step(s, k::BatchLevelStep::RefineFinish, gps, fin, nfin, 0, true, losses, info, steps);
 }
 
 }  // namespace
@@ -307,18 +329,28 @@ std::vector<int> batch_run(Context &c, const std::vector<k::BatchGP> &gps, doubl
     }
 
     // the FP64 64-wide level steps carry each next diagonal block's first
-    // sums from one launch to the next
-    long pre_count = 0;
+    // sums from one launch to the next; the mixed ones, the refinement's
+    // state
+    long pre_total = 0;
     for (int g : groups[0].ids)
-        pre_count += nb[g] >= 3 ? 1 : 0;
+        pre_total += nb[g] >= 3 ? k::BATCH_CARRY_DOUBLES : 0;
+    pre_total += static_cast<long>(groups[1].ids.size()) * k::BATCH_REFINE_STATE_DOUBLES;
     DevBuf<double> pre;
-    if (pre_count > 0)
+    if (pre_total > 0)
     {
-        pre = DevBuf<double>(c, static_cast<std::size_t>(pre_count) * k::BATCH_CARRY_DOUBLES);
-        long slot = 0;
+        pre = DevBuf<double>(c, static_cast<std::size_t>(pre_total));
+        long off = 0;
         for (int g : groups[0].ids)
             if (nb[g] >= 3)
-                dgps[g].pre = pre.get() + (slot++) * k::BATCH_CARRY_DOUBLES;
+            {
+                dgps[g].pre = pre.get() + off;
+                off += k::BATCH_CARRY_DOUBLES;
+            }
+        for (int g : groups[1].ids)
+        {
+            dgps[g].pre = pre.get() + off;
+            off += k::BATCH_REFINE_STATE_DOUBLES;
+        }
     }
 
     DevBuf<double> desc(c, (sizeof(k::BatchGP) * count + 7) / 8);
@@ -346,7 +378,7 @@ std::vector<int> batch_run(Context &c, const std::vector<k::BatchGP> &gps, doubl
     for (auto &gr : groups)
         if (!gr.ids.empty())
         {
-            gr.plan = level_plan(gr.ids, nb, gr.wide ? 128 : 64, !gr.mixed && !gr.wide);
+            gr.plan = level_plan(gr.ids, nb, gr.wide ? 128 : 64, !gr.mixed && !gr.wide, gr.mixed);
             gr.base = item_total;
             item_total += static_cast<long>(gr.plan.items.size());
         }

@@ -261,8 +261,12 @@ __device__ __forceinline__ void accumulate_row_products32(const double *L, long 
 }
 
 // vec <- L^-1 vec for one GP from its factor: forward block substitution
-// with the saved 64x64 inverses (W). xs and part are two 256-double LDS
-// buffers.
+// with the saved 64x64 inverses (W), by a workgroup of NT >= 256 threads.
+// xs and part are two 256-double LDS buffers. The first 256 threads form
+// the products with W_JJ (four fixed-order partial sums per row) and every
+// row's update sums its 64 products in k order, so the result does not
+// depend on NT.
+template <int NT>
 __device__ __forceinline__ void forward_solve(const double *L, const double *Wg, long np, int nblk, double *vec, double *xs, double *part)
 {
     const int tid = threadIdx.x;
@@ -271,6 +275,7 @@ __device__ __forceinline__ void forward_solve(const double *L, const double *Wg,
     {
         const double *W = Wg + static_cast<long>(J) * NB * NB;
         const long base = static_cast<long>(J) * NB;
+        if (tid < THREADS)
         {
             // x_J = W_JJ vec_J: four partial sums per row, fixed order
             double w[16], a[16];
@@ -293,7 +298,7 @@ __device__ __forceinline__ void forward_solve(const double *L, const double *Wg,
             vec[base + tid] = xs[tid];
         }
         __syncthreads();
-        for (long i = base + NB + tid; i < np; i += THREADS)
+        for (long i = base + NB + tid; i < np; i += NT)
         {
             const double *lr = L + i * np + base;
             // the row's 64 values all in flight before its products, which
@@ -312,7 +317,8 @@ __device__ __forceinline__ void forward_solve(const double *L, const double *Wg,
     }
 }
 
-// vec <- L^-T vec: the backward block substitution.
+// vec <- L^-T vec: the backward block substitution (NT as in forward_solve).
+template <int NT>
 __device__ __forceinline__ void backward_solve(const double *L, const double *Wg, long np, int nblk, double *vec, double *xs, double *part)
 {
     const int tid = threadIdx.x;
@@ -321,6 +327,7 @@ __device__ __forceinline__ void backward_solve(const double *L, const double *Wg
     {
         const double *W = Wg + static_cast<long>(J) * NB * NB;
         const long base = static_cast<long>(J) * NB;
+        if (tid < THREADS)
         {
             // x_J = W_JJ^T vec_J
             double w[16], a[16];
@@ -343,18 +350,25 @@ __device__ __forceinline__ void backward_solve(const double *L, const double *Wg
             vec[base + tid] = xs[tid];
         }
         __syncthreads();
-        // a column's 64 values all in flight before its products (a step
-        // is a chain of memory latencies otherwise); products in k order
-        for (long jcol = tid; jcol < base; jcol += THREADS)
+        // a column's values in flight before their products (a step is a
+        // chain of memory latencies otherwise): all 64 with 256 threads,
+        // two halves of 32 with more (the register budget halves); the
+        // products in k order either way
+        constexpr int HALF = NT > THREADS ? NB / 2 : NB;
+        for (long jcol = tid; jcol < base; jcol += NT)
         {
-            double v[NB];
-#pragma unroll
-            for (int q = 0; q < NB; q++)
-                v[q] = L[(base + q) * np + jcol];
             double s = 0.0;
 #pragma unroll
-            for (int q = 0; q < NB; q++)
-                s += v[q] * xs[q];
+            for (int h = 0; h < NB; h += HALF)
+            {
+                double v[HALF];
+#pragma unroll
+                for (int q = 0; q < HALF; q++)
+                    v[q] = L[(base + h + q) * np + jcol];
+#pragma unroll
+                for (int q = 0; q < HALF; q++)
+                    s += v[q] * xs[h + q];
+            }
             vec[jcol] -= s;
         }
         __syncthreads();
@@ -362,19 +376,22 @@ __device__ __forceinline__ void backward_solve(const double *L, const double *Wg
 }
 
 // vec <- K^-1 vec: forward, then backward.
+template <int NT>
 __device__ __forceinline__ void block_solves(const double *L, const double *Wg, long np, int nblk, double *vec, double *xs, double *part)
 {
-    forward_solve(L, Wg, np, nblk, vec, xs, part);
-    backward_solve(L, Wg, np, nblk, vec, xs, part);
+    forward_solve<NT>(L, Wg, np, nblk, vec, xs, part);
+    backward_solve<NT>(L, Wg, np, nblk, vec, xs, part);
 }
 
-// max over the workgroup of v (all threads get it); red holds THREADS doubles
+// max over the workgroup of NT threads of v (all threads get it); red holds
+// NT doubles
+template <int NT>
 __device__ __forceinline__ double block_max(double v, double *red)
 {
     const int tid = threadIdx.x;
     red[tid] = v;
     __syncthreads();
-    for (int w = THREADS / 2; w > 0; w >>= 1)
+    for (int w = NT / 2; w > 0; w >>= 1)
     {
         if (tid < w)
             red[tid] = fmax(red[tid], red[tid + w]);
@@ -385,38 +402,54 @@ __device__ __forceinline__ double block_max(double v, double *red)
     return m;
 }
 
-// r = y - K alpha for one GP (r[i] = 0 for padding rows), K regenerated
-// from the series in FP64: thread = row (its window in registers), the
-// column index is uniform across the workgroup, so the column's window and
-// alpha_j arrive through scalar loads. R: the window length (compile time).
+// r_i = y_i - (K alpha)_i for row i < n of one GP, K regenerated from the
+// series in FP64. The row's window is in registers; the column index is
+// uniform across the wavefront, so the column's window and alpha_j arrive
+// through scalar loads. R: the window length (compile time).
 template <int R>
-__device__ __forceinline__ void residual_rows(const BatchGP &g, double scale, const double *alpha, double *r)
+__device__ __forceinline__ double residual_row_w(const BatchGP &g, double scale, const double *alpha, long i)
 {
     const double *x = g.x;
-    for (long i0 = 0; i0 < g.n_pad; i0 += THREADS)
+    double xi[R];
+#pragma unroll
+    for (int k = 0; k < R; k++)
+        xi[k] = x[i + k];
+    double s = 0.0;
+    for (long j = 0; j < g.n; j++)
     {
-        const long i = i0 + threadIdx.x;
-        const bool act = i < g.n;
-        double xi[R];
+        double d = 0.0;
 #pragma unroll
         for (int k = 0; k < R; k++)
-            xi[k] = act ? x[i + k] : 0.0;
-        double s = 0.0;
-        if (i0 < g.n)
-            for (long j = 0; j < g.n; j++)
-            {
-                double d = 0.0;
-#pragma unroll
-                for (int k = 0; k < R; k++)
-                {
-                    const double t = xi[k] - x[j + k];
-                    d = fma(t, t, d);
-                }
-                s = fma(g.vertical * exp(scale * d), alpha[j], s);
-            }
-        if (i < g.n_pad)
-            r[i] = act ? g.y[i] - (s + g.noise * alpha[i]) : 0.0;
+        {
+            const double t = xi[k] - x[j + k];
+            d = fma(t, t, d);
+        }
+        s = fma(g.vertical * exp(scale * d), alpha[j], s);
     }
+    return g.y[i] - (s + g.noise * alpha[i]);
+}
+
+__device__ __forceinline__ double residual_row(const BatchGP &g, double scale, const double *alpha, long i)
+{
+    if (g.n_reg == 4)
+        return residual_row_w<4>(g, scale, alpha, i);
+    if (g.n_reg == 8)
+        return residual_row_w<8>(g, scale, alpha, i);
+    if (g.n_reg == 16)
+        return residual_row_w<16>(g, scale, alpha, i);
+    double s = 0.0;
+    for (long j = 0; j < g.n; j++)
+        s += g.vertical * exp(scale * window_sqdist(g.x, i, g.x, j, g.n_reg)) * alpha[j];
+    s += g.noise * alpha[i];
+    return g.y[i] - s;
+}
+
+// r = y - K alpha for one GP by the whole workgroup (r[i] = 0 for padding
+// rows).
+__device__ __forceinline__ void residual_rows(const BatchGP &g, double scale, const double *alpha, double *r)
+{
+    for (long i = threadIdx.x; i < g.n_pad; i += THREADS)
+        r[i] = i < g.n ? residual_row(g, scale, alpha, i) : 0.0;
 }
 
 // Rows [r0, r1) (whole 64-row blocks) of one GP's lower band: K on and
@@ -677,100 +710,25 @@ __device__ __forceinline__ void off_block(const BatchGP &g, int ib, int jb, Shar
     __syncthreads();
 }
 
-// alpha (refined in FP64 for MIXED) and the loss of one factored GP. With
-// forward_done, alpha already holds L^-1 y (the 64-wide FP64 diagonal steps
-// compute it) and only the backward substitution is left.
-template <bool MIXED>
-__device__ __forceinline__ void finish_gp(const BatchGP &g, int gi, double (*red)[THREADS], double *losses, int *refine_steps,
-                                          bool forward_done = false)
+// loss = 0.5 (y^T alpha + sum log L_ii^2 + n log 2 pi) / n of one factored
+// GP, summed by the workgroup's first THREADS threads in a fixed order (the
+// same bits for any workgroup size); red: [2][THREADS].
+__device__ __forceinline__ void gp_loss(const BatchGP &g, int gi, double (*red)[THREADS], double *losses)
 {
     const int tid = threadIdx.x;
-    const double *x = g.x;
-    const double *y = g.y;
-    const double *L = g.L;
-    const double *Wg = g.W;
-    double *alpha = g.alpha;
     const long np = g.n_pad;
-    const int nblk = static_cast<int>(np / NB);
-    const double scale = -0.5 / (g.lengthscale * g.lengthscale);
-
-    // alpha: forward then backward block substitution with W
-    double *xs = red[0];
-    double *part = red[1];
-    if (!forward_done)
+    if (tid < THREADS)
     {
-        for (long i = tid; i < np; i += THREADS)
-            alpha[i] = (i < g.n) ? y[i] : 0.0;
-        __syncthreads();
-        forward_solve(L, Wg, np, nblk, alpha, xs, part);
-    }
-    backward_solve(L, Wg, np, nblk, alpha, xs, part);
-    if constexpr (MIXED)
-    {
-        // FP64 refinement: r = y - K alpha with K regenerated from the
-        // series, delta = K^-1 r from the factor, alpha += delta, until
-        // the correction (or the next one, as the contraction so far
-        // predicts it) is at most 1e-12 max|alpha|, or stops shrinking
-        double *r = g.work;
-        double prev = 1e300;
-        int steps = 0;
-        for (int it = 0; it < 10; it++)
+        double s1 = 0.0, s2 = 0.0;
+        for (long i = tid; i < g.n; i += THREADS)
         {
-            if (g.n_reg == 4)
-                residual_rows<4>(g, scale, alpha, r);
-            else if (g.n_reg == 8)
-                residual_rows<8>(g, scale, alpha, r);
-            else if (g.n_reg == 16)
-                residual_rows<16>(g, scale, alpha, r);
-            else
-                for (long i = tid; i < np; i += THREADS)
-                {
-                    double s = 0.0;
-                    if (i < g.n)
-                    {
-                        for (long j = 0; j < g.n; j++)
-                            s += g.vertical * exp(scale * window_sqdist(x, i, x, j, g.n_reg)) * alpha[j];
-                        s += g.noise * alpha[i];
-                        s = y[i] - s;
-                    }
-                    r[i] = s;
-                }
-            __syncthreads();
-            block_solves(L, Wg, np, nblk, r, xs, part);
-            double dmax = 0.0, amax = 0.0;
-            for (long i = tid; i < np; i += THREADS)
-            {
-                dmax = fmax(dmax, fabs(r[i]));
-                amax = fmax(amax, fabs(alpha[i]));
-            }
-            dmax = block_max(dmax, part);
-            amax = block_max(amax, part);
-            // a correction that did not shrink is not applied
-            if (it > 0 && dmax >= prev)
-                break;
-            for (long i = tid; i < np; i += THREADS)
-                alpha[i] += r[i];
-            __syncthreads();
-            steps = it + 1;
-            const bool converged = dmax <= 1e-12 * amax || (it > 0 && dmax * (dmax / prev) <= 1e-12 * amax);
-            if (converged || !(dmax < 0.5 * prev))
-                break;
-            prev = dmax;
+            s1 += g.y[i] * g.alpha[i];
+            const double d = g.L[i * np + i];
+            s2 += log(d * d);
         }
-        if (refine_steps && tid == 0)
-            refine_steps[gi] = steps;
+        red[0][tid] = s1;
+        red[1][tid] = s2;
     }
-
-    // loss = 0.5 (y^T alpha + sum log L_ii^2 + n log 2 pi) / n
-    double s1 = 0.0, s2 = 0.0;
-    for (long i = tid; i < g.n; i += THREADS)
-    {
-        s1 += y[i] * alpha[i];
-        const double d = L[i * np + i];
-        s2 += log(d * d);
-    }
-    red[0][tid] = s1;
-    red[1][tid] = s2;
     __syncthreads();
     for (int w = THREADS / 2; w > 0; w >>= 1)
     {
@@ -789,6 +747,92 @@ __device__ __forceinline__ void finish_gp(const BatchGP &g, int gi, double (*red
         losses[gi] = 0.5 * l / Nn;
     }
     __syncthreads();
+}
+
+// alpha's FP64 refinement (mixed precision): r = y - K alpha with K
+// regenerated from the series, delta = K^-1 r from the factor, alpha +=
+// delta, for at most BATCH_REFINE_STEPS steps, until the correction (or the
+// next one, as the contraction so far predicts it) is at most
+// 1e-12 max|alpha|, or stops shrinking.
+
+// Step `it`'s decision from its correction (dmax = max|delta|, amax =
+// max|alpha|) and the previous step's dmax: whether delta is applied, and
+// whether another step follows.
+struct RefineDecision
+{
+    bool apply, more;
+};
+__device__ __forceinline__ RefineDecision refine_decision(int it, double dmax, double amax, double prev)
+{
+    // a correction that did not shrink is not applied
+    if (it > 0 && dmax >= prev)
+        return { false, false };
+    const bool converged = dmax <= 1e-12 * amax || (it > 0 && dmax * (dmax / prev) <= 1e-12 * amax);
+    return { true, !(converged || !(dmax < 0.5 * prev)) && it + 1 < BATCH_REFINE_STEPS };
+}
+
+// alpha (refined in FP64 for MIXED) and the loss of one factored GP, by a
+// workgroup of NT threads (THREADS when MIXED). With forward_done, alpha
+// already holds L^-1 y (the 64-wide FP64 diagonal steps compute it) and
+// only the backward substitution is left.
+template <bool MIXED, int NT>
+__device__ __forceinline__ void finish_gp(const BatchGP &g, int gi, double (*red)[THREADS], double *losses, int *refine_steps,
+                                          bool forward_done = false)
+{
+    static_assert(NT >= THREADS && (!MIXED || NT == THREADS), "finish_gp: workgroup size");
+    const int tid = threadIdx.x;
+    const double *y = g.y;
+    const double *L = g.L;
+    const double *Wg = g.W;
+    double *alpha = g.alpha;
+    const long np = g.n_pad;
+    const int nblk = static_cast<int>(np / NB);
+
+    // alpha: forward then backward block substitution with W
+    double *xs = red[0];
+    double *part = red[1];
+    if (!forward_done)
+    {
+        for (long i = tid; i < np; i += NT)
+            alpha[i] = (i < g.n) ? y[i] : 0.0;
+        __syncthreads();
+        forward_solve<NT>(L, Wg, np, nblk, alpha, xs, part);
+    }
+    backward_solve<NT>(L, Wg, np, nblk, alpha, xs, part);
+    if constexpr (MIXED)
+    {
+        const double scale = -0.5 / (g.lengthscale * g.lengthscale);
+        double *r = g.work;
+        double prev = 1e300;
+        int steps = 0;
+        for (int it = 0; it < BATCH_REFINE_STEPS; it++)
+        {
+            residual_rows(g, scale, alpha, r);
+            __syncthreads();
+            block_solves<NT>(L, Wg, np, nblk, r, xs, part);
+            double dmax = 0.0, amax = 0.0;
+            for (long i = tid; i < np; i += NT)
+            {
+                dmax = fmax(dmax, fabs(r[i]));
+                amax = fmax(amax, fabs(alpha[i]));
+            }
+            dmax = block_max<NT>(dmax, part);
+            amax = block_max<NT>(amax, part);
+            const RefineDecision d = refine_decision(it, dmax, amax, prev);
+            if (!d.apply)
+                break;
+            for (long i = tid; i < np; i += NT)
+                alpha[i] += r[i];
+            __syncthreads();
+            steps = it + 1;
+            if (!d.more)
+                break;
+            prev = dmax;
+        }
+        if (refine_steps && tid == 0)
+            refine_steps[gi] = steps;
+    }
+    gp_loss(g, gi, red, losses);
 }
 
 // Persistent path: each workgroup takes whole GPs from a device counter
@@ -827,7 +871,7 @@ __global__ __launch_bounds__(THREADS) void k_batch_fit(const BatchGP *__restrict
                 off_block<MIXED>(g, ib, jb, sh, wjj);
         }
         __syncthreads();  // the diagonal steps' alpha entries are visible to every thread
-        finish_gp<MIXED>(g, gi, red, losses, refine_steps, !MIXED);
+        finish_gp<MIXED, THREADS>(g, gi, red, losses, refine_steps, !MIXED);
     }
 }
 
@@ -999,14 +1043,105 @@ __global__ __launch_bounds__(THREADS) void k_batch_off_level(const BatchGP *__re
     }
 }
 
-template <bool MIXED>
-__global__ __launch_bounds__(THREADS) void k_batch_finish_level(const BatchGP *__restrict__ gps, const int2 *__restrict__ items,
-                                                                 double *__restrict__ losses, int *__restrict__ refine_steps, int forward_done)
+// FP64 finish of the level path: alpha's backward substitution (and the
+// forward one unless the diagonal steps did it) and the loss, by a
+// workgroup of SOLVE_THREADS threads: the substitutions stream each GP's
+// factor through one CU, so more wavefronts keep more of it in flight.
+constexpr int SOLVE_THREADS = 512;
+
+__global__ __launch_bounds__(SOLVE_THREADS) void k_batch_finish_level(const BatchGP *__restrict__ gps, const int2 *__restrict__ items,
+                                                                       double *__restrict__ losses, int forward_done)
 {
     __shared__ double red[2][THREADS];
     const int2 it = items[blockIdx.x];
     const BatchGP g = gps[it.x];
-    finish_gp<MIXED>(g, it.x, red, losses, refine_steps, forward_done != 0);
+    finish_gp<false, SOLVE_THREADS>(g, it.x, red, losses, nullptr, forward_done != 0);
+}
+
+// ---- mixed precision: alpha's refinement over several launches ----------
+// finish_gp's refinement loop spread over launches, so that its costly part,
+// the residual (K regenerated in FP64, n^2 exponentials per step), runs on
+// one wavefront per 64-row block instead of one workgroup per GP: a start
+// launch solves alpha = K^-1 y from the factor; each of BATCH_REFINE_STEPS
+// pairs of launches forms r = y - K alpha (per block), then solves for the
+// correction and decides (per GP); a finish launch writes the loss and the
+// step count. A GP's state (previous correction, whether it goes on, steps)
+// lives in g.pre; a GP that stopped returns at once from the remaining
+// launches. Per GP the operations and their order are finish_gp's, so the
+// persistent and level paths give the same bits.
+__global__ __launch_bounds__(SOLVE_THREADS) void k_batch_refine_start(const BatchGP *__restrict__ gps, const int2 *__restrict__ items)
+{
+    __shared__ double red[2][THREADS];
+    const BatchGP g = gps[items[blockIdx.x].x];
+    const long np = g.n_pad;
+    for (long i = threadIdx.x; i < np; i += SOLVE_THREADS)
+        g.alpha[i] = (i < g.n) ? g.y[i] : 0.0;
+    __syncthreads();
+    block_solves<SOLVE_THREADS>(g.L, g.W, np, static_cast<int>(np / NB), g.alpha, red[0], red[1]);
+    if (threadIdx.x == 0)
+    {
+        g.pre[0] = 1e300;  // previous correction
+        g.pre[1] = 1.0;    // refining
+        g.pre[2] = 0.0;    // steps applied
+    }
+}
+
+// Items {GP, block row}: r over the block's 64 rows, one row per lane.
+__global__ __launch_bounds__(NB) void k_batch_refine_residual(const BatchGP *__restrict__ gps, const int2 *__restrict__ items)
+{
+    const int2 it = items[blockIdx.x];
+    const BatchGP g = gps[it.x];
+    if (g.pre[1] == 0.0)
+        return;
+    const double scale = -0.5 / (g.lengthscale * g.lengthscale);
+    const long i = static_cast<long>(it.y) * NB + threadIdx.x;
+    g.work[i] = i < g.n ? residual_row(g, scale, g.alpha, i) : 0.0;
+}
+
+__global__ __launch_bounds__(SOLVE_THREADS) void k_batch_refine_correct(const BatchGP *__restrict__ gps, const int2 *__restrict__ items, int it)
+{
+    __shared__ double xs[NB];
+    __shared__ double red[SOLVE_THREADS];
+    const int tid = threadIdx.x;
+    const BatchGP g = gps[items[blockIdx.x].x];
+    if (g.pre[1] == 0.0)
+        return;
+    const double prev = g.pre[0];
+    const long np = g.n_pad;
+    double *r = g.work;
+    block_solves<SOLVE_THREADS>(g.L, g.W, np, static_cast<int>(np / NB), r, xs, red);
+    double dmax = 0.0, amax = 0.0;
+    for (long i = tid; i < np; i += SOLVE_THREADS)
+    {
+        dmax = fmax(dmax, fabs(r[i]));
+        amax = fmax(amax, fabs(g.alpha[i]));
+    }
+    dmax = block_max<SOLVE_THREADS>(dmax, red);
+    amax = block_max<SOLVE_THREADS>(amax, red);
+    const RefineDecision d = refine_decision(it, dmax, amax, prev);
+    if (d.apply)
+        for (long i = tid; i < np; i += SOLVE_THREADS)
+            g.alpha[i] += r[i];
+    if (tid == 0)
+    {
+        if (d.apply)
+        {
+            g.pre[0] = dmax;
+            g.pre[2] = it + 1;
+        }
+        g.pre[1] = d.more ? 1.0 : 0.0;
+    }
+}
+
+__global__ __launch_bounds__(THREADS) void k_batch_refine_finish(const BatchGP *__restrict__ gps, const int2 *__restrict__ items,
+                                                                  double *__restrict__ losses, int *__restrict__ refine_steps)
+{
+    __shared__ double red[2][THREADS];
+    const int gi = items[blockIdx.x].x;
+    const BatchGP g = gps[gi];
+    if (refine_steps && threadIdx.x == 0)
+        refine_steps[gi] = static_cast<int>(g.pre[2]);
+    gp_loss(g, gi, red, losses);
 }
 
 // ---------------------------------------------------------------------------
@@ -1247,14 +1382,28 @@ void batch_level_step(hipStream_t s, BatchLevelStep step, const BatchGP *gps, co
         break;
     case BatchLevelStep::Finish:
         // the 64-wide FP64 diagonal steps did alpha's forward solve
-        if (mixed)
-            hipLaunchKernelGGL(k_batch_finish_level<true>, grid, dim3(THREADS), 0, s, gps, items, losses, refine_steps, 0);
-        else
-            hipLaunchKernelGGL(k_batch_finish_level<false>, grid, dim3(THREADS), 0, s, gps, items, losses, refine_steps, 1);
+        require(!mixed, "batch: the mixed-precision level path ends with the refinement steps");
+        hipLaunchKernelGGL(k_batch_finish_level, grid, dim3(SOLVE_THREADS), 0, s, gps, items, losses, 1);
         break;
     case BatchLevelStep::Finish128:
         require(!mixed, "batch: the 128-wide level path is FP64");
-        hipLaunchKernelGGL(k_batch_finish_level<false>, grid, dim3(THREADS), 0, s, gps, items, losses, refine_steps, 0);
+        hipLaunchKernelGGL(k_batch_finish_level, grid, dim3(SOLVE_THREADS), 0, s, gps, items, losses, 0);
+        break;
+    case BatchLevelStep::RefineStart:
+        require(mixed, "batch: refinement steps are mixed precision");
+        hipLaunchKernelGGL(k_batch_refine_start, grid, dim3(SOLVE_THREADS), 0, s, gps, items);
+        break;
+    case BatchLevelStep::RefineResidual:
+        require(mixed, "batch: refinement steps are mixed precision");
+        hipLaunchKernelGGL(k_batch_refine_residual, grid, dim3(NB), 0, s, gps, items);
+        break;
+    case BatchLevelStep::RefineCorrect:
+        require(mixed && jb >= 0 && jb < BATCH_REFINE_STEPS, "batch: bad refinement step");
+        hipLaunchKernelGGL(k_batch_refine_correct, grid, dim3(SOLVE_THREADS), 0, s, gps, items, jb);
+        break;
+    case BatchLevelStep::RefineFinish:
+        require(mixed, "batch: refinement steps are mixed precision");
+        hipLaunchKernelGGL(k_batch_refine_finish, grid, dim3(THREADS), 0, s, gps, items, losses, refine_steps);
         break;
     }
     GP_CHECK_LAUNCH();

@@ -181,11 +181,15 @@ struct BatchGP
     int n, n_pad, n_reg, pad_;
     double lengthscale, vertical, noise;
     double *W128;  // 128-wide level path only: ceil(n_pad / 128) row-major 128x128 block-column inverses
-    double *pre;   // 64-wide FP64 level path, n_pad >= 192: BATCH_CARRY_DOUBLES, the next diagonal block's partial sums
+    double *pre;   // 64-wide level path: FP64, n_pad >= 192: BATCH_CARRY_DOUBLES, the next diagonal block's partial sums;
+                   // mixed precision: BATCH_REFINE_STATE_DOUBLES, the state of alpha's refinement between launches
 };
 // Doubles of a BatchGP's `pre` (one thread's 16 accumulator entries and one
 // forward-solve partial per thread of a 256-thread workgroup).
 constexpr int BATCH_CARRY_DOUBLES = 17 * 256;
+constexpr int BATCH_REFINE_STATE_DOUBLES = 4;
+// Most refinement steps of alpha in a mixed-precision batched fit.
+constexpr int BATCH_REFINE_STEPS = 10;
 // Persistent fit of `count` GPs (visited in `order`, largest first) by
 // `workgroups` workgroups; losses[g] gets GP g's loss, info[g] its first
 // non-positive pivot column (INT_MAX if none). mixed: the block products
@@ -202,8 +206,8 @@ void batch_info_reset(hipStream_t s, int *info, int count);
 // functions as the persistent path, so results are bit-identical. Order:
 // Assemble (items: every 64-row block of every GP), then for jb = 0, 1, ...:
 // Diagonal (items: GPs with more than jb blocks) and OffDiagonal (items:
-// blocks ib > jb of those GPs), then Finish (items: every GP; alpha, its
-// FP64 refinement for mixed, the loss).
+// blocks ib > jb of those GPs), then Finish (items: every GP; alpha and
+// the loss) or, for mixed precision, the refinement steps below.
 // FP64 OffDiagonal launches also take items {GP, -1}: the first jb sums of
 // the GP's diagonal block jb + 1 (for jb >= 1), which the Diagonal launch of
 // column jb + 1 then continues (the GP needs `pre`); the rounding is that
@@ -211,6 +215,12 @@ void batch_info_reset(hipStream_t s, int *info, int count);
 // Diagonal128 / OffDiagonal128: the same steps on 128-wide block columns
 // (FP64 only; each GP needs W128; its rounding differs from the 64-wide
 // steps').
+// Mixed precision ends with the refinement of alpha instead of Finish:
+// RefineStart (items: every GP; alpha = K^-1 y from the factor), then for
+// it = 0 .. BATCH_REFINE_STEPS - 1 RefineResidual (items: every 64-row
+// block of every GP) and RefineCorrect (items: every GP, jb = it), then
+// RefineFinish (items: every GP; the loss and the step count). Each GP
+// needs `pre` (BATCH_REFINE_STATE_DOUBLES) and `work`.
 enum class BatchLevelStep
 {
     Assemble,
@@ -219,7 +229,11 @@ enum class BatchLevelStep
     Diagonal128,
     OffDiagonal128,
     Finish,
-    Finish128  // Finish after 128-wide steps (their diagonal steps leave alpha's forward solve to it)
+    Finish128,  // Finish after 128-wide steps (their diagonal steps leave alpha's forward solve to it)
+    RefineStart,
+    RefineResidual,
+    RefineCorrect,
+    RefineFinish
 };
 void batch_level_step(hipStream_t s, BatchLevelStep step, const BatchGP *gps, const int2 *items, int n_items, int jb, bool mixed,
                       double *losses, int *info, int *refine_steps);

@@ -125,30 +125,38 @@ def force(monkeypatch):
     return set_
 
 
-def test_paths_and_graph_replays_are_bit_identical(ctx, force):
+@pytest.mark.parametrize("mixed", [False, True], ids=["fp64", "mixed"])
+def test_paths_and_graph_replays_are_bit_identical(ctx, force, mixed):
     """The same batch through the persistent kernel and the level path
     (first direct launches, then the capture on the composition's second
     fit, then replays), with other device work and synchronisations between
-    the fits: every replay equals the eager result bit for bit."""
+    the fits: every replay equals the eager result bit for bit. Mixed
+    precision: the level path spreads alpha's refinement over launches (the
+    residual per 64-row block), the persistent kernel runs it in one
+    workgroup per GP; the refinement steps taken agree too."""
     from gprat_amd import capi
     probs = _problems(24, seed=21, n_lo=64, n_hi=1500)
     fits = [(x, y, n, r, t) for x, y, n, r, t, _ in probs]
     picks = (0, 7, 23)
+    precision = capi.PRECISION_MIXED if mixed else capi.PRECISION_FP64
 
     def run():
-        b = capi.Batch(ctx, fits)
+        b = capi.Batch(ctx, fits, precision=precision)
         try:
-            return b.losses().copy(), [b.predict(g, probs[g][5], 50) for g in picks]
+            return b.losses().copy(), b.refinement().copy(), [b.predict(g, probs[g][5], 50) for g in picks]
         finally:
             b.release()
 
-    force(path="persistent")
-    ref_loss, ref_pred = run()
-    force(path="level")
+    force(path="persistent", mixed="requested" if mixed else None)
+    ref_loss, ref_steps, ref_pred = run()
+    if mixed:
+        assert np.all(ref_steps >= 1)
+    force(path="level", mixed="requested" if mixed else None)
     x, y, n, r, t, z = probs[3]
     for rep in range(4):
-        loss, pred = run()
+        loss, steps, pred = run()
         assert np.array_equal(loss, ref_loss), rep
+        assert np.array_equal(steps, ref_steps), rep
         for (m1, v1), (m2, v2) in zip(pred, ref_pred):
             assert np.array_equal(m1, m2) and np.array_equal(v1, v2), rep
         # eager work of the program between the fits
