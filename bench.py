@@ -94,6 +94,34 @@ def residual_flops(n_pad: int, m_pad: int) -> float:
     return 2.0 * m_pad * 128 * 128 * T * (T - 1) / 2.0
 
 
+def mixed_request_flops(n_pad: int, m_pad: int) -> dict:
+    """Algorithmic flops of one mixed-precision request (resident fit, then a
+    prediction with the refined variance) by the precision they run in. The
+    Cholesky's panel of width w at p0 is factored and solved for the rows
+    below it in FP64 (w^3/3 + (n - p0 - w) w^2); the rest of n^3/3, the
+    trailing update, runs on FP32 matrix cores. The variance's forward and
+    backward solves are n^2 m each on FP32 matrix cores, its residual
+    product (the strictly lower 128-row blocks of K times X) FP64. The
+    O(n^2) terms (alpha, its refinement, the mean) are left out."""
+    bo = outer_block(n_pad)
+    panels = 0.0
+    p0 = 0
+    while p0 < n_pad:
+        w = min(bo, n_pad - p0)
+        panels += w ** 3 / 3.0 + (n_pad - p0 - w) * float(w) * w
+        p0 += w
+    chol = n_pad ** 3 / 3.0
+    return {
+        "fp64": panels + residual_flops(n_pad, m_pad),
+        "fp32": (chol - panels) + 2.0 * n_pad ** 2 * m_pad,
+        "fp64_cholesky_panels": panels,
+        "fp32_cholesky_trailing": chol - panels,
+        "fp32_variance_solves": 2.0 * n_pad ** 2 * m_pad,
+        "fp64_variance_residual": residual_flops(n_pad, m_pad),
+        "panel_width": bo,
+    }
+
+
 def variance_update_flops(n_pad: int, m_pad: int, nbk: int, directions: int):
     """Algorithmic flops of the variance solves' update products over one
     request: step J of the forward solve updates the n_pad - j1 rows below
@@ -144,7 +172,7 @@ def host_description():
     return os.cpu_count() or 1, model
 
 
-def cpu_baseline(repeats=3):
+def cpu_baseline(repeats=5):
     """The oracle on a bounded sample of the headline workload: fit +
     predict_with_uncertainty at n = 16384 (16 tiles of 1024, n_reg 16, noise
     0.05) on 1024 test points, one warm-up and `repeats` timed runs, median.
@@ -694,6 +722,7 @@ def main():
     head["alpha_last_correction"] = last
     head["variance_max_correction"] = vcorr
     head["phase_ms_per_request"] = {k: v[1] / args.steps for k, v in fams.items()}
+    head["flops_by_precision"] = mixed_request_flops(n_pad, m_pad)
 
     result = {
         "metric": METRIC,
