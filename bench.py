@@ -403,6 +403,12 @@ def optimize_leg(repeats=1):
                               "theta": float(np.max(np.abs(th - ref["opt_theta"]) / np.abs(ref["opt_theta"])))},
         "flops_per_iteration_this_algorithm": n ** 3 / 3.0 * 3,
         "flops_per_iteration_reference_algorithm": n ** 3 / 3.0 + 2.0 * n ** 3,
+        # whole iterations (host Adam and transfers included) against the
+        # FP64 matrix peak: the executed algorithm's flops, and for
+        # comparison the reference algorithm's at the same time
+        "tflops_this_algorithm": n ** 3 / 3.0 * 3 / (dt * 1e-1) / 1e12,
+        "frac_of_fp64_peak_this_algorithm": n ** 3 / 3.0 * 3 / (dt * 1e-1) / 1e12 / FP64_MFMA_PEAK_TFLOPS,
+        "tflops_reference_algorithm_equivalent": (n ** 3 / 3.0 + 2.0 * n ** 3) / (dt * 1e-1) / 1e12,
         "workload": f"optimize: n={n} ({T} tiles of {b}), n_reg={r}, 10 Adam iterations at lr 0.1 (K^-1 = W^T W per iteration)",
     }
 
