@@ -316,12 +316,16 @@ typedef struct gprat_gp_desc
 
 typedef struct gprat_batch gprat_batch;
 
-/* Fit `count` independent GPs of any sizes in one persistent launch
- * (assembly, Cholesky, alpha and loss per GP); factors stay resident.
- * GPRAT_PRECISION_MIXED: the block products of each factorisation on FP32
- * matrix cores (diagonal blocks, storage and solves in FP64) and alpha
- * refined in FP64 against K regenerated from the series, inside the same
- * launch. No counterpart in the reference, which fits one GP per call. */
+/* Fit `count` independent GPs of any sizes in one call (assembly, Cholesky,
+ * alpha and loss per GP); factors stay resident. Per size class, from the
+ * tuning table: a persistent kernel takes small GPs whole, and
+ * level-synchronous launches over the 64x64 blocks of all large GPs take
+ * the rest (replayed from a captured HIP graph when a batch composition
+ * repeats on the context). GPRAT_PRECISION_MIXED, for the classes where
+ * the table measured it faster: the block products of each factorisation
+ * on FP32 matrix cores (diagonal blocks, storage and solves in FP64) and
+ * alpha refined in FP64 against K regenerated from the series. No
+ * counterpart in the reference, which fits one GP per call. */
 int gprat_batch_fit(gprat_context *ctx, int count, const gprat_gp_desc *gps, int precision, gprat_batch **out);
 /* losses[g] = loss of GP g at its hyperparameters (count values). */
 int gprat_batch_losses(gprat_batch *batch, double *losses);
