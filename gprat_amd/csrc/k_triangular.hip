@@ -267,8 +267,11 @@ __global__ __launch_bounds__(THREADS) void k_panel_tasks(PanelArgs a)
 // blocks), into X (row-major, ld ldx). Column block c costs c + 1 block
 // products, so workgroup (p, i) computes column blocks p and nbw - 1 - p of
 // row block i: every workgroup does nbw + 1 products (one extra for the
-// middle block of an odd nbw). The next pair of blocks is loaded into
-// registers while the current pair is multiplied.
+// middle block of an odd nbw). Each product is staged as two 32-deep
+// halves (35 KB of LDS, so four workgroups share a CU and hide each
+// other's staging; the MFMAs run in the whole product's order, so the
+// result is the same), and the next half is loaded into registers while
+// the current one is multiplied.
 __global__ __launch_bounds__(THREADS) void k_rows_solve(const double *__restrict__ A,
                                                          long lda,
                                                          long r0,
@@ -279,32 +282,37 @@ __global__ __launch_bounds__(THREADS) void k_rows_solve(const double *__restrict
                                                          double *__restrict__ X,
                                                          long ldx)
 {
-    __shared__ double a[NB * blk::XS];
-    __shared__ double b[NB * blk::XS];
+    __shared__ double a[NB * blk::XH];
+    __shared__ double b[NB * blk::XH];
     const int pair = blockIdx.x, i = blockIdx.y;
     const double *row = A + (r0 + static_cast<long>(NB) * i) * lda + p0;
-    for (int half = 0; half < 2; half++)
+    for (int side = 0; side < 2; side++)
     {
-        const int c = half == 0 ? pair : nbw - 1 - pair;
-        if (half == 1 && c <= pair)
+        const int c = side == 0 ? pair : nbw - 1 - pair;
+        if (side == 1 && c <= pair)
             break;
         blk::v4d acc[2][2];
         blk::zero(acc);
-        blk::d2 va[8], vb[8];
-        blk::load_block(row, lda, va);
-        blk::load_block(Wb + static_cast<long>(NB) * c, ldw, vb);  // t = 0: rows k of W(c, 0)^T
-        for (int t = 0; t <= c; t++)
+        // half-step s covers k = 32 s .. 32 s + 31 of the product's sum:
+        // A's columns there, and W(c, .)'s k-major rows
+        const double *wc = Wb + static_cast<long>(NB) * c;
+        const int steps = 2 * (c + 1);
+        blk::d2 va[4], vb[4];
+        blk::load_half(row, lda, va);
+        blk::load_half_t(wc, ldw, vb);
+        for (int st = 0; st < steps; st++)
         {
-            __syncthreads();  // the previous product's LDS reads are done
-            blk::store_block(a, va);    // [r][k] = A(r, k)
-            blk::store_block_t(b, vb);  // [cc][k] = W(c, t)(cc, k)
+            __syncthreads();  // the previous half's LDS reads are done
+            blk::store_half(a, va);    // [r][k] = A(r, k)
+            blk::store_half_t(b, vb);  // [cc][k] = W(c, t)(cc, k)
             __syncthreads();
-            if (t < c)
+            if (st + 1 < steps)
             {
-                blk::load_block(row + NB * (t + 1), lda, va);
-                blk::load_block(Wb + static_cast<long>(NB) * (t + 1) * ldw + NB * c, ldw, vb);
+                blk::load_half(row + 32 * (st + 1), lda, va);
+                blk::load_half_t(wc + static_cast<long>(32) * (st + 1) * ldw, ldw, vb);
             }
-            blk::mfma_nt_acc(a, b, acc);
+            __builtin_amdgcn_sched_barrier(0);
+            blk::mfma_nt_acc_half(a, b, acc);
         }
         blk::store_acc(X + static_cast<long>(NB) * i * ldx + NB * c, ldx, acc, true, false);
     }
