@@ -322,8 +322,8 @@ __global__ __launch_bounds__(THREADS) void k_rows_solve(const double *__restrict
 // (k-major: W(r, c) at Wk[c * ldw + r], nbr 64-blocks), B and V row-major.
 // V(i, j) = sum_{t <= i} W(i, t) B(t, j) over 64x64 blocks: row block i
 // costs i + 1 block products, so workgroup (j, p) computes row blocks p and
-// nbr - 1 - p of column block j, and loads the next pair of blocks into
-// registers while the current pair is multiplied.
+// nbr - 1 - p of column block j. Products on 32-deep half stages, the next
+// half loaded while the current one is multiplied (as k_rows_solve).
 __global__ __launch_bounds__(THREADS) void k_leaf_solve(const double *__restrict__ Wk,
                                                          long ldw,
                                                          int nbr,
@@ -332,31 +332,37 @@ __global__ __launch_bounds__(THREADS) void k_leaf_solve(const double *__restrict
                                                          double *__restrict__ V,
                                                          long ldv)
 {
-    __shared__ double a[NB * blk::XS];
-    __shared__ double b[NB * blk::XS];
+    __shared__ double a[NB * blk::XH];
+    __shared__ double b[NB * blk::XH];
     const int j = blockIdx.x, pair = blockIdx.y;
-    for (int half = 0; half < 2; half++)
+    for (int side = 0; side < 2; side++)
     {
-        const int i = half == 0 ? pair : nbr - 1 - pair;
-        if (half == 1 && i <= pair)
+        const int i = side == 0 ? pair : nbr - 1 - pair;
+        if (side == 1 && i <= pair)
             break;
         blk::v4d acc[2][2];
         blk::zero(acc);
-        blk::d2 va[8], vb[8];
-        blk::load_block(Wk + static_cast<long>(NB) * i, ldw, va);  // t = 0: rows k of W(i, 0)
-        blk::load_block(B + static_cast<long>(NB) * j, ldb, vb);   // rows k of B(0, j)
-        for (int t = 0; t <= i; t++)
+        // half-step s covers k = 32 s .. 32 s + 31: those k-major rows of
+        // W(i, .) and rows of B(., j)
+        const double *wi = Wk + static_cast<long>(NB) * i;
+        const double *bj = B + static_cast<long>(NB) * j;
+        const int steps = 2 * (i + 1);
+        blk::d2 va[4], vb[4];
+        blk::load_half_t(wi, ldw, va);
+        blk::load_half_t(bj, ldb, vb);
+        for (int st = 0; st < steps; st++)
         {
-            __syncthreads();              // the previous product's LDS reads are done
-            blk::store_block_t(a, va);    // [r][k] = W(i, t)(r, k)
-            blk::store_block_t(b, vb);    // [cc][k] = B(t, j)(k, cc)
+            __syncthreads();            // the previous half's LDS reads are done
+            blk::store_half_t(a, va);   // [r][k] = W(i, t)(r, k)
+            blk::store_half_t(b, vb);   // [cc][k] = B(t, j)(k, cc)
             __syncthreads();
-            if (t < i)
+            if (st + 1 < steps)
             {
-                blk::load_block(Wk + static_cast<long>(NB) * (t + 1) * ldw + NB * i, ldw, va);
-                blk::load_block(B + static_cast<long>(NB) * (t + 1) * ldb + NB * j, ldb, vb);
+                blk::load_half_t(wi + static_cast<long>(32) * (st + 1) * ldw, ldw, va);
+                blk::load_half_t(bj + static_cast<long>(32) * (st + 1) * ldb, ldb, vb);
             }
-            blk::mfma_nt_acc(a, b, acc);
+            __builtin_amdgcn_sched_barrier(0);
+            blk::mfma_nt_acc_half(a, b, acc);
         }
         blk::store_acc(V + static_cast<long>(NB) * i * ldv + NB * j, ldv, acc, true, false);
     }
