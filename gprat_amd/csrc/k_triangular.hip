@@ -268,11 +268,14 @@ __global__ __launch_bounds__(THREADS) void k_panel_tasks(PanelArgs a)
 // products, so workgroup (p, i) computes column blocks p and nbw - 1 - p of
 // row block i: every workgroup does nbw + 1 products (one extra for the
 // middle block of an odd nbw). Each product is staged as two 32-deep
-// halves (35 KB of LDS, so four workgroups share a CU and hide each
-// other's staging; the MFMAs run in the whole product's order, so the
+// halves (35 KB of LDS; with the register budget held to four wavefronts
+// per SIMD, a few values spill, four workgroups share a CU and hide each
+// other's staging: on MI355X whole blocks ran two per CU, halves with the
+// compiler's budget three, this four, the n = 32768 factorisation 204 ->
+// 202.5 -> 200.7 ms; the MFMAs run in the whole product's order, so the
 // result is the same), and the next half is loaded into registers while
 // the current one is multiplied.
-__global__ __launch_bounds__(THREADS) void k_rows_solve(const double *__restrict__ A,
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_rows_solve(const double *__restrict__ A,
                                                          long lda,
                                                          long r0,
                                                          long p0,
@@ -324,7 +327,7 @@ __global__ __launch_bounds__(THREADS) void k_rows_solve(const double *__restrict
 // costs i + 1 block products, so workgroup (j, p) computes row blocks p and
 // nbr - 1 - p of column block j. Products on 32-deep half stages, the next
 // half loaded while the current one is multiplied (as k_rows_solve).
-__global__ __launch_bounds__(THREADS) void k_leaf_solve(const double *__restrict__ Wk,
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_leaf_solve(const double *__restrict__ Wk,
                                                          long ldw,
                                                          int nbr,
                                                          const double *__restrict__ B,
