@@ -962,6 +962,66 @@ __device__ __forceinline__ void off_block_halves(const BatchGP &g, int ib, int j
     blk::store_acc(g.L + static_cast<long>(ib) * NB * np + jb * NB, np, xb, true, false);
 }
 
+// The mixed-precision level path's off-diagonal block (ib, jb) on 35 KB of
+// LDS instead of 66: the generated block of K, then the FP32 block pairs,
+// then the FP64 product with W_jj in 32-deep halves, each in turn in the
+// same space (three workgroups per CU instead of two). The operations and
+// their order are off_block<true>'s, so the results are the same bits.
+union MixedHalfShared
+{
+    struct
+    {
+        double a[NB * XH];
+        double b[NB * XH];
+    } h;
+    double kb[NB * XS];  // the generated block of K (stride XS)
+    Stage32 st32;
+};
+
+__device__ __forceinline__ void off_block_mixed_halves(const BatchGP &g, int ib, int jb, MixedHalfShared &sh)
+{
+    const long np = g.n_pad;
+    double kij[16];
+    gen_block(g, ib, jb, sh.kb);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+    {
+        int r, c;
+        acc_position<true>(q, r, c);
+        kij[q] = sh.kb[r * XS + c];
+    }
+    __syncthreads();  // every entry is read before the space is reused
+    v4f acc[2][2];
+    accumulate_row_products32(g.L, np, ib, jb, jb, sh.st32, acc);  // ends with a barrier
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+        kij[q] -= static_cast<double>(acc[q >> 3][(q >> 2) & 1][q & 3]);
+    // L_ij = (K_ij - acc) W_jj^T, the 64-deep product as two halves of C's
+    // and W_jj's columns
+    v4d xb[2][2];
+    blk::zero(xb);
+    const double *W = g.W + static_cast<long>(jb) * NB * NB;
+    for (int h = 0; h < 2; h++)
+    {
+        d2 rw[4];
+        blk::load_half(W + h * 32, NB, rw);
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+        {
+            int r, c;
+            acc_position<true>(q, r, c);
+            if ((c >> 5) == h)
+                sh.h.a[r * XH + (c & 31)] = kij[q];
+        }
+        blk::store_half(sh.h.b, rw);
+        __syncthreads();
+        blk::mfma_nt_acc_half(sh.h.a, sh.h.b, xb);
+        __syncthreads();
+    }
+    blk::store_acc(g.L + static_cast<long>(ib) * NB * np + jb * NB, np, xb, true, false);
+}
+
 // The first jn - 1 k-blocks of diagonal block jn's reduction (FP64), run in
 // the off-diagonal launch of column jn - 1 beside that column's blocks, so
 // the diagonal launch of column jn only adds block jn - 1 before it factors.
@@ -1024,14 +1084,14 @@ __global__ __launch_bounds__(THREADS) void k_batch_diag_level(const BatchGP *__r
 }
 
 template <bool MIXED>
-__global__ __launch_bounds__(THREADS) void k_batch_off_level(const BatchGP *__restrict__ gps, const int2 *__restrict__ items, int jb)
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_batch_off_level(const BatchGP *__restrict__ gps, const int2 *__restrict__ items, int jb)
 {
     const int2 it = items[blockIdx.x];
     const BatchGP g = gps[it.x];
     if constexpr (MIXED)
     {
-        __shared__ Shared sh;
-        off_block<true>(g, it.y, jb, sh, nullptr);
+        __shared__ MixedHalfShared sh;
+        off_block_mixed_halves(g, it.y, jb, sh);
     }
     else
     {
