@@ -675,7 +675,7 @@ def main():
                     help="run one leg alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this backend runs on one GPU; the GP path has no multi-GPU mode in this build "
+        print("bench.py: this backend runs on one GPU; the GP path has no multi-GPU mode "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 

@@ -276,9 +276,9 @@ int gprat_model_predict_full_cov(gprat_model *model,
 int gprat_model_loss(gprat_model *model, double *loss_out);
 
 /* Loss and softplus-space gradient at the model's hyperparameters from its
- * own factor (the model_grad of the survey's ABI sketch; same meaning as
- * gprat_loss_gradient's outputs). For mixed-precision models K^-1 comes from
- * the mixed-precision factor and alpha from the refined solve. */
+ * own factor, without refitting (same meaning as gprat_loss_gradient's
+ * outputs). For mixed-precision models K^-1 comes from the mixed-precision
+ * factor and alpha from the refined solve. */
 int gprat_model_gradient(gprat_model *model, const int trainable[3], double *loss_out, double grad_out[3]);
 
 /* Mixed-precision models: refinement steps taken for alpha and the last
